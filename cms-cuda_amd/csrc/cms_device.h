@@ -1,0 +1,149 @@
+// Device-side helpers for the CMS / SC kernels (gfx950, wave64).
+//
+// Floating-point contract: every kernel is compiled with -ffp-contract=off,
+// so a*b + c is never fused, and each helper below performs its additions
+// in a fixed order:
+//   dot_seq : (a0*b0 + a1*b1) + a2*b2    (np.sum(axis=-1), linalg.norm, cdist)
+//   dot_alt : (a0*b0 + a2*b2) + a1*b1    (np.einsum('ij,ij->i'))
+// With correctly rounded +, -, *, / and sqrt this reproduces the NumPy
+// reference bit for bit; only the transcendental functions (sin, cos,
+// acos, atan2, exp) may differ in the last ulp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cms {
+
+constexpr int WAVE = 64;
+constexpr int MAX_SUBDIV = 100;
+constexpr int KCAP = 256;      // max neighbours per atom row
+constexpr int BCAP = 256;      // max buried (other-molecule) partners per atom
+constexpr int NEARCAP = 128;   // max near low probes per low probe
+constexpr int8_t ATTEN_BLOCKER = 1;
+constexpr int8_t ATTEN_2 = 2;
+constexpr int8_t ATTEN_BURIED = 5;
+constexpr double TWO_PI = 6.283185307179586;
+
+// status bits (per complex)
+constexpr int ST_COINCIDENT = 1;
+constexpr int ST_NBR_CAP = 2;
+constexpr int ST_BUR_CAP = 4;
+constexpr int ST_NEAR_CAP = 8;
+constexpr int ST_POOL = 16;      // a device pool was too small: grow and rerun
+
+struct V3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator/(V3 a, double s) { return V3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V3 neg(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+
+__device__ __forceinline__ double dot_seq(V3 a, V3 b) {
+  double p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
+  return (p0 + p1) + p2;
+}
+__device__ __forceinline__ double dot_alt(V3 a, V3 b) {
+  double p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
+  return (p0 + p2) + p1;
+}
+__device__ __forceinline__ double norm_seq(V3 a) { return sqrt(dot_seq(a, a)); }
+// a x b with NumPy's component formulas
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 unit(V3 a) { return a / norm_seq(a); }
+
+__device__ __forceinline__ V3 ld3(const double4& a) { return V3{a.x, a.y, a.z}; }
+
+// ---------------------------------------------------------------- wave ops
+__device__ __forceinline__ int lane() { return __lane_id(); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ int wave_sum_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+// inclusive prefix sum over the wave
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  int l = lane();
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int t = __shfl_up(v, o, WAVE);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_min_double(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// ------------------------------------------------------- arc subdivision
+// Number of points a_i = delta*(i+1/2) <= angle with i < MAX_SUBDIV, where
+// delta = 1/(sqrt(density)*rad).  NaN angle or infinite delta give 0.
+__device__ __forceinline__ int subdiv_count(double delta, double angle) {
+  double raw = angle / delta;
+  if (!(raw > 0.0) || !(raw < 1e300)) return 0;
+  int n = 0;
+  int lim = (int)fmin((double)MAX_SUBDIV, ceil(raw) + 1.0);
+  for (int i = 0; i < lim; ++i) {
+    double a = delta * ((double)i + 0.5);
+    if (a <= angle) n = i + 1; else break;
+  }
+  return n;
+}
+
+// Circle point i: cen + x*c + y*s with c = rad*cos(a), s = rad*sin(a)
+__device__ __forceinline__ V3 subdiv_point(V3 cen, V3 x, V3 y, double rad, double delta, int i) {
+  double a = delta * ((double)i + 0.5);
+  double c = rad * cos(a);
+  double s = rad * sin(a);
+  return (cen + x * c) + y * s;
+}
+
+// Angle from x to v about axis in [0, 2pi), NaN if v is (numerically)
+// perpendicular to the plane.  Also returns y = axis x x.
+__device__ __forceinline__ double arc_angle(V3 x, V3 v, V3 axis, V3* y_out) {
+  V3 y = cross(axis, x);
+  double dt1 = dot_seq(v, x);
+  double dt2 = dot_seq(v, y);
+  double ang = atan2(dt2, dt1);
+  if (ang < 0.0) ang = ang + TWO_PI;
+  if (fabs(dt1) <= 1e-8 && fabs(dt2) <= 1e-8) ang = __builtin_nan("");
+  *y_out = y;
+  return ang;
+}
+
+// In-plane basis of the circle about axis.
+__device__ __forceinline__ void circle_frame(V3 axis, V3* xo, V3* yo) {
+  axis = unit(axis);
+  V3 v1 = v3(axis.y * axis.y + axis.z * axis.z, axis.x * axis.x + axis.z * axis.z,
+             axis.x * axis.x + axis.y * axis.y);
+  v1 = unit(v1);
+  double dt = dot_seq(v1, axis);
+  if (fabs(dt) > 0.99) v1 = v3(1.0, 0.0, 0.0);
+  V3 v2 = unit(cross(axis, v1));
+  V3 x = unit(cross(axis, v2));
+  *xo = x;
+  *yo = cross(axis, x);
+}
+
+__device__ __forceinline__ double dist_to_line(V3 cen, V3 axis, V3 p) {
+  V3 v = p - cen;
+  double dt = dot_seq(v, axis);
+  double d2 = dot_seq(v, v) - dt * dt;
+  return d2 < 0.0 ? 0.0 : sqrt(d2);
+}
+
+// grid-stride helpers: every kernel reads its unit count from device memory
+#define CMS_UNIT_LOOP(u, n) for (int64_t u = blockIdx.x; u < (int64_t)(n); u += gridDim.x)
+
+}  // namespace cms

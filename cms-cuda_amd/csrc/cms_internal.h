@@ -1,0 +1,103 @@
+// Internal data layout shared by the kernels and the host pipeline.
+//
+// HBM layout (one batch of B complexes, G = 2B molecule groups, group
+// g = 2*complex + molecule):
+//   atoms   : double4 (x, y, z, radius) per atom, complex-major, binder
+//             (molecule 0) atoms before target (molecule 1) atoms
+//   region  : ratom[nR] = atoms that need neighbour rows (ascending, hence
+//             grouped by g), rpos[atom] = its row index
+//   rows    : nbr[nR][KCAP], bur[nR][BCAP] global atom indices
+//   pairs   : pi/pj int32, pu double4 (u, rij), pt double4 (t, -), pflag
+//   probes  : patom int4 (a0, a1, a2, -), ppos double4 (p, height), palt
+//             double4 (alt, -), grouped by g, "+" probes before "-" probes
+//   dots    : dpos double4 (x, y, z, area), dnml double4 (normal, -),
+//             dbur/dtype int8, datom int32, grouped by g, each group in
+//             emission order convex | toroidal atom-1 arcs | toroidal
+//             atom-2 arcs | concave
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cms {
+
+struct Settings {
+  double rp, band, sep, weight, binwidth_dist, binwidth_norm;
+};
+
+struct BatchView {
+  const double4* atom;   // ntot
+  const double* dens;    // ntot
+  const int32_t* acplx;  // ntot complex id
+  const int8_t* amol;    // ntot molecule (0 binder, 1 target)
+  const int32_t* aoff;   // B+1 atom offsets
+  int ntot;
+  int B;
+};
+
+struct Work {
+  // per atom
+  int8_t* atten;
+  int8_t* access;
+  int* status;            // B
+  // region
+  const int* nR;          // device pointer to the region size
+  int* ratom;
+  int* rpos;              // exclusive scan of the region flag (ntot + 1)
+  int* nn;
+  int* nbr;
+  int* nb;
+  int* bur;
+  // pairs
+  int* pcnt;
+  int* pscan;
+  const int* nP;
+  int* pi;
+  int* pj;
+  double4* pu;
+  double4* pt;
+  int8_t* pflag;          // bit0 between, bit1 toroid queue
+  int* pgrp;              // G+1 pair offsets per group
+  // probes
+  int* prcnt;             // 2*nP, virtual (group, sign, pair) order
+  int* prscan;
+  const int* nQ;
+  int4* patom;
+  double4* ppos;
+  double4* palt;
+  int* prgrp;             // G+1 probe offsets per group
+  int* near;
+  int* nnear;
+  // dot counts
+  int* ccnt;
+  int* cscan;
+  int* t1cnt;
+  int* t1scan;
+  int* t2cnt;
+  int* t2scan;
+  int8_t* tflag;
+  int* kcnt;
+  int* kscan;
+  int64_t* dbase;         // G*4 stage bases
+  // dots
+  double4* dpos;
+  double4* dnml;
+  int8_t* dbur;
+  int8_t* dtype;
+  int* datom;
+  int64_t dot_cap;
+};
+
+template <bool EMIT> __global__ void k_pairs(BatchView, Settings, Work);
+template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
+template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
+template <bool EMIT> __global__ void k_toroid(BatchView, Settings, Work);
+template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);
+__global__ void k_attention(BatchView, Settings, int, int8_t*, int8_t*, unsigned long long*, int*);
+__global__ void k_flag_nonblocker(int, const int8_t*, int*);
+__global__ void k_region(BatchView, Settings, int, const int8_t*, const int*, const int*, int*);
+__global__ void k_compact_index(int, const int*, const int*, int*);
+__global__ void k_neighbors(BatchView, Settings, Work);
+__global__ void k_near(BatchView, Settings, Work);
+
+}  // namespace cms

@@ -1,0 +1,465 @@
+// Readouts: buried / accessible / trimmed dot lists, brute-force nearest
+// dot search, CMS weights and per-atom sums in dot order, NumPy-order
+// pairwise totals, SC trim and side statistics with the binned median.
+#include "cms_device.h"
+#include "cms_internal.h"
+#include "cms_readout.h"
+
+namespace cms {
+
+// ------------------------------------------------------------ lists
+__global__ void k_dot_flag(int n, const int8_t* __restrict__ bur, int want, int* __restrict__ flag) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flag[i] = (bur[i] != 0) == (want != 0) ? 1 : 0;
+}
+
+// list[scan[i]] = i for flagged i; also gathers the dot position record
+__global__ void k_dot_compact(int n, const int* __restrict__ flag, const int* __restrict__ scan,
+                              const double4* __restrict__ dpos, int* __restrict__ list,
+                              double4* __restrict__ xyz) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (flag[i]) {
+      const int o = scan[i];
+      list[o] = i;
+      xyz[o] = dpos[i];
+    }
+}
+
+// ------------------------------------------------------------ search
+__global__ void k_tile_counts(int G, const int* __restrict__ qgrp, int tq, int* __restrict__ cnt) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
+    cnt[g] = (qgrp[g + 1] - qgrp[g] + tq - 1) / tq;
+}
+
+// Nearest reference point of every query, FP64.  Query group g searches the
+// reference group g ^ xmask.  d^2 = (dx^2 + dy^2) + dz^2 as in
+// scipy cdist(sqeuclidean); ties keep the first (lowest) reference index,
+// as np.argmin does.  Reference points are staged in LDS, 256 at a time.
+constexpr int NT = 256;
+template <bool IDX>
+__global__ __launch_bounds__(NT) void k_nearest64(const double4* __restrict__ Q, const int* __restrict__ qgrp,
+                                                  const double4* __restrict__ R, const int* __restrict__ rgrp,
+                                                  int xmask, const int* __restrict__ toff, int G,
+                                                  double* __restrict__ out_d2, int* __restrict__ out_idx) {
+  __shared__ double rx[NT], ry[NT], rz[NT];
+  const int ntiles = toff[G];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int lo = 0, hi = G;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (toff[mid] <= tile) lo = mid; else hi = mid;
+    }
+    const int g = lo;
+    const int q = qgrp[g] + (tile - toff[g]) * NT + threadIdx.x;
+    const bool live = q < qgrp[g + 1];
+    const int h = g ^ xmask;
+    const int r0 = rgrp[h], r1 = rgrp[h + 1];
+    double qx = 0.0, qy = 0.0, qz = 0.0;
+    if (live) {
+      const double4 a = Q[q];
+      qx = a.x; qy = a.y; qz = a.z;
+    }
+    double best = INFINITY;
+    int bi = -1;
+    for (int rb = r0; rb < r1; rb += NT) {
+      __syncthreads();
+      const int rr = rb + threadIdx.x;
+      if (rr < r1) {
+        const double4 a = R[rr];
+        rx[threadIdx.x] = a.x; ry[threadIdx.x] = a.y; rz[threadIdx.x] = a.z;
+      }
+      __syncthreads();
+      const int n = min(NT, r1 - rb);
+      for (int t = 0; t < n; ++t) {
+        const double dx = rx[t] - qx, dy = ry[t] - qy, dz = rz[t] - qz;
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        if (d2 < best) { best = d2; bi = rb - r0 + t; }
+      }
+    }
+    if (live) {
+      out_d2[q] = best;
+      if (IDX) out_idx[q] = bi;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ CMS weights
+// per buried dot: area * exp(-d^2 * weight)   (_core.py:995)
+__global__ void k_cms_weights(int n, const double4* __restrict__ bxyz, const double* __restrict__ d2,
+                              double weight, double* __restrict__ w) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    w[i] = bxyz[i].w * exp(-d2[i] * weight);
+}
+
+// ------------------------------------------------------------ per-atom sums
+// np.bincount(owner, weights) accumulates item values per atom in item
+// order starting from 0.0.  Counting sort by owner (integer atomics), then
+// each atom sorts its own (short) segment by item index and sums in order,
+// so the result is deterministic and bit-identical to bincount.
+__global__ void k_owner_count(int n, const int* __restrict__ list, const int* __restrict__ datom,
+                              int* __restrict__ cnt) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&cnt[datom[list ? list[i] : i]], 1);
+}
+__global__ void k_owner_scatter(int n, const int* __restrict__ list, const int* __restrict__ datom,
+                                const int* __restrict__ start, int* __restrict__ cur,
+                                int* __restrict__ perm) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int a = datom[list ? list[i] : i];
+    perm[start[a] + atomicAdd(&cur[a], 1)] = i;
+  }
+}
+__global__ void k_owner_sums(int natoms, const int* __restrict__ start, int* __restrict__ perm,
+                             const double* __restrict__ val, int64_t vstride, double* __restrict__ out) {
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < natoms; a += gridDim.x * blockDim.x) {
+    const int lo = start[a], hi = start[a + 1];
+    for (int i = lo + 1; i < hi; ++i) {
+      const int v = perm[i];
+      int j = i - 1;
+      while (j >= lo && perm[j] > v) {
+        perm[j + 1] = perm[j];
+        --j;
+      }
+      perm[j + 1] = v;
+    }
+    double s = 0.0;
+    for (int i = lo; i < hi; ++i) s += val[(int64_t)perm[i] * vstride];
+    out[a] = s;
+  }
+}
+
+// ------------------------------------------------------------ pairwise sums
+// np.add.reduce on a contiguous float64 array: blocks of <= 128 values are
+// summed with 8 interleaved accumulators, longer ranges split at
+// n/2 rounded down to a multiple of 8 and the halves added.
+__device__ double np_leaf_sum(const double* a, int64_t stride, int n) {
+  if (n < 8) {
+    double res = 0.0;
+    for (int i = 0; i < n; ++i) res += a[i * stride];
+    return res;
+  }
+  double r[8];
+  for (int j = 0; j < 8; ++j) r[j] = a[j * stride];
+  int i = 8;
+  for (; i < n - (n % 8); i += 8)
+    for (int j = 0; j < 8; ++j) r[j] += a[(int64_t)(i + j) * stride];
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res += a[(int64_t)i * stride];
+  return res;
+}
+
+constexpr int PW_MAXTOK = 2048;
+
+// Block-wide np.sum of a[0..n) (stride in doubles).  Thread 0 lays out the
+// pairwise tree as postfix tokens (leaf / add), all threads sum the leaves,
+// thread 0 evaluates the postfix.  Returns the sum in every thread.
+__device__ double block_np_sum(const double* a, int64_t stride, int n) {
+  __shared__ int tok_lo[PW_MAXTOK];
+  __shared__ int tok_n[PW_MAXTOK];   // > 0 leaf of that length, 0 = add
+  __shared__ double leaf[PW_MAXTOK];
+  __shared__ int ntok, nleaf, overflow;
+  __shared__ double result;
+  if (threadIdx.x == 0) {
+    int stk_n[64], stk_lo[64], stk_v[64];
+    int sp = 0, nt = 0, nl = 0;
+    int of = 0;
+    stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
+    while (sp >= 0 && !of) {
+      const int m = stk_n[sp], lo = stk_lo[sp], v = stk_v[sp];
+      --sp;
+      if (nt >= PW_MAXTOK) { of = 1; break; }
+      if (m <= 128) {
+        tok_lo[nt] = lo; tok_n[nt] = m > 0 ? m : -1; ++nt; ++nl;
+      } else if (!v) {
+        int n2 = m / 2;
+        n2 -= n2 % 8;
+        ++sp; stk_n[sp] = m; stk_lo[sp] = lo; stk_v[sp] = 1;
+        ++sp; stk_n[sp] = m - n2; stk_lo[sp] = lo + n2; stk_v[sp] = 0;
+        ++sp; stk_n[sp] = n2; stk_lo[sp] = lo; stk_v[sp] = 0;
+      } else {
+        tok_lo[nt] = 0; tok_n[nt] = 0; ++nt;
+      }
+    }
+    ntok = nt; nleaf = nl; overflow = of;
+  }
+  __syncthreads();
+  if (overflow) {
+    // very long arrays: a single thread follows the same tree recursively
+    if (threadIdx.x == 0) {
+      double vstk[64];
+      int stk_n[64], stk_lo[64], stk_v[64];
+      int sp = 0, vp = 0;
+      stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
+      while (sp >= 0) {
+        const int m = stk_n[sp], lo = stk_lo[sp], v = stk_v[sp];
+        --sp;
+        if (m <= 128) {
+          vstk[vp++] = np_leaf_sum(a + (int64_t)lo * stride, stride, m);
+        } else if (!v) {
+          int n2 = m / 2;
+          n2 -= n2 % 8;
+          ++sp; stk_n[sp] = m; stk_lo[sp] = lo; stk_v[sp] = 1;
+          ++sp; stk_n[sp] = m - n2; stk_lo[sp] = lo + n2; stk_v[sp] = 0;
+          ++sp; stk_n[sp] = n2; stk_lo[sp] = lo; stk_v[sp] = 0;
+        } else {
+          const double b2 = vstk[--vp];
+          const double b1 = vstk[--vp];
+          vstk[vp++] = b1 + b2;
+        }
+      }
+      result = vstk[0];
+    }
+    __syncthreads();
+    return result;
+  }
+  for (int t = threadIdx.x; t < ntok; t += blockDim.x) {
+    const int m = tok_n[t];
+    if (m != 0) leaf[t] = np_leaf_sum(a + (int64_t)tok_lo[t] * stride, stride, m > 0 ? m : 0);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double vstk[64];
+    int vp = 0;
+    for (int t = 0; t < ntok; ++t) {
+      if (tok_n[t] != 0) {
+        vstk[vp++] = leaf[t];
+      } else {
+        const double b2 = vstk[--vp];
+        const double b1 = vstk[--vp];
+        vstk[vp++] = b1 + b2;
+      }
+    }
+    result = vp > 0 ? vstk[0] : 0.0;
+  }
+  __syncthreads();
+  const double r = result;
+  __syncthreads();
+  return r;
+}
+
+// out[g] = np.sum(vals[start[g] .. start[g+1]))
+__global__ __launch_bounds__(256) void k_group_np_sum(int G, const int* __restrict__ start,
+                                                      const double* __restrict__ vals, int64_t stride,
+                                                      double* __restrict__ out) {
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    const int lo = start[g], hi = start[g + 1];
+    const double s = block_np_sum(vals + (int64_t)lo * stride, stride, hi - lo);
+    if (threadIdx.x == 0) out[g] = s;
+  }
+}
+
+// ------------------------------------------------------------ trim
+__global__ void k_trim_flag(int n, const double* __restrict__ d2acc, double band2, int* __restrict__ flag) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flag[i] = d2acc[i] > band2 ? 1 : 0;
+}
+__global__ void k_trim_compact(int n, const int* __restrict__ flag, const int* __restrict__ scan,
+                               const int* __restrict__ blist, const double4* __restrict__ dpos,
+                               const double4* __restrict__ dnml, int* __restrict__ tlist,
+                               double4* __restrict__ txyz, double4* __restrict__ tnml) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (flag[i]) {
+      const int o = scan[i];
+      const int d = blist[i];
+      tlist[o] = d;
+      txyz[o] = dpos[d];
+      tnml[o] = dnml[d];
+    }
+}
+
+// Per trimmed dot: distance to the nearest trimmed dot of the other side and
+// r = clip((n . n_other) * exp(-dist^2 * weight), +-0.999)  (_core.py:1134-1141)
+__global__ void k_sc_terms(int n, const int* __restrict__ tgrp_of, const int* __restrict__ tgrp,
+                           const double* __restrict__ d2, const int* __restrict__ idx,
+                           const double4* __restrict__ tnml, double weight,
+                           double* __restrict__ dist, double* __restrict__ rr) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (idx[i] < 0) {  // the other side has no trimmed dots: statistics are skipped
+      dist[i] = 0.0;
+      rr[i] = 0.0;
+      continue;
+    }
+    const int g = tgrp_of[i];
+    const int h = g ^ 1;
+    const int j = tgrp[h] + idx[i];
+    const double dm = sqrt(d2[i]);
+    const V3 a = ld3(tnml[i]), b = ld3(tnml[j]);
+    double r = dot_alt(a, b);
+    r = r * exp(-(dm * dm) * weight);
+    r = fmin(fmax(r, -0.999), 0.999);
+    dist[i] = dm;
+    rr[i] = r;
+  }
+}
+
+__global__ void k_group_of_items(int G, const int* __restrict__ start, int* __restrict__ gof) {
+  for (int g = blockIdx.x; g < G; g += gridDim.x)
+    for (int i = start[g] + threadIdx.x; i < start[g + 1]; i += blockDim.x) gof[i] = g;
+}
+
+// ------------------------------------------------------------ binned median
+// Grouped-data median of v[0..n) (_core.py:1080-1104): bins floor(v/bw),
+// first bin whose cumulative percentage reaches 50, linear interpolation.
+constexpr int HBINS = 4096;
+__device__ double block_binned_median(const double* v, int n, double bw) {
+  __shared__ long long smin, smax;
+  __shared__ int hist[HBINS];
+  __shared__ int wsum[4];
+  __shared__ long long found_bin;
+  __shared__ long long found_cum, found_cnt;
+  if (n == 0) return 0.0;
+  if (threadIdx.x == 0) { smin = LLONG_MAX; smax = LLONG_MIN; found_bin = LLONG_MAX; }
+  __syncthreads();
+  long long lmin = LLONG_MAX, lmax = LLONG_MIN;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const long long b = (long long)floor(v[i] / bw);
+    lmin = b < lmin ? b : lmin;
+    lmax = b > lmax ? b : lmax;
+  }
+  atomicMin(&smin, lmin);
+  atomicMax(&smax, lmax);
+  __syncthreads();
+  const long long bmin = smin, bmax = smax;
+  const double pct = 100.0 / (double)n;
+  if (bmax - bmin + 1 <= HBINS) {
+    const int nbin = (int)(bmax - bmin + 1);
+    for (int i = threadIdx.x; i < HBINS; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const long long b = (long long)floor(v[i] / bw);
+      atomicAdd(&hist[b - bmin], 1);
+    }
+    __syncthreads();
+    const int per = HBINS / 256;
+    const int b0 = threadIdx.x * per;
+    int loc = 0;
+    for (int k = 0; k < per; ++k) loc += hist[b0 + k];
+    const int inc = wave_incl_scan(loc);
+    const int wid = threadIdx.x / WAVE;
+    if (lane() == WAVE - 1) wsum[wid] = inc;
+    __syncthreads();
+    int cum = inc - loc;
+    for (int k = 0; k < wid; ++k) cum += wsum[k];
+    for (int k = 0; k < per; ++k) {
+      const int b = b0 + k;
+      const int c = hist[b];
+      if (c > 0 && b < nbin) {
+        cum += c;
+        if ((double)cum * pct >= 50.0) {
+          atomicMin(&found_bin, (long long)b);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long fb = found_bin;
+      if (fb == LLONG_MAX) fb = nbin - 1;  // no bin reaches 50%: last bin
+      long long cc = 0;
+      for (int b = 0; b <= fb; ++b) cc += hist[b];
+      found_bin = fb + bmin;
+      found_cum = cc;
+      found_cnt = hist[fb];
+    }
+    __syncthreads();
+  } else {
+    // wide range: binary search on the bin value with block-wide counts
+    __shared__ int bcount;
+    long long lo = bmin, hi = bmax;   // smallest b with cum(b)*pct >= 50
+    while (lo < hi) {
+      const long long mid = lo + (hi - lo) / 2;
+      if (threadIdx.x == 0) bcount = 0;
+      __syncthreads();
+      int c = 0;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) c += (long long)floor(v[i] / bw) <= mid;
+      atomicAdd(&bcount, c);
+      __syncthreads();
+      const int cum = bcount;
+      __syncthreads();
+      if ((double)cum * pct >= 50.0) hi = mid; else lo = mid + 1;
+    }
+    if (threadIdx.x == 0) { bcount = 0; found_cnt = 0; }
+    __syncthreads();
+    int c = 0, e = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const long long b = (long long)floor(v[i] / bw);
+      c += b <= lo;
+      e += b == lo;
+    }
+    atomicAdd(&bcount, c);
+    atomicAdd((unsigned long long*)&found_cnt, (unsigned long long)e);
+    __syncthreads();
+    if (threadIdx.x == 0) { found_bin = lo; found_cum = bcount; }
+    __syncthreads();
+  }
+  const double cumperc = (double)found_cum * pct;
+  const double binperc = (double)found_cnt * pct;
+  const double before = cumperc - binperc;
+  const double med = (double)found_bin * bw + (50.0 - before) * bw / binperc;
+  __syncthreads();
+  return med;
+}
+
+// Per molecule group (side): trimmed area, distance / normal statistics.
+// side[g*8 + k]: d_mean, d_median, s_mean, s_median, trimmed_area,
+// n_trimmed, n_dots, valid
+__global__ __launch_bounds__(256) void k_sc_sides(int G, const int* __restrict__ tgrp,
+                                                  const double4* __restrict__ txyz,
+                                                  const double* __restrict__ dist,
+                                                  const double* __restrict__ rr,
+                                                  const int* __restrict__ dgrp, Settings s,
+                                                  double* __restrict__ side) {
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    const int lo = tgrp[g], n = tgrp[g + 1] - lo;
+    const int h = g ^ 1;
+    const int no = tgrp[h + 1] - tgrp[h];
+    const double area = block_np_sum(&txyz[lo].w, 4, n);
+    double dmean = 0.0, dmed = 0.0, smean = 0.0, smed = 0.0;
+    if (n > 0 && no > 0) {
+      const double sd = block_np_sum(dist + lo, 1, n);
+      const double sr = block_np_sum(rr + lo, 1, n);
+      dmed = block_binned_median(dist + lo, n, s.binwidth_dist);
+      const double rmed = block_binned_median(rr + lo, n, s.binwidth_norm);
+      dmean = sd / (double)n;
+      smean = -(sr / (double)n);
+      smed = -rmed;
+    }
+    if (threadIdx.x == 0) {
+      double* o = side + 8 * (int64_t)g;
+      o[0] = dmean; o[1] = dmed; o[2] = smean; o[3] = smed;
+      o[4] = area; o[5] = (double)n; o[6] = (double)(dgrp[g + 1] - dgrp[g]); o[7] = 1.0;
+    }
+    __syncthreads();
+  }
+}
+
+// (_core.py:849-861) averages of the two sides per complex.
+// res[c*CMS_SC_NRES + k]: sc, area, distance, d_mean, s_mean, then the two
+// sides' 8 values at 8 and 16.
+__global__ void k_sc_combine(int B, const double* __restrict__ side, double* __restrict__ res) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < B; c += gridDim.x * blockDim.x) {
+    const double* s0 = side + 16 * (int64_t)c;
+    const double* s1 = s0 + 8;
+    double* o = res + 24 * (int64_t)c;
+    o[0] = (s0[3] + s1[3]) / 2;
+    o[1] = s0[4] + s1[4];
+    o[2] = (s0[1] + s1[1]) / 2;
+    o[3] = (s0[0] + s1[0]) / 2;
+    o[4] = (s0[2] + s1[2]) / 2;
+    o[5] = 1.0;
+    o[6] = s0[5] + s1[5];
+    o[7] = s0[6] + s1[6];
+    for (int k = 0; k < 8; ++k) {
+      o[8 + k] = s0[k];
+      o[16 + k] = s1[k];
+    }
+  }
+}
+
+template __global__ void k_nearest64<false>(const double4*, const int*, const double4*, const int*, int,
+                                            const int*, int, double*, int*);
+template __global__ void k_nearest64<true>(const double4*, const int*, const double4*, const int*, int,
+                                           const int*, int, double*, int*);
+
+}  // namespace cms

@@ -1,0 +1,868 @@
+// Surface generation kernels: attention, interface region, neighbour rows,
+// torus pairs, probes, and convex / toroidal / concave dots with burial.
+//
+// Every kernel works on a batch of complexes laid out atom-contiguously
+// (complex-major, binder atoms before target atoms) and loops grid-stride
+// over its units with the unit count read from device memory, so the whole
+// pipeline launches without host synchronisation.  Units whose output size
+// is data dependent run twice: a COUNT pass and, after a device-wide scan,
+// an EMIT pass that writes at the scanned offset.  Units are visited in the
+// reference's emission order, so dot arrays come out in the same order as
+// the NumPy calculator (cms-cuda/_core.py) produces them.
+//
+// One wavefront (64 lanes, one 64-thread workgroup) handles one unit; lanes
+// split the unit's inner work (neighbour slots, circle points, latitudes).
+#include "cms_device.h"
+#include "cms_internal.h"
+
+namespace cms {
+
+// ------------------------------------------------------------ attention
+// _core.py:1150-1195: atten = 1 (blocker) if the nearest atom of the other
+// molecule is >= sep away, else 5.  The same sweep checks coincident
+// same-molecule atoms (d^2 <= 1e-4, _core.py:1285-1293) and records the
+// first such (i, j) pair in row-major order per complex.
+__global__ __launch_bounds__(64) void k_attention(BatchView b, Settings s, int all_atoms,
+                                                  int8_t* __restrict__ atten,
+                                                  int8_t* __restrict__ access,
+                                                  unsigned long long* __restrict__ coinc,
+                                                  int* __restrict__ status) {
+  CMS_UNIT_LOOP(g, b.ntot) {
+    const int c = b.acplx[g];
+    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
+    const int m = b.amol[g];
+    const V3 xi = ld3(b.atom[g]);
+    double mn = INFINITY;
+    unsigned long long ckey = ~0ull;
+    for (int j0 = a0; j0 < a1; j0 += WAVE) {
+      const int j = j0 + lane();
+      if (j < a1) {
+        const V3 d = xi - ld3(b.atom[j]);
+        const double d2 = dot_seq(d, d);
+        if (b.amol[j] != m) {
+          mn = fmin(mn, d2);
+        } else if (j != g && d2 <= 0.0001) {
+          unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
+          ckey = ckey < key ? ckey : key;
+        }
+      }
+    }
+    mn = wave_min_double(mn);
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long t = __shfl_xor(ckey, o, WAVE);
+      ckey = ckey < t ? ckey : t;
+    }
+    if (lane() == 0) {
+      double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
+      atten[g] = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
+      access[g] = 0;
+      if (ckey != ~0ull) {
+        atomicMin(&coinc[c], ckey);
+        atomicOr(&status[c], ST_COINCIDENT);
+      }
+    }
+  }
+}
+
+__global__ void k_flag_nonblocker(int n, const int8_t* __restrict__ atten, int* __restrict__ flag) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x)
+    flag[g] = atten[g] > ATTEN_BLOCKER ? 1 : 0;
+}
+
+// Interface region: atoms that are non-blockers or lie within bridge
+// distance (r_i + r_j + 2rp) of a same-molecule non-blocker.  Only these
+// atoms can own dots, appear in a toroid-queue pair or in a probe triple
+// with a non-blocker, so restricting neighbour rows, pairs and probes to them
+// leaves every dot, probe and readout unchanged.  With full != 0 every atom
+// is kept (full run-state introspection, max-CMS mode).
+__global__ void k_region(BatchView b, Settings s, int full, const int8_t* __restrict__ atten,
+                         const int* __restrict__ nb_scan, const int* __restrict__ nbl,
+                         int* __restrict__ need) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < b.ntot; g += gridDim.x * blockDim.x) {
+    int keep = full || atten[g] > ATTEN_BLOCKER;
+    if (!keep) {
+      const int c = b.acplx[g];
+      const int m = b.amol[g];
+      const double4 ai = b.atom[g];
+      const V3 xi = ld3(ai);
+      const int lo = nb_scan[b.aoff[c]], hi = nb_scan[b.aoff[c + 1]];
+      for (int t = lo; t < hi && !keep; ++t) {
+        const int j = nbl[t];
+        if (b.amol[j] != m) continue;
+        const double4 aj = b.atom[j];
+        const V3 d = xi - ld3(aj);
+        const double d2 = dot_seq(d, d);
+        const double br = (ai.w + aj.w) + 2.0 * s.rp;
+        if (d2 < br * br) keep = 1;
+      }
+    }
+    need[g] = keep;
+  }
+}
+
+__global__ void k_compact_index(int n, const int* __restrict__ flag, const int* __restrict__ scan,
+                                int* __restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (flag[i]) out[scan[i]] = i;
+}
+
+// ------------------------------------------------------------ neighbours
+// _core.py:1251-1349.  Row of atom i: same-molecule atoms j != i with
+// d^2 < (r_i + r_j + 2rp)^2 sorted by (d^2, j); buried row: other-molecule
+// atoms with atten >= 5 inside the same bridge, ascending j.
+__global__ __launch_bounds__(64) void k_neighbors(BatchView b, Settings s, Work w) {
+  __shared__ int cj[KCAP];
+  __shared__ double cd[KCAP];
+  const int nR = *w.nR;
+  CMS_UNIT_LOOP(r, nR) {
+    const int g = w.ratom[r];
+    const int c = b.acplx[g];
+    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
+    const int m = b.amol[g];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    int cnt = 0, cntb = 0;
+    int* brow = w.bur + (int64_t)r * BCAP;
+    for (int j0 = a0; j0 < a1; j0 += WAVE) {
+      const int j = j0 + lane();
+      bool isn = false, isb = false;
+      double d2 = 0.0;
+      if (j < a1) {
+        const double4 aj = b.atom[j];
+        const V3 d = xi - ld3(aj);
+        d2 = dot_seq(d, d);
+        const double br = (ai.w + aj.w) + 2.0 * s.rp;
+        const bool inb = d2 < br * br;
+        if (b.amol[j] == m) isn = inb && j != g;
+        else isb = inb && w.atten[j] >= ATTEN_BURIED;
+      }
+      const uint64_t mn = ballot(isn), mb = ballot(isb);
+      if (isn) {
+        const int p = cnt + lanes_below(mn);
+        if (p < KCAP) { cj[p] = j; cd[p] = d2; }
+      }
+      if (isb) {
+        const int p = cntb + lanes_below(mb);
+        if (p < BCAP) brow[p] = j;
+      }
+      cnt += __popcll(mn);
+      cntb += __popcll(mb);
+    }
+    __syncthreads();
+    if (cnt > KCAP || cntb > BCAP) {
+      if (lane() == 0) atomicOr(&w.status[c], cnt > KCAP ? ST_NBR_CAP : ST_BUR_CAP);
+      cnt = min(cnt, KCAP);
+      cntb = min(cntb, BCAP);
+    }
+    int* row = w.nbr + (int64_t)r * KCAP;
+    for (int p = lane(); p < cnt; p += WAVE) {
+      const double dp = cd[p];
+      const int jp = cj[p];
+      int rank = 0;
+      for (int t = 0; t < cnt; ++t) {
+        const double dt = cd[t];
+        rank += (dt < dp) || (dt == dp && cj[t] < jp);
+      }
+      row[rank] = jp;
+    }
+    if (lane() == 0) {
+      w.nn[r] = cnt;
+      w.nb[r] = cntb;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ pairs
+// _core.py:1351-1442.  Forward pairs (j > i) of atoms with neighbours, torus
+// centre/radius; the single-neighbour shortcut marks access and emits none.
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
+  const int nR = *w.nR;
+  CMS_UNIT_LOOP(r, nR) {
+    const int g = w.ratom[r];
+    const int nn = w.nn[r];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    const double ri = ai.w, eri = ri + s.rp;
+    const int* row = w.nbr + (int64_t)r * KCAP;
+    int base = EMIT ? w.pscan[r] : 0;
+    int total = 0;
+    bool shortcut = false;
+    for (int s0 = 0; s0 < nn; s0 += WAVE) {
+      const int sl = s0 + lane();
+      bool ok = false;
+      int j = -1;
+      V3 u{0, 0, 0}, t{0, 0, 0};
+      double rij = 0.0;
+      bool between = false;
+      if (sl < nn) {
+        j = row[sl];
+        if (j > g) {
+          const double4 aj = b.atom[j];
+          const V3 xj = ld3(aj);
+          const double rj = aj.w, erj = rj + s.rp;
+          const V3 diff = xj - xi;
+          const double dsq = dot_alt(diff, diff);
+          const double dij = sqrt(dsq);
+          const double asymm = (eri * eri - erj * erj) / dij;
+          between = fabs(asymm) < dij;
+          u = diff / dij;
+          t = (xi + xj) * 0.5 + u * (asymm * 0.5);
+          const double far_sq = (eri + erj) * (eri + erj) - dsq;
+          const double cont_sq = dsq - (ri - rj) * (ri - rj);
+          ok = (far_sq > 0.0) && (cont_sq > 0.0);
+          if (ok) rij = (0.5 * sqrt(far_sq)) * sqrt(cont_sq) / dij;
+        }
+      }
+      if (nn <= 1) {
+        // nn == 1 here: shortcut when slot 0 is a valid forward pair
+        shortcut = wave_any(ok);
+        if (shortcut) {
+          if (!EMIT && sl == 0) {
+            w.access[g] = 1;
+            w.access[j] = 1;
+          }
+          break;
+        }
+      }
+      const uint64_t mk = ballot(ok);
+      if (EMIT && ok) {
+        const int o = base + total + lanes_below(mk);
+        w.pi[o] = g;
+        w.pj[o] = j;
+        w.pu[o] = make_double4(u.x, u.y, u.z, rij);
+        w.pt[o] = make_double4(t.x, t.y, t.z, 0.0);
+        const int8_t ai8 = w.atten[g], aj8 = w.atten[j];
+        const bool tor = (ai8 > ATTEN_BLOCKER) || ((aj8 > ATTEN_BLOCKER) && (s.rp > 0.0));
+        w.pflag[o] = (int8_t)((between ? 1 : 0) | (tor ? 2 : 0));
+      }
+      total += __popcll(mk);
+    }
+    if (!EMIT && lane() == 0) w.pcnt[r] = shortcut ? 0 : total;
+  }
+}
+
+// ------------------------------------------------------------ probes
+// _core.py:1445-1643.  For pair (i, j) and neighbour k of i with k > j:
+// triangle tests, pair kill by degenerate triples, probe base/height, the
+// two probe positions, and their collision with i's other neighbours.
+// Probe order per molecule group: all "+" probes, then all "-" probes, each
+// in (pair, k) order.
+__device__ __forceinline__ int group_of_atom(const BatchView& b, int g) {
+  return 2 * b.acplx[g] + b.amol[g];
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_probes(BatchView b, Settings s, Work w) {
+  __shared__ int lj[KCAP];
+  __shared__ double4 lx[KCAP];
+  const int nP = *w.nP;
+  CMS_UNIT_LOOP(p, nP) {
+    const int i = w.pi[p], j = w.pj[p];
+    const int r = w.rpos[i];
+    const int nn = w.nn[r];
+    const int* row = w.nbr + (int64_t)r * KCAP;
+    for (int sl = lane(); sl < nn; sl += WAVE) {
+      const int k = row[sl];
+      lj[sl] = k;
+      lx[sl] = b.atom[k];
+    }
+    __syncthreads();
+    const double4 ai = b.atom[i], aj = b.atom[j];
+    const V3 xi = ld3(ai), xj = ld3(aj);
+    const double eri = ai.w + s.rp, erj = aj.w + s.rp;
+    const double4 pu = w.pu[p], ptt = w.pt[p];
+    const V3 uij = ld3(pu), tij = ld3(ptt);
+    const double rij = pu.w;
+    const bool bi = w.atten[i] <= ATTEN_BLOCKER, bj = w.atten[j] <= ATTEN_BLOCKER;
+
+    // pass 1: does a degenerate triple kill the pair?
+    bool killed = false;
+    for (int s0 = 0; s0 < nn; s0 += WAVE) {
+      const int sl = s0 + lane();
+      bool kill = false;
+      if (sl < nn) {
+        const int k = lj[sl];
+        const V3 xk = ld3(lx[sl]);
+        const double erk = lx[sl].w + s.rp;
+        bool valid = k > j;
+        if (valid) valid = norm_seq(xk - xj) < erj + erk;
+        double dik = 0.0;
+        if (valid) { dik = norm_seq(xk - xi); valid = dik < eri + erk; }
+        if (valid) valid = !(bi && bj && w.atten[k] <= ATTEN_BLOCKER);
+        if (valid) {
+          const V3 uik = (xk - xi) / dik;
+          const double dt = dot_alt(uij, uik);
+          const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+          const double sw = sin(wk);
+          const bool deg = (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
+          if (deg) {
+            const V3 dk = tij - xk;
+            kill = dot_seq(dk, dk) < erk * erk - rij * rij;
+          }
+        }
+      }
+      killed = killed || wave_any(kill);
+    }
+    int np_plus = 0, np_minus = 0;
+    int ob_plus = 0, ob_minus = 0;
+    if (EMIT) {
+      const int grp = group_of_atom(b, i);
+      const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
+      const int64_t vp = 2ll * g0 + (p - g0);
+      const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
+      ob_plus = w.prscan[vp];
+      ob_minus = w.prscan[vm];
+    }
+    if (!killed) {
+      for (int s0 = 0; s0 < nn; s0 += WAVE) {
+        const int sl = s0 + lane();
+        bool okp = false, okm = false;
+        int k = -1;
+        V3 pp{0, 0, 0}, pm{0, 0, 0}, uijk{0, 0, 0};
+        double h = 0.0;
+        if (sl < nn) {
+          k = lj[sl];
+          const V3 xk = ld3(lx[sl]);
+          const double erk = lx[sl].w + s.rp;
+          bool valid = k > j;
+          if (valid) valid = norm_seq(xk - xj) < erj + erk;
+          double dik = 0.0;
+          if (valid) { dik = norm_seq(xk - xi); valid = dik < eri + erk; }
+          if (valid) valid = !(bi && bj && w.atten[k] <= ATTEN_BLOCKER);
+          if (valid) {
+            const V3 uik = (xk - xi) / dik;
+            const double dt = dot_alt(uij, uik);
+            const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+            const double sw = sin(wk);
+            const bool deg = (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
+            if (!deg) {
+              uijk = cross(uij, uik) / sw;
+              const V3 utb = cross(uijk, uij);
+              const double asymm = (eri * eri - erk * erk) / dik;
+              const V3 tik = (xi + xk) * 0.5 + uik * (asymm * 0.5);
+              const double dtb = dot_alt(uik, tik - tij);
+              const V3 bijk = tij + utb * (dtb / sw);
+              const V3 bi_ = bijk - xi;
+              const double hsq = eri * eri - dot_alt(bi_, bi_);
+              if (hsq > 0.0) {
+                h = sqrt(hsq);
+                pp = bijk + uijk * h;
+                pm = bijk + uijk * (-h);
+                okp = okm = true;
+                for (int t = 0; t < nn && (okp || okm); ++t) {
+                  const int n = lj[t];
+                  if (n == j || n == k) continue;
+                  const double rr = lx[t].w + s.rp;
+                  const double lim = rr * rr;
+                  const V3 xs = ld3(lx[t]);
+                  const V3 dp = pp - xs, dm = pm - xs;
+                  if (dot_alt(dp, dp) <= lim) okp = false;
+                  if (dot_alt(dm, dm) <= lim) okm = false;
+                }
+              }
+            }
+          }
+        }
+        const uint64_t mp = ballot(okp), mm = ballot(okm);
+        if (!EMIT) {
+          if (okp) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
+          if (okm) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
+        } else {
+          if (okp) {
+            const int o = ob_plus + np_plus + lanes_below(mp);
+            w.patom[o] = make_int4(i, j, k, 0);
+            w.ppos[o] = make_double4(pp.x, pp.y, pp.z, h);
+            w.palt[o] = make_double4(uijk.x, uijk.y, uijk.z, 0.0);
+          }
+          if (okm) {
+            const int o = ob_minus + np_minus + lanes_below(mm);
+            w.patom[o] = make_int4(j, i, k, 0);
+            w.ppos[o] = make_double4(pm.x, pm.y, pm.z, h);
+            w.palt[o] = make_double4(-uijk.x, -uijk.y, -uijk.z, 0.0);
+          }
+        }
+        np_plus += __popcll(mp);
+        np_minus += __popcll(mm);
+      }
+    }
+    if (!EMIT && lane() == 0) {
+      const int grp = group_of_atom(b, i);
+      const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
+      w.prcnt[2ll * g0 + (p - g0)] = np_plus;
+      w.prcnt[2ll * g0 + (g1 - g0) + (p - g0)] = np_minus;
+    }
+    __syncthreads();
+  }
+}
+
+// Low probes (h < rp) within 2rp of another low probe of the same complex
+// (_core.py:2105-2134; self excluded).
+__global__ __launch_bounds__(64) void k_near(BatchView b, Settings s, Work w) {
+  const int nQ = *w.nQ;
+  const double rp2 = s.rp * s.rp;
+  const double lim = 4.0 * rp2;
+  CMS_UNIT_LOOP(q, nQ) {
+    const double4 pq = w.ppos[q];
+    int cnt = 0;
+    if (pq.w < s.rp) {
+      const int c = b.acplx[w.patom[q].x];
+      const int lo = w.prgrp[2 * c], hi = w.prgrp[2 * c + 2];
+      const V3 xq = ld3(pq);
+      int* out = w.near + (int64_t)q * NEARCAP;
+      for (int l0 = lo; l0 < hi; l0 += WAVE) {
+        const int l = l0 + lane();
+        bool ok = false;
+        if (l < hi && l != q) {
+          const double4 pl = w.ppos[l];
+          if (pl.w < s.rp) {
+            const V3 d = xq - ld3(pl);
+            ok = dot_seq(d, d) <= lim;
+          }
+        }
+        const uint64_t mk = ballot(ok);
+        if (ok) {
+          const int o = cnt + lanes_below(mk);
+          if (o < NEARCAP) out[o] = l;
+        }
+        cnt += __popcll(mk);
+      }
+      if (cnt > NEARCAP) {
+        if (lane() == 0) atomicOr(&w.status[c], ST_NEAR_CAP);
+        cnt = NEARCAP;
+      }
+    }
+    if (lane() == 0) w.nnear[q] = cnt;
+  }
+}
+
+// ------------------------------------------------------------ dot output
+// Outward normal and burial test against the owning atom's buried row
+// (_core.py:2337-2377), then the store.
+__device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s, const Work& w,
+                                          int64_t o, V3 pt, V3 pc, double area, int owner,
+                                          int rowner, int8_t type) {
+  V3 nml = (s.rp <= 0.0) ? pt - ld3(b.atom[owner]) : (pc - pt) / s.rp;
+  // every dot owner has a neighbour row; a missing one would be a bug, so
+  // it is flagged (pool status) instead of read out of bounds
+  int nbur = 0;
+  if (rowner >= 0 && rowner < *w.nR && w.ratom[rowner] == owner) {
+    nbur = w.nb[rowner];
+  } else {
+    atomicOr(&w.status[b.acplx[owner]], ST_POOL);
+  }
+  const int* brow = w.bur + (int64_t)rowner * BCAP;
+  int8_t buried = 0;
+  for (int t = 0; t < nbur; ++t) {
+    const double4 ab = b.atom[brow[t]];
+    const V3 d = ld3(ab) - pc;
+    const double lim = ab.w + s.rp;
+    if (dot_seq(d, d) <= lim * lim) { buried = 1; break; }
+  }
+  if (o >= w.dot_cap) return;
+  w.dpos[o] = make_double4(pt.x, pt.y, pt.z, area);
+  w.dnml[o] = make_double4(nml.x, nml.y, nml.z, 0.0);
+  w.dbur[o] = buried;
+  w.dtype[o] = type;
+  w.datom[o] = owner;
+}
+
+// ------------------------------------------------------------ convex
+// _core.py:1646-1840.  Pole from the nearest neighbour, latitude arcs from
+// the pole to the torus-derived south point, circles at each latitude,
+// collision with every neighbour but the nearest one.
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) {
+  __shared__ double4 lx[KCAP];
+  __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
+  __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
+  __shared__ int lat_start[MAX_SUBDIV + 1];
+  const int nR = *w.nR;
+  CMS_UNIT_LOOP(r, nR) {
+    const int g = w.ratom[r];
+    const int nn = w.nn[r];
+    int total = 0;
+    if (nn > 0 && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER) {
+      const int* row = w.nbr + (int64_t)r * KCAP;
+      for (int sl = lane(); sl < nn; sl += WAVE) lx[sl] = b.atom[row[sl]];
+      const double4 ai = b.atom[g];
+      const V3 xi = ld3(ai);
+      const double ri = ai.w, eri = ri + s.rp;
+      const double dens = b.dens[g];
+      const double4 a0 = b.atom[row[0]];
+      const V3 x0 = ld3(a0);
+      const double r0 = a0.w;
+      const V3 north = unit(xi - x0);
+      V3 vt = v3(north.y * north.y + north.z * north.z, north.x * north.x + north.z * north.z,
+                 north.x * north.x + north.y * north.y);
+      vt = unit(vt);
+      if (fabs(dot_seq(vt, north)) > 0.99) vt = v3(1.0, 0.0, 0.0);
+      const V3 eq = unit(cross(north, vt));
+      const V3 vql = cross(eq, north);
+      const double erj = r0 + s.rp;
+      const double dij = norm_seq(x0 - xi);
+      const V3 uij = (x0 - xi) / dij;
+      const double asymm = (eri * eri - erj * erj) / dij;
+      const V3 tij = ((xi + x0) * 0.5) + uij * (asymm * 0.5);
+      const double far_sq = (eri + erj) * (eri + erj) - dij * dij;
+      const double cont_sq = dij * dij - (ri - r0) * (ri - r0);
+      V3 south = v3(0.0, 0.0, -1.0);
+      if (far_sq > 0.0 && cont_sq > 0.0) {
+        const double rij = 0.5 * sqrt(far_sq) * sqrt(cont_sq) / dij;
+        const V3 pij = tij + vql * rij;
+        south = (pij - xi) / eri;
+      }
+      V3 y;
+      const double ang = arc_angle(north, south, eq, &y);
+      const double delta = 1.0 / (sqrt(dens) * ri);
+      const int L = subdiv_count(delta, ang);
+      const double cs = L > 0 ? (ri * ang) / (double)L : 0.0;
+      V3 cx, cy;
+      circle_frame(north, &cx, &cy);
+      // per-latitude circles
+      int run = 0;
+      for (int t0 = 0; t0 < L; t0 += WAVE) {
+        const int t = t0 + lane();
+        int P = 0;
+        if (t < L) {
+          const V3 lat = subdiv_point(v3(0.0, 0.0, 0.0), north, y, ri, delta, t);
+          const double dt = dot_seq(lat, north);
+          const V3 cen = xi + north * dt;
+          const double rsq = ri * ri - dt * dt;
+          double rad = 0.0, cdelta = 0.0, ps = 0.0;
+          if (rsq > 0.0) {
+            rad = sqrt(rsq);
+            cdelta = 1.0 / (sqrt(dens) * rad);
+            P = subdiv_count(cdelta, TWO_PI);
+            ps = P > 0 ? (rad * TWO_PI) / (double)P : 0.0;
+          }
+          lat_cx[t] = cen.x; lat_cy[t] = cen.y; lat_cz[t] = cen.z;
+          lat_rad[t] = rad; lat_delta[t] = cdelta; lat_ps[t] = ps;
+        }
+        const int inc = wave_incl_scan(P);
+        if (t < L) lat_start[t] = run + inc - P;
+        run += __shfl(inc, WAVE - 1, WAVE);
+      }
+      if (lane() == 0) lat_start[L] = run;
+      __syncthreads();
+      const int C = run;
+      int64_t obase = 0;
+      if (EMIT) {
+        const int grp = group_of_atom(b, g);
+        obase = w.dbase[grp * 4 + 0] + w.cscan[r];
+      }
+      const double scale = eri / ri;
+      for (int q0 = 0; q0 < C; q0 += WAVE) {
+        const int q = q0 + lane();
+        bool keep = false;
+        V3 pt{0, 0, 0}, pc{0, 0, 0};
+        int t = 0;
+        if (q < C) {
+          int lo = 0, hi = L;  // find t with lat_start[t] <= q < lat_start[t+1]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lat_start[mid] <= q) lo = mid; else hi = mid;
+          }
+          t = lo;
+          const int kk = q - lat_start[t];
+          pt = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], lat_delta[t], kk);
+          pc = xi + (pt - xi) * scale;
+          keep = true;
+          for (int sl = 1; sl < nn; ++sl) {
+            const V3 d = ld3(lx[sl]) - pc;
+            const double lim = lx[sl].w + s.rp;
+            if (dot_seq(d, d) <= lim * lim) { keep = false; break; }
+          }
+        }
+        const uint64_t mk = ballot(keep);
+        if (EMIT && keep) {
+          const int64_t o = obase + total + lanes_below(mk);
+          store_dot(b, s, w, o, pt, pc, lat_ps[t] * cs, g, r, 1);
+        }
+        total += __popcll(mk);
+      }
+    }
+    if (!EMIT && lane() == 0) w.ccnt[r] = total;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ toroidal
+// _core.py:1847-2067.  Torus circle of the pair, collision with atom 1's
+// neighbours other than atom 2 (strict <), then the saddle arcs towards atom
+// 1 (owner i) and atom 2 (owner j).  Cusp pairs keep pqj = 0, which leaves
+// the atom-2 arc without an angle and hence without dots.
+struct TorusSub {
+  V3 p, pi, pj, axis, pqi, pqj;
+  double ang1, ang2;
+  V3 y1, y2;
+  int n1, n2;
+};
+
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) {
+  __shared__ int lj[KCAP];
+  __shared__ double4 lx[KCAP];
+  const int nP = *w.nP;
+  CMS_UNIT_LOOP(p, nP) {
+    int tot1 = 0, tot2 = 0;
+    bool any_live = false;
+    if (w.pflag[p] & 2) {
+      const int i = w.pi[p], j = w.pj[p];
+      const int ri_ = w.rpos[i], rj_ = w.rpos[j];
+      const int nn = w.nn[ri_];
+      const int* row = w.nbr + (int64_t)ri_ * KCAP;
+      for (int sl = lane(); sl < nn; sl += WAVE) {
+        const int k = row[sl];
+        lj[sl] = k;
+        lx[sl] = b.atom[k];
+      }
+      __syncthreads();
+      const double4 ai = b.atom[i], aj = b.atom[j];
+      const V3 xi = ld3(ai), xj = ld3(aj);
+      const double ri = ai.w, rj = aj.w;
+      const double eri = ri + s.rp, erj = rj + s.rp;
+      const double dens = (b.dens[i] + b.dens[j]) * 0.5;
+      const double4 pu = w.pu[p], ptt = w.pt[p];
+      const V3 uij = ld3(pu), tij = ld3(ptt);
+      const double rij = pu.w;
+      const bool between = (w.pflag[p] & 1) != 0;
+      const double rci = rij * ri / eri;
+      const double rcj = rij * rj / erj;
+      const double rb = fmax(rij - s.rp, 0.0);
+      const double rs = (rci + 2 * rb + rcj) * 0.25;
+      const double e = rs / rij;
+      const double edens = e * e * dens;
+      V3 cx, cy;
+      circle_frame(uij, &cx, &cy);
+      const double tdelta = 1.0 / (sqrt(edens) * rij);
+      const int M = subdiv_count(tdelta, TWO_PI);
+      const double ts = M > 0 ? (rij * TWO_PI) / (double)M : 0.0;
+      const double adelta = 1.0 / (sqrt(dens) * s.rp);
+      const double rp2 = s.rp * s.rp;
+      const bool arc1_on = w.atten[i] >= ATTEN_2, arc2_on = w.atten[j] >= ATTEN_2;
+      int64_t ob1 = 0, ob2 = 0;
+      if (EMIT) {
+        const int grp = group_of_atom(b, i);
+        ob1 = w.dbase[grp * 4 + 1] + w.t1scan[p];
+        ob2 = w.dbase[grp * 4 + 2] + w.t2scan[p];
+      }
+      for (int m0 = 0; m0 < M; m0 += WAVE) {
+        const int m = m0 + lane();
+        TorusSub ts_{};
+        ts_.n1 = ts_.n2 = 0;
+        bool live = false;
+        if (m < M) {
+          const V3 pm = subdiv_point(tij, cx, cy, rij, tdelta, m);
+          live = true;
+          for (int sl = 0; sl < nn; ++sl) {
+            if (lj[sl] == j) continue;
+            const V3 d = pm - ld3(lx[sl]);
+            const double lim = lx[sl].w + s.rp;
+            if (dot_alt(d, d) < lim * lim) { live = false; break; }
+          }
+          if (live) {
+            ts_.p = pm;
+            ts_.pi = (xi - pm) / eri;
+            ts_.pj = (xj - pm) / erj;
+            ts_.axis = unit(cross(ts_.pi, ts_.pj));
+            const double dtq = rp2 - rij * rij;
+            if (dtq > 0.0 && between) {
+              const double dq = sqrt(dtq);
+              const V3 qij = tij - uij * dq;
+              ts_.pqi = (qij - pm) / s.rp;
+              ts_.pqj = v3(0.0, 0.0, 0.0);
+            } else {
+              const V3 tmp = unit(ts_.pi + ts_.pj);
+              ts_.pqi = tmp;
+              ts_.pqj = tmp;
+            }
+            const double dt1 = dot_seq(ts_.pqi, ts_.pi);
+            const double dt2 = dot_seq(ts_.pqj, ts_.pj);
+            if (fabs(dt1) < 1.0 && fabs(dt2) < 1.0) {
+              if (arc1_on) {
+                ts_.ang1 = arc_angle(ts_.pi, ts_.pqi, ts_.axis, &ts_.y1);
+                ts_.n1 = subdiv_count(adelta, ts_.ang1);
+              }
+              if (arc2_on) {
+                ts_.ang2 = arc_angle(ts_.pqj, ts_.pj, ts_.axis, &ts_.y2);
+                ts_.n2 = subdiv_count(adelta, ts_.ang2);
+              }
+            }
+          }
+        }
+        any_live = any_live || wave_any(live);
+        const int inc1 = wave_incl_scan(ts_.n1);
+        const int inc2 = wave_incl_scan(ts_.n2);
+        if (EMIT) {
+          if (ts_.n1 > 0) {
+            const double ps = (s.rp * ts_.ang1) / (double)ts_.n1;
+            const int64_t o0 = ob1 + tot1 + inc1 - ts_.n1;
+            for (int k = 0; k < ts_.n1; ++k) {
+              const V3 pt = subdiv_point(ts_.p, ts_.pi, ts_.y1, s.rp, adelta, k);
+              const double dist = dist_to_line(tij, uij, pt);
+              const double area = ps * ts * dist / rij;
+              store_dot(b, s, w, o0 + k, pt, ts_.p, area, i, ri_, 2);
+            }
+          }
+          if (ts_.n2 > 0) {
+            const double ps = (s.rp * ts_.ang2) / (double)ts_.n2;
+            const int64_t o0 = ob2 + tot2 + inc2 - ts_.n2;
+            for (int k = 0; k < ts_.n2; ++k) {
+              const V3 pt = subdiv_point(ts_.p, ts_.pqj, ts_.y2, s.rp, adelta, k);
+              const double dist = dist_to_line(tij, uij, pt);
+              const double area = ps * ts * dist / rij;
+              store_dot(b, s, w, o0 + k, pt, ts_.p, area, j, rj_, 2);
+            }
+          }
+        }
+        tot1 += __shfl(inc1, WAVE - 1, WAVE);
+        tot2 += __shfl(inc2, WAVE - 1, WAVE);
+      }
+    }
+    if (!EMIT && lane() == 0) {
+      w.t1cnt[p] = tot1;
+      w.t2cnt[p] = tot2;
+      w.tflag[p] = any_live ? 1 : 0;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ concave
+// _core.py:2069-2333.  Spherical triangle of each probe: latitude arcs from
+// the highest vertex towards -alt, circles, the three plane tests, low-probe
+// rejection, and the closest of the three atoms as owner.
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w) {
+  __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
+  __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
+  __shared__ int lat_start[MAX_SUBDIV + 1];
+  __shared__ double4 lnear[NEARCAP];
+  const int nQ = *w.nQ;
+  const double rp = s.rp, rp2 = s.rp * s.rp;
+  CMS_UNIT_LOOP(q, nQ) {
+    const int4 at = w.patom[q];
+    const double4 pq = w.ppos[q];
+    const V3 p = ld3(pq);
+    const double h = pq.w;
+    const V3 u = ld3(w.palt[q]);
+    const int aid[3] = {at.x, at.y, at.z};
+    const double4 A0 = b.atom[at.x], A1 = b.atom[at.y], A2 = b.atom[at.z];
+    const V3 xa[3] = {ld3(A0), ld3(A1), ld3(A2)};
+    const double ra[3] = {A0.w, A1.w, A2.w};
+    const double dens = ((b.dens[at.x] + b.dens[at.y]) + b.dens[at.z]) / 3.0;
+    V3 vp[3];
+    for (int a = 0; a < 3; ++a) vp[a] = unit(xa[a] - p);
+    const V3 vec0 = unit(cross(vp[0], vp[1]));
+    const V3 vec1 = unit(cross(vp[1], vp[2]));
+    const V3 vec2 = unit(cross(vp[2], vp[0]));
+    int mm = 0;
+    double best = dot_seq(u, vp[0]);
+    for (int a = 1; a < 3; ++a) {
+      const double d = dot_seq(u, vp[a]);
+      if (d > best) { best = d; mm = a; }
+    }
+    const V3 top = vp[mm];
+    const V3 south = neg(u);
+    const V3 axis = unit(cross(top, south));
+    V3 y;
+    const double ang = arc_angle(top, south, axis, &y);
+    const double delta = 1.0 / (sqrt(dens) * rp);
+    const int L = subdiv_count(delta, ang);
+    const double cs = L > 0 ? (rp * ang) / (double)L : 0.0;
+    V3 cx, cy;
+    circle_frame(south, &cx, &cy);
+    const int nnear = (h < rp) ? w.nnear[q] : 0;
+    for (int t = lane(); t < nnear; t += WAVE) lnear[t] = w.ppos[w.near[(int64_t)q * NEARCAP + t]];
+    int run = 0;
+    for (int t0 = 0; t0 < L; t0 += WAVE) {
+      const int t = t0 + lane();
+      int P = 0;
+      if (t < L) {
+        const V3 lat = subdiv_point(v3(0.0, 0.0, 0.0), top, y, rp, delta, t);
+        const double dtl = dot_seq(lat, south);
+        const V3 cen = south * dtl;
+        const double rsq = rp2 - dtl * dtl;
+        const double rad = sqrt(fmax(rsq, 0.0));
+        double cdelta = 0.0, ps = 0.0;
+        if (rad > 0.0) {
+          cdelta = 1.0 / (sqrt(dens) * rad);
+          P = subdiv_count(cdelta, TWO_PI);
+          ps = P > 0 ? (rad * TWO_PI) / (double)P : 0.0;
+        }
+        lat_cx[t] = cen.x; lat_cy[t] = cen.y; lat_cz[t] = cen.z;
+        lat_rad[t] = rad; lat_delta[t] = cdelta; lat_ps[t] = ps;
+      }
+      const int inc = wave_incl_scan(P);
+      if (t < L) lat_start[t] = run + inc - P;
+      run += __shfl(inc, WAVE - 1, WAVE);
+    }
+    if (lane() == 0) lat_start[L] = run;
+    __syncthreads();
+    const int C = run;
+    int64_t obase = 0;
+    if (EMIT) {
+      const int grp = group_of_atom(b, at.x);
+      obase = w.dbase[grp * 4 + 3] + w.kscan[q];
+    }
+    int total = 0;
+    for (int q0 = 0; q0 < C; q0 += WAVE) {
+      const int qq = q0 + lane();
+      bool keep = false;
+      V3 pts{0, 0, 0};
+      int t = 0;
+      if (qq < C) {
+        int lo = 0, hi = L;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (lat_start[mid] <= qq) lo = mid; else hi = mid;
+        }
+        t = lo;
+        const int kk = qq - lat_start[t];
+        const V3 rel = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t],
+                                    lat_delta[t], kk);
+        const bool bail = dot_seq(rel, vec0) >= 0.0 || dot_seq(rel, vec1) >= 0.0 ||
+                          dot_seq(rel, vec2) >= 0.0;
+        if (!bail) {
+          pts = rel + p;
+          keep = true;
+          for (int l = 0; l < nnear; ++l) {
+            const V3 d = pts - ld3(lnear[l]);
+            if (dot_alt(d, d) < rp2) { keep = false; break; }
+          }
+        }
+      }
+      const uint64_t mk = ballot(keep);
+      if (EMIT && keep) {
+        int mc = 0;
+        double dbest = norm_seq(pts - xa[0]) - ra[0];
+        for (int a = 1; a < 3; ++a) {
+          const double d = norm_seq(pts - xa[a]) - ra[a];
+          if (d < dbest) { dbest = d; mc = a; }
+        }
+        const int owner = aid[mc];
+        const int64_t o = obase + total + lanes_below(mk);
+        store_dot(b, s, w, o, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3);
+      }
+      total += __popcll(mk);
+    }
+    if (!EMIT && lane() == 0) w.kcnt[q] = total;
+    __syncthreads();
+  }
+}
+
+// explicit instantiations used by the host pipeline
+template __global__ void k_pairs<false>(BatchView, Settings, Work);
+template __global__ void k_pairs<true>(BatchView, Settings, Work);
+template __global__ void k_probes<false>(BatchView, Settings, Work);
+template __global__ void k_probes<true>(BatchView, Settings, Work);
+template __global__ void k_convex<false>(BatchView, Settings, Work);
+template __global__ void k_convex<true>(BatchView, Settings, Work);
+template __global__ void k_toroid<false>(BatchView, Settings, Work);
+template __global__ void k_toroid<true>(BatchView, Settings, Work);
+template __global__ void k_concave<false>(BatchView, Settings, Work);
+template __global__ void k_concave<true>(BatchView, Settings, Work);
+
+}  // namespace cms

@@ -1,0 +1,26 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cms_internal.h"
+
+namespace cms {
+
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total.  part needs
+// 1025 ints of scratch.
+void scan_excl(const int* in, int* out, int64_t n, int* part, hipStream_t st);
+void fill_int(int* p, int64_t n, int v, hipStream_t st);
+void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
+void fill_double(double* p, int64_t n, double v, hipStream_t st);
+
+__global__ void k_pack_atoms(int ntot, const double* xyz, const double* rad, const double* dens_in,
+                             double dens_default, const int* aoff, const int* nb0, int B,
+                             double4* atom, double* dens, int* acplx, int8_t* amol);
+__global__ void k_group_starts_from_atoms(int n, const int* items, const int* acplx,
+                                          const int8_t* amol, int G, int* start);
+__global__ void k_gather_scan(int G, const int* idx, int mul, const int* scan, int* out);
+__global__ void k_dot_bases(int G, const int* rgrp, const int* pgrp, const int* prgrp,
+                            const int* cscan, const int* t1scan, const int* t2scan,
+                            const int* kscan, int64_t* dbase, int* dgrp);
+
+}  // namespace cms

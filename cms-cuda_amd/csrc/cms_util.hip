@@ -1,0 +1,198 @@
+// Device-wide utilities: exclusive scan, fills, group offsets.
+#include "cms_device.h"
+#include "cms_internal.h"
+#include "cms_util.h"
+
+namespace cms {
+
+constexpr int SCAN_THREADS = 256;
+
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+  // 256 threads = 4 waves
+  __shared__ int wsum[SCAN_THREADS / WAVE];
+  const int l = lane(), wid = threadIdx.x / WAVE;
+  const int inc = wave_incl_scan(v);
+  if (l == WAVE - 1) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int i = 0; i < SCAN_THREADS / WAVE; ++i) {
+    if (i < wid) off += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partial(const int* __restrict__ in, int64_t n,
+                                                              int64_t chunk, int* __restrict__ part) {
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  int acc = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += SCAN_THREADS) acc += in[i];
+  int tot;
+  block_excl_scan(acc, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int* __restrict__ part, int nb) {
+  int carry = 0;
+  for (int base = 0; base < nb; base += SCAN_THREADS) {
+    const int i = base + threadIdx.x;
+    const int v = i < nb ? part[i] : 0;
+    int tot;
+    const int e = block_excl_scan(v, &tot);
+    if (i < nb) part[i] = carry + e;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[nb] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restrict__ in, int* __restrict__ out,
+                                                            int64_t n, int64_t chunk,
+                                                            const int* __restrict__ part, int nb) {
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  int carry = part[blockIdx.x];
+  for (int64_t base = lo; base < hi; base += SCAN_THREADS) {
+    const int64_t i = base + threadIdx.x;
+    const int v = i < hi ? in[i] : 0;
+    int tot;
+    const int e = block_excl_scan(v, &tot);
+    if (i < hi) out[i] = carry + e;
+    carry += tot;
+  }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = part[nb];
+}
+
+void scan_excl(const int* in, int* out, int64_t n, int* part, hipStream_t st) {
+  // part must hold at least 1025 ints
+  int nb = (int)((n + 4095) / 4096);
+  if (nb < 1) nb = 1;
+  if (nb > 1024) nb = 1024;
+  const int64_t chunk = (n + nb - 1) / nb > 0 ? (n + nb - 1) / nb : 1;
+  k_scan_partial<<<nb, SCAN_THREADS, 0, st>>>(in, n, chunk, part);
+  k_scan_top<<<1, SCAN_THREADS, 0, st>>>(part, nb);
+  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(in, out, n, chunk, part, nb);
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_double(double* p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void fill_int(int* p, int64_t n, int v, hipStream_t st) {
+  if (n <= 0) return;
+  int g = (int)((n + 255) / 256);
+  if (g > 4096) g = 4096;
+  k_fill_int<<<g, 256, 0, st>>>(p, n, v);
+}
+void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st) {
+  if (n <= 0) return;
+  int g = (int)((n + 255) / 256);
+  if (g > 4096) g = 4096;
+  k_fill_u64<<<g, 256, 0, st>>>(p, n, v);
+}
+void fill_double(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  int g = (int)((n + 255) / 256);
+  if (g > 4096) g = 4096;
+  k_fill_double<<<g, 256, 0, st>>>(p, n, v);
+}
+
+// Pack (x, y, z) + radius into double4 and derive per-atom complex id and
+// molecule from the complex offsets and binder counts.
+__global__ void k_pack_atoms(int ntot, const double* __restrict__ xyz, const double* __restrict__ rad,
+                             const double* __restrict__ dens_in, double dens_default,
+                             const int* __restrict__ aoff, const int* __restrict__ nb0, int B,
+                             double4* __restrict__ atom, double* __restrict__ dens,
+                             int* __restrict__ acplx, int8_t* __restrict__ amol) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ntot; g += gridDim.x * blockDim.x) {
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (aoff[mid] <= g) lo = mid; else hi = mid;
+    }
+    atom[g] = make_double4(xyz[3 * (int64_t)g], xyz[3 * (int64_t)g + 1], xyz[3 * (int64_t)g + 2], rad[g]);
+    dens[g] = dens_in ? dens_in[g] : dens_default;
+    acplx[g] = lo;
+    amol[g] = (g - aoff[lo]) >= nb0[lo] ? 1 : 0;
+  }
+}
+
+// First index of each group in a list sorted by group (group of item i is
+// key(i)); writes start[0..G], start[G] = n.
+__global__ void k_group_starts_from_atoms(int n, const int* __restrict__ items,
+                                          const int* __restrict__ acplx, const int8_t* __restrict__ amol,
+                                          int G, int* __restrict__ start) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+    const int k = i < n ? 2 * acplx[items[i]] + amol[items[i]] : G;
+    const int kp = i > 0 ? 2 * acplx[items[i - 1]] + amol[items[i - 1]] : -1;
+    for (int g = kp + 1; g <= k; ++g) start[g] = i;
+  }
+}
+
+// out[g] = scan[idx[g] * mul] for g in [0, G]
+__global__ void k_gather_scan(int G, const int* __restrict__ idx, int mul, const int* __restrict__ scan,
+                              int* __restrict__ out) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x)
+    out[g] = scan[(int64_t)idx[g] * mul];
+}
+
+// Dot group bases: per group the four stage totals, group offsets (scan over
+// G) and per-stage bases so that a unit's output offset is
+// dbase[4g + stage] + stage_scan[unit].
+__global__ __launch_bounds__(256) void k_dot_bases(int G, const int* __restrict__ rgrp,
+                                                   const int* __restrict__ pgrp,
+                                                   const int* __restrict__ prgrp,
+                                                   const int* __restrict__ cscan,
+                                                   const int* __restrict__ t1scan,
+                                                   const int* __restrict__ t2scan,
+                                                   const int* __restrict__ kscan,
+                                                   int64_t* __restrict__ dbase,
+                                                   int* __restrict__ dgrp) {
+  __shared__ int wsum[4];
+  int carry = 0;
+  for (int base = 0; base < G; base += 256) {
+    const int g = base + threadIdx.x;
+    int T0 = 0, T1 = 0, T2 = 0, T3 = 0;
+    if (g < G) {
+      T0 = cscan[rgrp[g + 1]] - cscan[rgrp[g]];
+      T1 = t1scan[pgrp[g + 1]] - t1scan[pgrp[g]];
+      T2 = t2scan[pgrp[g + 1]] - t2scan[pgrp[g]];
+      T3 = kscan[prgrp[g + 1]] - kscan[prgrp[g]];
+    }
+    const int D = T0 + T1 + T2 + T3;
+    const int inc = wave_incl_scan(D);
+    const int wid = threadIdx.x / WAVE;
+    if (lane() == WAVE - 1) wsum[wid] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < 4; ++i) {
+      if (i < wid) off += wsum[i];
+      tot += wsum[i];
+    }
+    const int start = carry + off + inc - D;
+    if (g < G) {
+      dgrp[g] = start;
+      dbase[4 * g + 0] = (int64_t)start - cscan[rgrp[g]];
+      dbase[4 * g + 1] = (int64_t)start + T0 - t1scan[pgrp[g]];
+      dbase[4 * g + 2] = (int64_t)start + T0 + T1 - t2scan[pgrp[g]];
+      dbase[4 * g + 3] = (int64_t)start + T0 + T1 + T2 - kscan[prgrp[g]];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dgrp[G] = carry;
+}
+
+}  // namespace cms

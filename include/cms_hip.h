@@ -1,0 +1,186 @@
+/*
+ * cms_hip.h -- C ABI of libcms_hip.so, the MI355X (gfx950) implementation of
+ * the contact-molecular-surface (CMS) and Lawrence-Colman shape
+ * complementarity (SC) pipeline of cms-cuda's MolecularSurfaceCalculator.
+ *
+ * Plain C: device pointers, host pointers and sizes only.  All device work
+ * is issued on the stream given to cms_create(); functions that return
+ * sizes or status synchronise that stream, the others are asynchronous
+ * unless stated.  The library never frees caller memory; it owns a
+ * workspace per context that grows on demand.  One context per host thread.
+ *
+ * Batches: a call works on B >= 1 complexes at once.  Atoms are passed
+ * complex-major; within a complex the binder (molecule 0) atoms come first,
+ * then the target (molecule 1) atoms -- the order add_binder_and_target()
+ * builds (cms-cuda/_core.py:918-943).  Atom indices reported back (dot
+ * owners, coincident pairs) are local to their complex, numbered like the
+ * reference's natom: binder 0..N0-1, target N0..N0+N1-1.
+ */
+#ifndef CMS_HIP_H
+#define CMS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define CMS_OK 0
+#define CMS_ERR_INVALID 1    /* bad argument or call order                      */
+#define CMS_ERR_HIP 2        /* HIP runtime error (message in cms_last_error)    */
+#define CMS_ERR_COINCIDENT 3 /* two atoms of one molecule within 0.01 A          */
+#define CMS_ERR_CAPACITY 4   /* a neighbour / near-probe row exceeded its limit  */
+#define CMS_ERR_STATE 5      /* surface cached in the other all_atoms mode       */
+
+/* precision of the nearest-dot searches */
+#define CMS_PREC_FP64 0        /* plain float64 search                                */
+#define CMS_PREC_FP32_RESCORE 1 /* float32 search on recentred coordinates, float64
+                                   rescoring of the winner and of every candidate
+                                   within the proven error margin                    */
+
+/* per-complex counters written by cms_counts() */
+#define CMS_NCOUNTS 12
+/* [0] dots mol 0, [1] dots mol 1, [2] buried dots mol 0, [3] buried dots mol 1,
+   [4] convex, [5] toroidal, [6] concave, [7] probes, [8] region atoms,
+   [9] pairs, [10] blocked atoms mol 0, [11] blocked atoms mol 1            */
+
+/* per-complex SC record written by cms_shape_complementarity() */
+#define CMS_SC_NRES 24
+/* [0] sc  [1] area  [2] distance  [3] d_mean  [4] s_mean  [5] valid
+   [6] trimmed dots  [7] all dots
+   [8..15]  molecule 0 side: d_mean, d_median, s_mean, s_median,
+            trimmed_area, n_trimmed, n_dots, valid
+   [16..23] molecule 1 side, same fields                                   */
+
+/* Calculator settings; defaults rp 1.7, density 15, band 1.5, sep 8,
+   weight 0.5, binwidth_dist 0.02, binwidth_norm 0.02
+   (replaces MolecularSurfaceCalculator.settings, cms-cuda/_core.py:766-773). */
+typedef struct cms_settings {
+  double rp;
+  double density;
+  double band;
+  double sep;
+  double weight;
+  double binwidth_dist;
+  double binwidth_norm;
+} cms_settings;
+
+typedef struct cms_ctx cms_ctx;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int cms_version(void);
+
+/* Create a context bound to HIP device `device` and stream `stream`
+   (NULL = the legacy default stream).  Replaces the per-thread stream and
+   lazily compiled module of cms-cuda/_gpu.py:70-83, _gpu_kernels.py:407-417. */
+int cms_create(int device, void* stream, cms_ctx** out);
+int cms_destroy(cms_ctx* ctx);
+/* Message of the last error on this context (or of cms_create when ctx is
+   NULL).  Never NULL. */
+const char* cms_last_error(const cms_ctx* ctx);
+/* Wait for all work issued on the context's stream. */
+int cms_synchronize(cms_ctx* ctx);
+
+/* Load B complexes.  xyz: device, 3*ntot float64; radius: device, ntot;
+   density: device, ntot per-atom dot densities or NULL for
+   settings->density; n_binder / n_target: host arrays of B counts.
+   Replaces reset + AddMolecule(0, ...) + AddMolecule(1, ...)
+   (cms-cuda/_core.py:918-943; _gpu.py:469-497).  Invalidates any cached
+   surface.  Synchronous with respect to the host arrays only. */
+int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t* n_target,
+             const double* xyz, const double* radius, const double* density,
+             const cms_settings* settings);
+
+/* Attention numbers, neighbour rows, probes and convex / toroidal / concave
+   dots with burial flags for every loaded complex.  all_atoms = 0 is the
+   CMS / SC mode, 1 the max-CMS mode (every atom attention 5).  full_state
+   = 1 builds neighbour rows and pairs for every atom instead of only the
+   interface region (identical dots; needed only for full run-state
+   inspection).  Cached: a second call with the same all_atoms is a no-op,
+   one with the other mode returns CMS_ERR_STATE.  Replaces
+   _ensure_surfaces_generated (cms-cuda/_core.py:865-909, 1197-2377;
+   _gpu.py:421-461, 691-1765).  Returns CMS_ERR_COINCIDENT when a complex
+   holds coincident atoms (see cms_coincident). */
+int cms_generate(cms_ctx* ctx, int all_atoms, int full_state);
+
+/* Per-complex status word after cms_generate (bit 1 coincident atoms, 2 / 4
+   neighbour / buried row over capacity, 8 near-probe list over capacity). */
+int cms_status(cms_ctx* ctx, int32_t* status_host);
+
+/* First coincident pair (row-major, as np.argwhere) of complex c, local
+   atom indices; -1/-1 if none.  Mirrors the RuntimeError of
+   cms-cuda/_core.py:1285-1293. */
+int cms_coincident(cms_ctx* ctx, int complex_index, int32_t* i, int32_t* j);
+
+/* Counters per complex, CMS_NCOUNTS int64 each (host array of
+   B*CMS_NCOUNTS).  Counterpart of calc.run.results
+   (cms-cuda/_core.py:272-304). */
+int cms_counts(cms_ctx* ctx, int64_t* out_host);
+
+/* Contact molecular surface of every complex, both sides at once.
+   per_atom: device, ntot float64 -- the CMS of each atom as a query atom
+   (binder atoms: binder side; target atoms: target side); totals: device,
+   2*B float64, totals[2c] binder side, totals[2c+1] target side (the value
+   CalcLoaded returns).  Replaces calc_contact_molecular_surface(target_side)
+   (cms-cuda/_core.py:946-1004; _gpu.py:515-571). */
+int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals);
+
+/* Maximum possible CMS (surface area of molecule 0).  Requires a surface
+   generated with all_atoms = 1.  per_atom: device, ntot; totals: device, B.
+   Replaces calc_max_possible_contact_molecular_surface
+   (cms-cuda/_core.py:1007-1034; _gpu.py:573-600). */
+int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals);
+
+/* Shape complementarity of every complex into results (device,
+   B*CMS_SC_NRES float64).  Replaces CalcLoadedSC
+   (cms-cuda/_core.py:807-863, 1037-1147; _gpu.py:350-419, 604-686). */
+int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results);
+
+/* Number of dots of (complex, molecule) and a copy of them into device
+   buffers (coor, nml: 3n float64; area: n; buried, type: n int8; atom: n
+   int32 local atom index).  Any pointer may be NULL.  Counterpart of
+   calc.run.dots[mol] (cms-cuda/_core.py:454-519). */
+int cms_dot_count(cms_ctx* ctx, int complex_index, int mol, int64_t* n);
+int cms_copy_dots(cms_ctx* ctx, int complex_index, int mol, double* coor, double* nml,
+                  double* area, int8_t* buried, int8_t* type, int32_t* atom);
+
+/* Per-atom attention numbers and access flags (device, ntot int8 each) as
+   left by surface generation before the toroidal stage, plus the per-pair
+   toroid-queue flags needed to finish access the reference's way.
+   Counterpart of calc.run.atoms.atten / .access (_core.py:1185-1188,
+   1315, 1396-1400, 1442). */
+int cms_copy_atom_state(cms_ctx* ctx, int8_t* atten, int8_t* access);
+
+/* Probes of complex c (host count; device copies: atoms 3n int32 local,
+   height n, point 3n, alt 3n).  Counterpart of calc.run.probes
+   (cms-cuda/_core.py:522-590), ordered within each molecule as the
+   reference orders them. */
+int cms_probe_count(cms_ctx* ctx, int complex_index, int64_t* n);
+int cms_copy_probes(cms_ctx* ctx, int complex_index, int32_t* atoms, double* height,
+                    double* point, double* alt);
+
+/* Standalone nearest-point search: for each of nq query points (device,
+   3*nq float64) the squared distance to the nearest of nr reference
+   points (device, 3*nr) and, if out_idx != NULL, its first index.  The
+   primitive behind every readout (replaces _gpu_kernels.min_sqdist,
+   cms-cuda/_gpu_kernels.py:508-566). */
+int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r,
+                int64_t nr, double* out_d2, int32_t* out_idx);
+
+/* Kernel timing with HIP events on the context's stream.  filter: NULL or
+   "" = off, "*" = every instrumented kernel, otherwise one kernel name
+   ("attention", "neighbors", "pairs_count", "probes_count", "convex_emit",
+   "toroid_emit", "concave_emit", "nearest", "owner_sums", "sc_sides", ...).
+   cms_profile_read synchronises, returns the summed milliseconds and the
+   number of timed launches of `name` ("*" = all) recorded since the last
+   read, and forgets them.  No reference counterpart (the reference times
+   stages with NVTX ranges, cms-cuda/_gpu.py:30). */
+int cms_profile(cms_ctx* ctx, const char* filter);
+int cms_profile_read(cms_ctx* ctx, const char* name, double* total_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMS_HIP_H */

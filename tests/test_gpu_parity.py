@@ -1,0 +1,224 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's golden numbers.
+
+Bar (integer / index / flag work bit-exact; floats within tolerance):
+* dot counts, dot owners (atom_idx), buried flags and dot types: identical,
+  in the same order;
+* dot coordinates, normals, areas: |a-b| <= 1e-9 * max(1, |a|, |b|) (the
+  reference's own CPU/GPU parity tolerance, test/test_gpu_parity.py:47);
+  only sin/cos/atan2/acos/exp may differ from NumPy in the last ulp;
+* per-atom and total CMS, SC, area, distance: <= 1e-10 relative.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_DOT = 1e-9
+TOL_OUT = 1e-10
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.maximum(np.abs(a), np.abs(b)))))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cms_cuda():
+    import cms_cuda
+    assert cms_cuda.gpu_available(), "HIP library or GPU missing"
+    return cms_cuda
+
+
+def compare_dots(gpu_calc, oracle_calc):
+    for m in (0, 1):
+        g = gpu_calc.run.dots[m]
+        o = oracle_calc.dots[m]
+        assert len(g) == len(o), (m, len(g), len(o))
+        assert np.array_equal(g.atom_idx, o.atom), m
+        assert np.array_equal(g.buried, o.buried), m
+        assert np.array_equal(g.type_, o.type), m
+        assert rel_err(g.coor_xyz, o.coor) <= TOL_DOT
+        assert rel_err(g.outnml_xyz, o.nml) <= TOL_DOT
+        assert rel_err(g.area, o.area) <= TOL_DOT
+
+
+def test_nearest_bruteforce(cms_cuda):
+    """Nearest-point primitive: exact squared distances (same summation
+    order as cdist) and first-index ties, including duplicated refs."""
+    import ctypes
+    import torch
+    from cms_cuda import _lib
+    from scipy.spatial.distance import cdist
+    say("nearest: random 3000 x 2500 with duplicates")
+    rng = np.random.default_rng(11)
+    q = rng.uniform(-20, 20, size=(3000, 3))
+    r = rng.uniform(-20, 20, size=(2500, 3))
+    r[1500:1600] = r[100:200]          # exact duplicates -> ties
+    q[:50] = r[100:150]                # zero distances
+    ctx = _lib.Context(0, torch.cuda.current_stream().cuda_stream)
+    dq = torch.from_numpy(q).cuda()
+    dr = torch.from_numpy(r).cuda()
+    d2 = torch.empty(len(q), dtype=torch.float64, device="cuda")
+    idx = torch.empty(len(q), dtype=torch.int32, device="cuda")
+    ctx("cms_nearest", 0, ctypes.c_void_p(dq.data_ptr()), len(q), ctypes.c_void_p(dr.data_ptr()), len(r),
+        ctypes.c_void_p(d2.data_ptr()), ctypes.c_void_p(idx.data_ptr()))
+    D = cdist(r, q, metric="sqeuclidean")
+    ref_idx = np.argmin(D, axis=0)
+    ref_d2 = D[ref_idx, np.arange(len(q))]
+    assert np.array_equal(d2.cpu().numpy(), ref_d2)
+    assert np.array_equal(idx.cpu().numpy(), ref_idx)
+    ctx.close()
+
+
+def test_golden_complex(cms_cuda, golden_complex, golden_oracle):
+    from test_oracle import GOLDEN
+    c = golden_complex
+    say("golden complex: CMS")
+    cms, per_atom, calc = cms_cuda.calculate_contact_ms(c["binder_xyz"], c["binder_radii"],
+                                                        c["target_xyz"], c["target_radii"])
+    assert isinstance(cms, float) and per_atom.dtype == np.float64 and per_atom.shape == (242,)
+    assert rel_err(cms, GOLDEN["cms"]) <= 1e-9
+    ocms, oper = golden_oracle["cms"]
+    assert rel_err(cms, ocms) <= TOL_OUT
+    assert rel_err(per_atom, oper) <= TOL_OUT
+    res = calc.run.results
+    assert (res.dots.convex, res.dots.toroidal, res.dots.concave) == (
+        GOLDEN["convex"], GOLDEN["toroidal"], GOLDEN["concave"])
+    compare_dots(calc, golden_oracle["calc"])
+    say("golden complex: binder side")
+    bcms, bper = calc.calc_contact_molecular_surface(target_side=False)
+    assert rel_err(bcms, golden_oracle["cms_binder"][0]) <= TOL_OUT
+    assert rel_err(bper, golden_oracle["cms_binder"][1]) <= TOL_OUT
+    say("golden complex: SC on the same calculator")
+    sc, area, dist = calc.CalcLoadedSC()
+    assert rel_err(sc, GOLDEN["sc"]) <= 1e-9
+    assert rel_err(area, GOLDEN["area"]) <= 1e-9
+    assert rel_err(dist, GOLDEN["distance"]) <= 1e-9
+    osc = golden_oracle["sc"]
+    assert rel_err([sc, area, dist], list(osc)) <= TOL_OUT
+    s = calc.run.results.surface
+    assert (s[0].nTrimmedDots, s[1].nTrimmedDots) == GOLDEN["trimmed"]
+    atoms = calc.run.atoms
+    assert int((atoms.atten == 5).sum()) == GOLDEN["atten5"]
+    assert np.array_equal(atoms.atten, golden_oracle["calc"].atten)
+
+
+def test_golden_probes(cms_cuda, golden_complex, golden_oracle):
+    c = golden_complex
+    say("golden complex: probes")
+    calc = cms_cuda.MolecularSurfaceCalculator()
+    calc.add_binder_and_target(c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"])
+    calc.CalcLoaded()
+    pr = calc.run.probes
+    o = golden_oracle["calc"]
+    assert len(pr) == len(o.pr_h)
+    # within each molecule the GPU keeps the reference order; molecules are
+    # stored one after the other
+    mol = (np.asarray(pr.atom_idx_0) >= len(c["binder_radii"])).astype(int)
+    omol = (o.pr_a[:, 0] >= len(c["binder_radii"])).astype(int)
+    for m in (0, 1):
+        g = np.stack([pr.atom_idx_0, pr.atom_idx_1, pr.atom_idx_2], 1)[mol == m]
+        assert np.array_equal(g, o.pr_a[omol == m])
+        assert rel_err(pr.point_xyz[mol == m], o.pr_p[omol == m]) <= TOL_DOT
+
+
+def test_synthetic_config1(cms_cuda):
+    from cms_cuda import synth
+    from oracle.cms_oracle import OracleCalculator
+    say("synthetic 400 + 1000 complex")
+    c = synth.config1(seed=1)
+    args = (c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"])
+    o = OracleCalculator()
+    o.add_binder_and_target(*args)
+    ocms, oper = o.contact_ms(True)
+    osc = o.shape_complementarity()
+    calc = cms_cuda.MolecularSurfaceCalculator()
+    calc.add_binder_and_target(*args)
+    cms, per = calc.CalcLoaded()
+    sc = calc.CalcLoadedSC()
+    compare_dots(calc, o)
+    assert rel_err(cms, ocms) <= TOL_OUT
+    assert rel_err(per, oper) <= TOL_OUT
+    assert rel_err(list(sc), list(osc)) <= TOL_OUT
+
+
+def test_max_cms_golden_molecules(cms_cuda, golden_complex):
+    from oracle.cms_oracle import calculate_maximum_possible_contact_ms as omax
+    for key in ("binder", "target"):
+        say("max-CMS " + key)
+        xyz, r = golden_complex[key + "_xyz"], golden_complex[key + "_radii"]
+        cms, per, calc = cms_cuda.calculate_maximum_possible_contact_ms(xyz, r)
+        ocms, oper, ocalc = omax(xyz, r)
+        assert rel_err(cms, ocms) <= TOL_OUT
+        assert rel_err(per, oper) <= TOL_OUT
+        assert len(calc.run.dots[0]) == len(ocalc.dots[0])
+        assert np.array_equal(calc.run.dots[0].atom_idx, ocalc.dots[0].atom)
+
+
+def test_two_atom_molecule(cms_cuda):
+    from test_oracle import two_atom_area
+    say("two-atom molecule")
+    xyz = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0]])
+    cms, per, calc = cms_cuda.calculate_maximum_possible_contact_ms(xyz, np.array([1.8, 1.6]))
+    assert abs(cms - two_atom_area(3.0, 1.8, 1.6)) <= 1e-9 * cms
+
+
+def test_radius_zero_variant(cms_cuda, golden_complex):
+    from oracle.cms_oracle import OracleCalculator
+    say("radius-0 variant")
+    c = golden_complex
+    br = c["binder_radii"].copy()
+    tr = c["target_radii"].copy()
+    br[::7] = 0.0
+    tr[3::5] = 0.0
+    args = (c["binder_xyz"], br, c["target_xyz"], tr)
+    o = OracleCalculator()
+    o.add_binder_and_target(*args)
+    ocms, oper = o.contact_ms(True)
+    osc = o.shape_complementarity()
+    calc = cms_cuda.MolecularSurfaceCalculator()
+    calc.add_binder_and_target(*args)
+    cms, per = calc.CalcLoaded()
+    sc = calc.CalcLoadedSC()
+    compare_dots(calc, o)
+    assert rel_err(cms, ocms) <= TOL_OUT and rel_err(per, oper) <= TOL_OUT
+    assert rel_err(list(sc), list(osc)) <= TOL_OUT
+
+
+def test_far_apart_and_errors(cms_cuda):
+    say("far apart / coincident / cache mode")
+    rng = np.random.default_rng(2)
+    a = rng.uniform(-3, 3, size=(12, 3))
+    r = np.full(12, 1.8)
+    cms, per, calc = cms_cuda.calculate_contact_ms(a, r, a + 40.0, r)
+    assert cms == 0.0 and per.shape == (12,) and not per.any()
+    assert calc.CalcLoadedSC() == (0.0, 0.0, 0.0)
+    with pytest.raises(RuntimeError, match="reuse it with all_atoms=True"):
+        calc.CalcLoadedMaxPossibleCMS()
+    xyz = np.array([[0.0, 0, 0], [1.5, 0, 0], [1.5, 0.005, 0]])
+    with pytest.raises(RuntimeError, match="Coincident atoms: 1:: == 2::"):
+        cms_cuda.calculate_maximum_possible_contact_ms(xyz, np.full(3, 1.8))
+
+
+def test_deterministic(cms_cuda, golden_complex):
+    say("determinism")
+    c = golden_complex
+    args = (c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"])
+    r1 = cms_cuda.calculate_contact_ms(*args)
+    r2 = cms_cuda.calculate_contact_ms(*args)
+    assert r1[0] == r2[0] and np.array_equal(r1[1], r2[1])
+    d1, d2 = r1[2].run.dots[1], r2[2].run.dots[1]
+    assert np.array_equal(d1.coor_xyz, d2.coor_xyz) and np.array_equal(d1.area, d2.area)
