@@ -1,0 +1,136 @@
+"""Batched CMS / SC: many complexes of uneven size in one pass.
+
+All complexes of a batch go through every stage together (one launch per
+stage for the whole batch, per-complex offsets on the device), which is what
+makes small complexes cheap: a single complex leaves most of the GPU idle.
+Results per complex are bit-identical to running it alone.
+
+    calc = BatchCalculator()
+    calc.load([(binder_xyz, binder_radii, target_xyz, target_radii), ...])
+    cms = calc.contact_ms()          # dict: target, binder totals + per-atom
+    sc = calc.shape_complementarity() # (B, 24) records (see include/cms_hip.h)
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._calc import Settings, _device, _ptr
+
+
+class BatchCalculator:
+    def __init__(self, settings=None, device=None, stream=None):
+        import torch
+        self.settings = settings or Settings()
+        self.dev = device or _device()
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        self.ctx = _lib.Context(self.dev.index, self.stream.cuda_stream)
+        self.B = 0
+        self.mode = None
+
+    def close(self):
+        if self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
+
+    # ------------------------------------------------------------ inputs
+    def load(self, complexes):
+        """complexes: iterable of (binder_xyz, binder_radii, target_xyz,
+        target_radii).  Host arrays are copied to the GPU once."""
+        import torch
+        cs = list(complexes)
+        if not cs:
+            raise ValueError("empty batch")
+        nb = np.array([len(np.asarray(c[1]).reshape(-1)) for c in cs], dtype=np.int32)
+        nt = np.array([len(np.asarray(c[3]).reshape(-1)) for c in cs], dtype=np.int32)
+        xyz = np.concatenate([np.concatenate([np.asarray(c[0], np.float64).reshape(-1, 3),
+                                              np.asarray(c[2], np.float64).reshape(-1, 3)]) for c in cs])
+        rad = np.concatenate([np.concatenate([np.asarray(c[1], np.float64).reshape(-1),
+                                              np.asarray(c[3], np.float64).reshape(-1)]) for c in cs])
+        self.d_xyz = torch.from_numpy(np.ascontiguousarray(xyz)).to(self.dev)
+        self.d_rad = torch.from_numpy(np.ascontiguousarray(rad)).to(self.dev)
+        self.load_device(nb, nt, self.d_xyz, self.d_rad)
+
+    def load_device(self, n_binder, n_target, d_xyz, d_rad, d_dens=None):
+        """Inputs already in HBM: d_xyz (ntot, 3) and d_rad (ntot,) float64
+        torch tensors, complexes back to back, binder atoms first."""
+        self.nb = np.asarray(n_binder, dtype=np.int32)
+        self.nt = np.asarray(n_target, dtype=np.int32)
+        self.B = len(self.nb)
+        self.off = np.concatenate([[0], np.cumsum(self.nb + self.nt)]).astype(np.int64)
+        self.ntot = int(self.off[-1])
+        self._keep = (d_xyz, d_rad, d_dens)
+        self._nb_c = (ctypes.c_int32 * self.B)(*self.nb.tolist())
+        self._nt_c = (ctypes.c_int32 * self.B)(*self.nt.tolist())
+        self._st = self.settings.to_c()
+        self.ctx("cms_load", self.B, self._nb_c, self._nt_c, _ptr(d_xyz), _ptr(d_rad), _ptr(d_dens),
+                 ctypes.byref(self._st))
+        self.mode = None
+
+    # ------------------------------------------------------------ stages
+    def generate(self, all_atoms=False, full_state=False):
+        self.ctx("cms_generate", 1 if all_atoms else 0, 1 if full_state else 0)
+        self.mode = bool(all_atoms)
+
+    def contact_ms(self, precision=_lib.PREC_FP64, out=None):
+        """CMS of every complex, both sides.  Returns dict with numpy arrays
+        target (B,), binder (B,) and per_atom (ntot,), unless out=(per_atom,
+        totals) device tensors are given (then nothing is copied back)."""
+        import torch
+        if self.mode is None:
+            self.generate(False)
+        if out is not None:
+            per_atom, totals = out
+        else:
+            per_atom = torch.empty(max(self.ntot, 1), dtype=torch.float64, device=self.dev)
+            totals = torch.empty(2 * self.B, dtype=torch.float64, device=self.dev)
+        self.ctx("cms_contact_ms", int(precision), _ptr(per_atom), _ptr(totals))
+        if out is not None:
+            return None
+        t = totals.cpu().numpy().reshape(self.B, 2)
+        return dict(target=t[:, 1].copy(), binder=t[:, 0].copy(), per_atom=per_atom[:self.ntot].cpu().numpy())
+
+    def per_atom_split(self, per_atom):
+        """Split a batch per-atom array into [(binder (N0,), target (N1,))]."""
+        out = []
+        for c in range(self.B):
+            a = per_atom[self.off[c]:self.off[c + 1]]
+            out.append((a[:self.nb[c]], a[self.nb[c]:]))
+        return out
+
+    def shape_complementarity(self, precision=_lib.PREC_FP64, out=None):
+        import torch
+        if self.mode is None:
+            self.generate(False)
+        res = out if out is not None else torch.empty(self.B * _lib.SC_NRES, dtype=torch.float64, device=self.dev)
+        self.ctx("cms_shape_complementarity", int(precision), _ptr(res))
+        if out is not None:
+            return None
+        return res.cpu().numpy().reshape(self.B, _lib.SC_NRES)
+
+    def max_contact_ms(self):
+        import torch
+        if self.mode is None:
+            self.generate(True)
+        per_atom = torch.empty(max(self.ntot, 1), dtype=torch.float64, device=self.dev)
+        totals = torch.empty(self.B, dtype=torch.float64, device=self.dev)
+        self.ctx("cms_max_contact_ms", _ptr(per_atom), _ptr(totals))
+        return dict(total=totals.cpu().numpy(), per_atom=per_atom[:self.ntot].cpu().numpy())
+
+    def counts(self):
+        out = np.zeros(self.B * _lib.NCOUNTS, dtype=np.int64)
+        self.ctx("cms_counts", ctypes.c_void_p(out.ctypes.data))
+        return out.reshape(self.B, _lib.NCOUNTS)
+
+
+def calculate_batch(complexes, sc=True, precision=_lib.PREC_FP64):
+    """Convenience: CMS (both sides) and optionally SC of every complex."""
+    calc = BatchCalculator()
+    calc.load(complexes)
+    out = calc.contact_ms(precision)
+    if sc:
+        out["sc"] = calc.shape_complementarity(precision)
+    out["per_atom"] = calc.per_atom_split(out["per_atom"])
+    calc.close()
+    return out

@@ -83,7 +83,7 @@ struct cms_ctx {
   DBuf bflag, bscan, blist, bxyz, bgrp, aflag, ascan, alist, axyz, agrpd;
   DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm;
   DBuf bacc, trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
-  DBuf qpack, rpack, grp1, gtot;
+  DBuf qpack, rpack, grp1, gtot, npd2, npidx;
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
   struct Ev { std::string name; hipEvent_t a, b; };
@@ -264,7 +264,7 @@ int cms_destroy(cms_ctx* ctx) {
                   &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
                   &ctx->bacc, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
-                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot};
+                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx};
   for (DBuf* b : bufs) b->release();
   delete ctx;
   return CMS_OK;
@@ -579,19 +579,45 @@ static int prepare_lists(cms_ctx* ctx) {
   return CMS_OK;
 }
 
-// nearest search of query groups qgrp over reference groups rgrp (g ^ xmask)
-static int launch_nearest(cms_ctx* ctx, const double4* Q, const int* qgrp, int nq, const double4* R,
-                          const int* rgrp, int xmask, int G, double* out_d2, int* out_idx) {
+// nearest search of query groups qgrp over reference groups rgrp (g ^ xmask).
+// Small query sets split every group's references over several workgroups
+// so that a single complex still spreads over the GPU.
+static int launch_nearest(cms_ctx* ctx, const double4* Q, const int* qgrp, int nq, int nr_total,
+                          const double4* R, const int* rgrp, int xmask, int G, double* out_d2,
+                          int* out_idx) {
   hipStream_t st = ctx->stream;
   CK(ctx->tcnt.ensure(sizeof(int) * (G + 1)));
   CK(ctx->toff.ensure(sizeof(int) * (G + 1)));
   k_tile_counts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, qgrp, 256, ctx->tcnt.as<int>());
   scan_excl(ctx->tcnt.as<int>(), ctx->toff.as<int>(), G, ctx->part.as<int>(), st);
-  const int grid = grid_for((int64_t)nq / 256 + G, 1, 8192);
-  if (out_idx)
-    TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, out_d2, out_idx));
-  else
-    TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, out_d2, nullptr));
+  const int64_t tiles_est = (int64_t)nq / 256 + G;
+  const int64_t target = 2048;
+  int nsplit = 1;
+  if (tiles_est < target) {
+    nsplit = (int)std::min<int64_t>((target + tiles_est - 1) / tiles_est, 64);
+    // at least 512 references per split on average
+    const int64_t per_group_refs = std::max<int64_t>(1, (int64_t)nr_total / std::max(G, 1));
+    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, per_group_refs / 512));
+  }
+  const int grid = grid_for(tiles_est * nsplit, 1, 16384);
+  if (nsplit == 1) {
+    if (out_idx)
+      TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, 1, 0, out_d2, out_idx));
+    else
+      TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, 1, 0, out_d2, nullptr));
+  } else {
+    const int64_t ps = nq;
+    CK(ctx->npd2.ensure(sizeof(double) * ((int64_t)nsplit * ps + 1)));
+    CK(ctx->npidx.ensure(sizeof(int) * ((int64_t)nsplit * ps + 1)));
+    double* pd2 = ctx->npd2.as<double>();
+    int* pidx = out_idx ? ctx->npidx.as<int>() : nullptr;
+    if (out_idx)
+      TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, nsplit, ps, pd2, pidx));
+    else
+      TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, nsplit, ps, pd2, nullptr));
+    if (nq > 0)
+      k_nearest_combine<<<grid_for(nq, 256, 4096), 256, 0, st>>>(nq, nsplit, ps, pd2, pidx, out_d2, out_idx);
+  }
   CKL();
   return CMS_OK;
 }
@@ -621,7 +647,7 @@ static int owner_sums(cms_ctx* ctx, int nitems, const int* list, const double* v
                                                                   ctx->ostart.as<int>(), ctx->ocur.as<int>(),
                                                                   ctx->operm.as<int>());
   if (n > 0)
-    TIMED("owner_sums", k_owner_sums<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->ostart.as<int>(), ctx->operm.as<int>(), val,
+    TIMED("owner_sums", k_owner_sums<<<grid_for(n, 1, 16384), 64, 0, st>>>(n, ctx->ostart.as<int>(), ctx->operm.as<int>(), val,
                                                           vstride, per_atom));
   CKL();
   // np.sum of the per-atom values of every molecule group; gstride 1 keeps
@@ -647,7 +673,7 @@ int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals
   hipStream_t st = ctx->stream;
   CK(ctx->bmin.ensure(sizeof(double) * (nB + 1)));
   CK(ctx->wv.ensure(sizeof(double) * (nB + 1)));
-  r = launch_nearest(ctx, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), nB, ctx->bxyz.as<double4>(),
+  r = launch_nearest(ctx, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), nB, nB, ctx->bxyz.as<double4>(),
                      ctx->bgrp.as<int>(), 1, G, ctx->bmin.as<double>(), nullptr);
   if (r) return r;
   if (nB > 0)
@@ -685,7 +711,7 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
   CK(ctx->tgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->side.ensure(sizeof(double) * 8 * G));
   // trim: buried dots of g against accessible dots of g
-  r = launch_nearest(ctx, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), nB, ctx->axyz.as<double4>(),
+  r = launch_nearest(ctx, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), nB, ctx->nA, ctx->axyz.as<double4>(),
                      ctx->agrpd.as<int>(), 0, G, ctx->bacc.as<double>(), nullptr);
   if (r) return r;
   if (nB > 0)
@@ -712,7 +738,7 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
                                                              ctx->dnml.as<double4>(), ctx->tlist.as<int>(),
                                                              ctx->txyz.as<double4>(), ctx->tnml.as<double4>());
   CKL();
-  r = launch_nearest(ctx, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), nT, ctx->txyz.as<double4>(),
+  r = launch_nearest(ctx, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), nT, nT, ctx->txyz.as<double4>(),
                      ctx->tgrp.as<int>(), 1, G, ctx->tmin.as<double>(), ctx->tidx.as<int>());
   if (r) return r;
   k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->tgof.as<int>());
@@ -872,7 +898,7 @@ int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const 
   if (nr > 0) k_pack3<<<grid_for(nr, 256, 4096), 256, 0, st>>>((int)nr, r, ctx->rpack.as<double4>());
   int g[4] = {0, (int)nq, 0, (int)nr};
   CK(hipMemcpyAsync(ctx->grp1.p, g, sizeof g, hipMemcpyHostToDevice, st));
-  int rc = launch_nearest(ctx, ctx->qpack.as<double4>(), ctx->grp1.as<int>(), (int)nq, ctx->rpack.as<double4>(),
+  int rc = launch_nearest(ctx, ctx->qpack.as<double4>(), ctx->grp1.as<int>(), (int)nq, (int)nr, ctx->rpack.as<double4>(),
                           ctx->grp1.as<int>() + 2, 0, 1, out_d2, out_idx);
   if (rc) return rc;
   CK(hipStreamSynchronize(st));

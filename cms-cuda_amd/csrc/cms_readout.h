@@ -12,7 +12,10 @@ __global__ void k_dot_compact(int n, const int* flag, const int* scan, const dou
 __global__ void k_tile_counts(int G, const int* qgrp, int tq, int* cnt);
 template <bool IDX>
 __global__ void k_nearest64(const double4* Q, const int* qgrp, const double4* R, const int* rgrp,
-                            int xmask, const int* toff, int G, double* out_d2, int* out_idx);
+                            int xmask, const int* toff, int G, int nsplit, int64_t pstride,
+                            double* out_d2, int* out_idx);
+__global__ void k_nearest_combine(int nq, int nsplit, int64_t pstride, const double* pd2, const int* pidx,
+                                  double* out_d2, int* out_idx);
 __global__ void k_cms_weights(int n, const double4* bxyz, const double* d2, double weight, double* w);
 __global__ void k_owner_count(int n, const int* list, const int* datom, int* cnt);
 __global__ void k_owner_scatter(int n, const int* list, const int* datom, const int* start, int* cur,

@@ -34,16 +34,23 @@ __global__ void k_tile_counts(int G, const int* __restrict__ qgrp, int tq, int* 
 // Nearest reference point of every query, FP64.  Query group g searches the
 // reference group g ^ xmask.  d^2 = (dx^2 + dy^2) + dz^2 as in
 // scipy cdist(sqeuclidean); ties keep the first (lowest) reference index,
-// as np.argmin does.  Reference points are staged in LDS, 256 at a time.
+// as np.argmin does.  Work item = (query tile of 256, reference split s of
+// nsplit): split s scans the s-th contiguous slice of the group's
+// references, staged in LDS 256 at a time.  With nsplit > 1 every split
+// writes a partial (d^2, index) and k_nearest_combine takes the minimum in
+// split order, so the first index still wins ties.
 constexpr int NT = 256;
 template <bool IDX>
 __global__ __launch_bounds__(NT) void k_nearest64(const double4* __restrict__ Q, const int* __restrict__ qgrp,
                                                   const double4* __restrict__ R, const int* __restrict__ rgrp,
-                                                  int xmask, const int* __restrict__ toff, int G,
-                                                  double* __restrict__ out_d2, int* __restrict__ out_idx) {
+                                                  int xmask, const int* __restrict__ toff, int G, int nsplit,
+                                                  int64_t pstride, double* __restrict__ out_d2,
+                                                  int* __restrict__ out_idx) {
   __shared__ double rx[NT], ry[NT], rz[NT];
-  const int ntiles = toff[G];
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int64_t nwork = (int64_t)toff[G] * nsplit;
+  for (int64_t wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
+    const int tile = (int)(wk / nsplit);
+    const int sp = (int)(wk - (int64_t)tile * nsplit);
     int lo = 0, hi = G;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
@@ -53,7 +60,9 @@ __global__ __launch_bounds__(NT) void k_nearest64(const double4* __restrict__ Q,
     const int q = qgrp[g] + (tile - toff[g]) * NT + threadIdx.x;
     const bool live = q < qgrp[g + 1];
     const int h = g ^ xmask;
-    const int r0 = rgrp[h], r1 = rgrp[h + 1];
+    const int rg0 = rgrp[h], rn = rgrp[h + 1] - rg0;
+    const int r0 = rg0 + (int)(((int64_t)rn * sp) / nsplit);
+    const int r1 = rg0 + (int)(((int64_t)rn * (sp + 1)) / nsplit);
     double qx = 0.0, qy = 0.0, qz = 0.0;
     if (live) {
       const double4 a = Q[q];
@@ -73,14 +82,33 @@ __global__ __launch_bounds__(NT) void k_nearest64(const double4* __restrict__ Q,
       for (int t = 0; t < n; ++t) {
         const double dx = rx[t] - qx, dy = ry[t] - qy, dz = rz[t] - qz;
         const double d2 = (dx * dx + dy * dy) + dz * dz;
-        if (d2 < best) { best = d2; bi = rb - r0 + t; }
+        if (d2 < best) { best = d2; bi = rb - rg0 + t; }
       }
     }
     if (live) {
-      out_d2[q] = best;
-      if (IDX) out_idx[q] = bi;
+      const int64_t o = (int64_t)sp * pstride + q;
+      out_d2[o] = best;
+      if (IDX) out_idx[o] = bi;
     }
     __syncthreads();
+  }
+}
+
+__global__ void k_nearest_combine(int nq, int nsplit, int64_t pstride, const double* __restrict__ pd2,
+                                  const int* __restrict__ pidx, double* __restrict__ out_d2,
+                                  int* __restrict__ out_idx) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    double best = INFINITY;
+    int bi = -1;
+    for (int s = 0; s < nsplit; ++s) {
+      const double d = pd2[(int64_t)s * pstride + q];
+      if (d < best) {
+        best = d;
+        if (pidx) bi = pidx[(int64_t)s * pstride + q];
+      }
+    }
+    out_d2[q] = best;
+    if (out_idx) out_idx[q] = bi;
   }
 }
 
@@ -110,22 +138,60 @@ __global__ void k_owner_scatter(int n, const int* __restrict__ list, const int* 
     perm[start[a] + atomicAdd(&cur[a], 1)] = i;
   }
 }
-__global__ void k_owner_sums(int natoms, const int* __restrict__ start, int* __restrict__ perm,
-                             const double* __restrict__ val, int64_t vstride, double* __restrict__ out) {
-  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < natoms; a += gridDim.x * blockDim.x) {
-    const int lo = start[a], hi = start[a + 1];
-    for (int i = lo + 1; i < hi; ++i) {
-      const int v = perm[i];
-      int j = i - 1;
-      while (j >= lo && perm[j] > v) {
-        perm[j + 1] = perm[j];
-        --j;
-      }
-      perm[j + 1] = v;
+// One wavefront per atom: the atom's item indices (scattered in arbitrary
+// order by k_owner_scatter) are sorted in LDS with a bitonic network, their
+// values gathered in that order and summed left to right by lane 0.
+constexpr int OWN_CAP = 1024;
+__global__ __launch_bounds__(64) void k_owner_sums(int natoms, const int* __restrict__ start,
+                                                   int* __restrict__ perm, const double* __restrict__ val,
+                                                   int64_t vstride, double* __restrict__ out) {
+  __shared__ int key[OWN_CAP];
+  __shared__ double v[OWN_CAP];
+  for (int a = blockIdx.x; a < natoms; a += gridDim.x) {
+    const int lo = start[a], n = start[a + 1] - lo;
+    if (n <= 1) {
+      if (lane() == 0) out[a] = n == 1 ? 0.0 + val[(int64_t)perm[lo] * vstride] : 0.0;
+      continue;
     }
-    double s = 0.0;
-    for (int i = lo; i < hi; ++i) s += val[(int64_t)perm[i] * vstride];
-    out[a] = s;
+    if (n > OWN_CAP) {
+      // very long segment: in-place insertion sort by one lane
+      if (lane() == 0) {
+        for (int i = lo + 1; i < lo + n; ++i) {
+          const int x = perm[i];
+          int j = i - 1;
+          while (j >= lo && perm[j] > x) { perm[j + 1] = perm[j]; --j; }
+          perm[j + 1] = x;
+        }
+        double s = 0.0;
+        for (int i = lo; i < lo + n; ++i) s += val[(int64_t)perm[i] * vstride];
+        out[a] = s;
+      }
+      continue;
+    }
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int k = lane(); k < P; k += WAVE) key[k] = k < n ? perm[lo + k] : 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = lane(); t < P / 2; t += WAVE) {
+          const int i = 2 * t - (t & (stride - 1));
+          const int j = i + stride;
+          const bool up = (i & size) == 0;
+          const int x = key[i], y = key[j];
+          if ((x > y) == up) { key[i] = y; key[j] = x; }
+        }
+        __syncthreads();
+      }
+    }
+    for (int k = lane(); k < n; k += WAVE) v[k] = val[(int64_t)key[k] * vstride];
+    __syncthreads();
+    if (lane() == 0) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += v[k];
+      out[a] = s;
+    }
+    __syncthreads();
   }
 }
 
@@ -458,8 +524,8 @@ __global__ void k_sc_combine(int B, const double* __restrict__ side, double* __r
 }
 
 template __global__ void k_nearest64<false>(const double4*, const int*, const double4*, const int*, int,
-                                            const int*, int, double*, int*);
+                                            const int*, int, int, int64_t, double*, int*);
 template __global__ void k_nearest64<true>(const double4*, const int*, const double4*, const int*, int,
-                                           const int*, int, double*, int*);
+                                           const int*, int, int, int64_t, double*, int*);
 
 }  // namespace cms

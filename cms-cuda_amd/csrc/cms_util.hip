@@ -66,8 +66,42 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restri
   if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = part[nb];
 }
 
+// Small arrays: one workgroup of 1024 threads, each owning a contiguous run
+// of at most 64 elements.
+constexpr int SS_THREADS = 1024;
+constexpr int64_t SS_MAX = 64 * SS_THREADS;
+__global__ __launch_bounds__(SS_THREADS) void k_scan_single(const int* __restrict__ in, int* __restrict__ out,
+                                                            int64_t n) {
+  __shared__ int wsum[SS_THREADS / WAVE];
+  const int64_t per = (n + SS_THREADS - 1) / SS_THREADS;
+  const int64_t lo = threadIdx.x * per;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  int acc = 0;
+  for (int64_t i = lo; i < hi; ++i) acc += in[i];
+  const int l = lane(), wid = threadIdx.x / WAVE;
+  const int inc = wave_incl_scan(acc);
+  if (l == WAVE - 1) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int k = 0; k < SS_THREADS / WAVE; ++k) {
+    if (k < wid) off += wsum[k];
+    tot += wsum[k];
+  }
+  int run = off + inc - acc;
+  for (int64_t i = lo; i < hi; ++i) {
+    const int v = in[i];
+    out[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) out[n] = tot;
+}
+
 void scan_excl(const int* in, int* out, int64_t n, int* part, hipStream_t st) {
   // part must hold at least 1025 ints
+  if (n <= SS_MAX) {
+    k_scan_single<<<1, SS_THREADS, 0, st>>>(in, out, n);
+    return;
+  }
   int nb = (int)((n + 4095) / 4096);
   if (nb < 1) nb = 1;
   if (nb > 1024) nb = 1024;

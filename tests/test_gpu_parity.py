@@ -222,3 +222,34 @@ def test_deterministic(cms_cuda, golden_complex):
     assert r1[0] == r2[0] and np.array_equal(r1[1], r2[1])
     d1, d2 = r1[2].run.dots[1], r2[2].run.dots[1]
     assert np.array_equal(d1.coor_xyz, d2.coor_xyz) and np.array_equal(d1.area, d2.area)
+
+
+def test_batch_matches_single(cms_cuda, golden_complex):
+    """A batch of uneven complexes (ligand, small, golden, radius-0) gives
+    results bit-identical to running each complex alone."""
+    from cms_cuda import synth
+    from cms_cuda.batch import BatchCalculator
+    say("batch vs single")
+    g = golden_complex
+    cs = [(g["binder_xyz"], g["binder_radii"], g["target_xyz"], g["target_radii"])]
+    for seed, (nt, nb) in enumerate([(300, 30), (500, 120), (250, 250)]):
+        c = synth.make_complex(nt, nb, 100 + seed, gap=3.5)
+        cs.append((c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"]))
+    r0 = g["binder_radii"].copy()
+    r0[::4] = 0.0
+    cs.append((g["binder_xyz"], r0, g["target_xyz"], g["target_radii"]))
+    bc = BatchCalculator()
+    bc.load(cs)
+    out = bc.contact_ms()
+    sc = bc.shape_complementarity()
+    per = bc.per_atom_split(out["per_atom"])
+    for k, c in enumerate(cs):
+        calc = cms_cuda.MolecularSurfaceCalculator()
+        calc.add_binder_and_target(*c)
+        cms_t, per_t = calc.CalcLoaded()
+        cms_b, per_b = calc.calc_contact_molecular_surface(target_side=False)
+        s = calc.CalcLoadedSC()
+        assert out["target"][k] == cms_t and out["binder"][k] == cms_b, k
+        assert np.array_equal(per[k][1], per_t) and np.array_equal(per[k][0], per_b), k
+        assert tuple(sc[k, :3]) == s, k
+    bc.close()
