@@ -38,6 +38,9 @@ def log(msg):
 
 def make_workload(name, batch, seed0):
     from cms_cuda import synth
+    if name == "mixed":
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        return synth.mixed_batch(batch, seed=seed0, workers=workers)
     cs = []
     for b in range(batch):
         if name == "c1":
@@ -61,7 +64,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "mixed"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--batch", type=int, default=1)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -100,11 +104,13 @@ def main():
     nt_c = (ctypes.c_int32 * B)(*nt.tolist())
     P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
 
+    prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_FP32_RESCORE
+
     def step():
         ctx("cms_load", B, nb_c, nt_c, P(d_xyz), P(d_rad), None, ctypes.byref(st))
         ctx("cms_generate", 0, 0)
-        ctx("cms_contact_ms", _lib.PREC_FP64, P(per_atom), P(totals))
-        ctx("cms_shape_complementarity", _lib.PREC_FP64, P(scres))
+        ctx("cms_contact_ms", prec, P(per_atom), P(totals))
+        ctx("cms_shape_complementarity", prec, P(scres))
 
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
@@ -116,7 +122,7 @@ def main():
     torch.cuda.synchronize()
     names = ["attention", "region", "neighbors", "pairs_count", "pairs_emit", "probes_count",
              "probes_emit", "near", "convex_count", "convex_emit", "toroid_count", "toroid_emit",
-             "concave_count", "concave_emit", "nearest", "owner_sums", "sc_sides"]
+             "concave_count", "concave_emit", "nearest", "nearest32", "owner_sums", "sc_sides"]
     breakdown = {}
     for nm in names:
         ms = ctypes.c_double()
@@ -197,11 +203,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32-search+f64",
         "data": "synthetic",
-        "config": {"workload": "%s plain FP64 CMS(both sides)+SC, %d complex(es)/step, binder %s + target %s atoms"
-                               % (args.workload, B, sorted(set(nb.tolist())), sorted(set(nt.tolist()))),
-                   "precision": "fp64", "batch": B},
+        "config": {"workload": "%s %s CMS(both sides)+SC, %d complex(es)/step, binder %d-%d + target %d-%d atoms"
+                               % (args.workload, args.precision, B, nb.min(), nb.max(), nt.min(), nt.max()),
+                   "precision": args.precision, "batch": B},
         "roofline": {"bound": bound, "kernel": args.kernel, "achieved": achieved, "peak": peak,
                      "unit": unit, "frac": achieved / peak, "traffic": None,
                      "avg_launch_us": avg_s * 1e6, "launches": cnt.value, "work_per_launch": work},

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.normpath(os.path.join(HERE, "..", "csrc"))
 INCLUDE = os.path.normpath(os.path.join(HERE, "..", "..", "include"))
 LIB_PATH = os.path.join(HERE, "libcms_hip.so")
-SOURCES = ["cms_surface.hip", "cms_readout.hip", "cms_util.hip", "cms_search32.hip",
+SOURCES = ["cms_cells.hip", "cms_surface.hip", "cms_readout.hip", "cms_util.hip", "cms_search32.hip",
            "cms_backward.hip", "cms_host.hip"]
 ARCH = os.environ.get("CMS_HIP_ARCH", "gfx950")
 

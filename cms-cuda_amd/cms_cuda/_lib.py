@@ -56,6 +56,7 @@ _PROTOS = {
     "cms_copy_probes": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp]),
     "cms_nearest": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),
     "cms_profile": (_c.c_int, [_vp, _c.c_char_p]),
+    "cms_search_stats": (_c.c_int, [_vp, _c.POINTER(_c.c_uint64)]),
     "cms_profile_read": (_c.c_int, [_vp, _c.c_char_p, _c.POINTER(_c.c_double), _c.POINTER(_i64)]),
 }
 OPTIONAL = {

@@ -72,16 +72,19 @@ def ellipsoid_axes(n, density, ratio):
     return np.array([a, a, ratio * a])
 
 
-def poisson_disk(n, axes, rng, spacing=MIN_SPACING, max_rounds=2000):
+def poisson_disk(n, axes, rng, spacing=MIN_SPACING, max_rounds=5000):
     """n points inside the axis-aligned ellipsoid with pairwise distance >=
-    spacing, by random sequential addition in batches."""
+    spacing.  Random sequential addition in rounds: each round proposes a
+    batch of uniform points, drops those within spacing of an accepted point,
+    and within the batch keeps a proposal only if no earlier proposal of the
+    batch is within spacing (so every round stays vectorised)."""
     pts = np.zeros((0, 3))
     tree = None
     for _ in range(max_rounds):
         need = n - len(pts)
         if need <= 0:
             break
-        m = max(64, 4 * need)
+        m = max(64, 2 * need)
         cand = rng.uniform(-1.0, 1.0, size=(m, 3))
         cand = cand[np.sum(cand * cand, axis=1) <= 1.0] * axes
         if tree is not None and len(cand):
@@ -89,21 +92,11 @@ def poisson_disk(n, axes, rng, spacing=MIN_SPACING, max_rounds=2000):
             cand = cand[d >= spacing]
         if len(cand) == 0:
             continue
-        # greedy sequential acceptance among the batch, in proposal order
-        ctree = cKDTree(cand)
-        pairs = ctree.query_pairs(spacing, output_type="ndarray")
-        blocked = np.zeros(len(cand), dtype=bool)
+        pairs = cKDTree(cand).query_pairs(spacing, output_type="ndarray")
+        keep = np.ones(len(cand), dtype=bool)
         if len(pairs):
-            nbr = [[] for _ in range(len(cand))]
-            for a, b in pairs:
-                lo, hi = (a, b) if a < b else (b, a)
-                nbr[lo].append(hi)
-            for i in range(len(cand)):
-                if blocked[i]:
-                    continue
-                for j in nbr[i]:
-                    blocked[j] = True
-        acc = cand[~blocked][:need]
+            keep[np.maximum(pairs[:, 0], pairs[:, 1])] = False
+        acc = cand[keep][:need]
         pts = np.concatenate([pts, acc])
         tree = cKDTree(pts)
     if len(pts) < n:
@@ -122,7 +115,7 @@ def random_rotation(rng):
     ])
 
 
-def point_ellipsoid_distance(p, axes, iters=60):
+def point_ellipsoid_distance(p, axes, iters=40):
     """Exact distance from points outside an axis-aligned ellipsoid to its
     surface.  Newton on the Lagrange multiplier t of the closest point
     a_i^2 p_i / (t + a_i^2): f(t) = sum (a_i p_i / (t + a_i^2))^2 - 1 is
@@ -132,14 +125,15 @@ def point_ellipsoid_distance(p, axes, iters=60):
     t = np.zeros(len(p))
     for _ in range(iters):
         q = axes * p / (t[:, None] + a2)
-        f = np.sum(q * q, axis=1) - 1.0
-        df = -2.0 * np.sum(q * q / (t[:, None] + a2), axis=1)
+        qq = q * q
+        f = qq.sum(axis=1) - 1.0
+        df = -2.0 * (qq / (t[:, None] + a2)).sum(axis=1)
         t = t - f / df
     close = a2 * p / (t[:, None] + a2)
     return np.linalg.norm(p - close, axis=1)
 
 
-def _surface_points(axes, n=6000):
+def _surface_points(axes, n=3000):
     k = np.arange(n) + 0.5
     phi = np.arccos(1 - 2 * k / n)
     th = np.pi * (1 + 5 ** 0.5) * k
@@ -149,8 +143,11 @@ def _surface_points(axes, n=6000):
 
 def place_binder(t_axes, b_axes, rot, direction, gap):
     """Offset along direction at which the binder ellipsoid (rotated by rot)
-    is gap away from the target ellipsoid."""
+    is gap away from the target ellipsoid: bisection (to 1e-9 A) on the
+    minimum over the binder-surface sample facing the target of the exact
+    point-to-ellipsoid distance."""
     surf = _surface_points(b_axes) @ rot.T
+    surf = surf[surf @ direction < 0.25 * np.linalg.norm(surf, axis=1)]
 
     def surface_gap(off):
         p = surf + off * direction
@@ -224,7 +221,20 @@ def batch_sizes(n, seed, binder_lo=20, binder_hi=1800, target_lo=800, target_hi=
     return nt, nb, gaps
 
 
-def mixed_batch(n, seed=3, **size_kw):
+def _make_one(args):
+    nt, nb, seed, gap = args
+    return make_complex(nt, nb, seed, gap=gap)
+
+
+def mixed_batch(n, seed=3, workers=1, **size_kw):
+    """n complexes with batch_sizes(); workers > 1 generates them in a
+    process pool (spawned, so it is safe after GPU initialisation)."""
     nt, nb, gaps = batch_sizes(n, seed, **size_kw)
-    return [make_complex(int(nt[i]), int(nb[i]), seed * 100003 + i, gap=float(gaps[i]))
-            for i in range(n)]
+    jobs = [(int(nt[i]), int(nb[i]), seed * 100003 + i, float(gaps[i])) for i in range(n)]
+    if workers <= 1 or n < 4:
+        return [_make_one(j) for j in jobs]
+    import concurrent.futures
+    import multiprocessing
+    ctx = multiprocessing.get_context("spawn")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+        return list(ex.map(_make_one, jobs, chunksize=max(1, n // (4 * workers))))

@@ -1,0 +1,417 @@
+// Cell lists: per-complex uniform grids used by the attention sweep, the
+// coincident-atom check, the interface region and the neighbour rows.
+//
+// Cell edge h >= max(sep, 2 r_max + 2 rp) of the complex (slightly padded so
+// that rounding of the cell index cannot push two atoms closer than h into
+// non-adjacent cells), so every partner an atom can have -- other-molecule
+// atoms closer than sep, same-molecule atoms inside the bridge distance
+// r_i + r_j + 2rp, coincident atoms -- lies in the 27 cells around it.
+// Results do not depend on the order atoms are visited in: minima, "any"
+// tests, atomic minima of (i, j) keys, and rows sorted afterwards.
+#include "cms_cells.h"
+#include "cms_device.h"
+#include "cms_internal.h"
+
+namespace cms {
+
+// Per complex: bounding box, cell edge, grid dims (capped at 8 N + 64 cells
+// by growing the edge), and the FP32-search frame (centre, R).
+__global__ __launch_bounds__(256) void k_grid_setup(int B, const int* __restrict__ aoff,
+                                                    const double4* __restrict__ atom, Settings s,
+                                                    GridInfo* __restrict__ grid, double4* __restrict__ frame,
+                                                    int* __restrict__ ncell) {
+  __shared__ double red[7][256];
+  for (int c = blockIdx.x; c < B; c += gridDim.x) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double rmax = 0.0;
+    for (int i = aoff[c] + threadIdx.x; i < aoff[c + 1]; i += blockDim.x) {
+      const double4 a = atom[i];
+      mn[0] = fmin(mn[0], a.x); mn[1] = fmin(mn[1], a.y); mn[2] = fmin(mn[2], a.z);
+      mx[0] = fmax(mx[0], a.x); mx[1] = fmax(mx[1], a.y); mx[2] = fmax(mx[2], a.z);
+      rmax = fmax(rmax, a.w);
+    }
+    for (int k = 0; k < 3; ++k) { red[k][threadIdx.x] = mn[k]; red[3 + k][threadIdx.x] = mx[k]; }
+    red[6][threadIdx.x] = rmax;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+        for (int k = 3; k < 7; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const int n = aoff[c + 1] - aoff[c];
+      GridInfo gi;
+      double lo[3] = {red[0][0], red[1][0], red[2][0]};
+      double hi[3] = {red[3][0], red[4][0], red[5][0]};
+      if (n == 0 || !(hi[0] >= lo[0]) || !(hi[1] >= lo[1]) || !(hi[2] >= lo[2])) {
+        lo[0] = lo[1] = lo[2] = 0.0;
+        hi[0] = hi[1] = hi[2] = 0.0;
+      }
+      const double rm = red[6][0];
+      double h = fmax(s.sep, 2.0 * rm + 2.0 * s.rp);
+      h = h * (1.0 + 1e-9) + 1e-9;
+      if (!(h > 0.0)) h = 1.0;
+      int d[3];
+      const int64_t cap = 8ll * n + 64;
+      for (int it = 0; it < 64; ++it) {
+        int64_t tot = 1;
+        for (int k = 0; k < 3; ++k) {
+          const double e = (hi[k] - lo[k]) / h;
+          d[k] = e < 1e6 ? (int)floor(e) + 1 : 1000001;
+          tot *= d[k];
+        }
+        if (tot <= cap) break;
+        h *= 1.5;
+      }
+      gi.x0 = lo[0]; gi.y0 = lo[1]; gi.z0 = lo[2]; gi.h = h;
+      gi.nx = d[0]; gi.ny = d[1]; gi.nz = d[2];
+      gi.cell0 = 0;
+      grid[c] = gi;
+      ncell[c] = d[0] * d[1] * d[2];
+      const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+      const double half = fmax(fmax(hi[0] - cx, hi[1] - cy), hi[2] - cz);
+      frame[c] = make_double4(cx, cy, cz, half + rm + 2.0 * s.rp + 1.0);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_grid_offsets(int B, const int* __restrict__ cscan, GridInfo* __restrict__ grid) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < B; c += gridDim.x * blockDim.x)
+    grid[c].cell0 = cscan[c];
+}
+
+__device__ __forceinline__ void cell_coords(const GridInfo& gi, double x, double y, double z, int* ix, int* iy,
+                                            int* iz) {
+  *ix = min(max((int)floor((x - gi.x0) / gi.h), 0), gi.nx - 1);
+  *iy = min(max((int)floor((y - gi.y0) / gi.h), 0), gi.ny - 1);
+  *iz = min(max((int)floor((z - gi.z0) / gi.h), 0), gi.nz - 1);
+}
+
+__global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int* __restrict__ acell,
+                             int* __restrict__ ccount) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < b.ntot; g += gridDim.x * blockDim.x) {
+    const GridInfo gi = grid[b.acplx[g]];
+    const double4 a = b.atom[g];
+    int ix, iy, iz;
+    cell_coords(gi, a.x, a.y, a.z, &ix, &iy, &iz);
+    const int cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+    acell[g] = cell;
+    atomicAdd(&ccount[cell], 1);
+  }
+}
+
+__global__ void k_cell_scatter(int n, const int* __restrict__ acell, const int* __restrict__ cstart,
+                               int* __restrict__ cursor, int* __restrict__ catom) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    const int cell = acell[g];
+    catom[cstart[cell] + atomicAdd(&cursor[cell], 1)] = g;
+  }
+}
+
+// Visits the (up to) 27 cells around cell (ix, iy, iz) of grid gi.
+#define FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cellvar)                                   \
+  for (int dx_ = -1; dx_ <= 1; ++dx_)                                                \
+    for (int dy_ = -1; dy_ <= 1; ++dy_)                                              \
+      for (int dz_ = -1; dz_ <= 1; ++dz_)                                            \
+        if ((ix) + dx_ >= 0 && (ix) + dx_ < (gi).nx && (iy) + dy_ >= 0 &&            \
+            (iy) + dy_ < (gi).ny && (iz) + dz_ >= 0 && (iz) + dz_ < (gi).nz)         \
+          for (int cellvar = (gi).cell0 + (((ix) + dx_) * (gi).ny + (iy) + dy_) * (gi).nz + (iz) + dz_, \
+                   once_ = 1; once_; once_ = 0)
+
+// Flattened candidate list of the 27 cells around (ix, iy, iz): cbeg[k] /
+// cpre[k] are the start in catom and the prefix count of cell k; cpre[27]
+// is the total.  Wave-cooperative (lanes 0..26 each own a cell).
+__device__ __forceinline__ void cells_around(const GridInfo& gi, int ix, int iy, int iz,
+                                             const int* __restrict__ cstart, int* cbeg, int* cpre) {
+  const int k = lane();
+  int cnt = 0, beg = 0;
+  if (k < 27) {
+    const int cx = ix + k / 9 - 1, cy = iy + (k / 3) % 3 - 1, cz = iz + k % 3 - 1;
+    if (cx >= 0 && cx < gi.nx && cy >= 0 && cy < gi.ny && cz >= 0 && cz < gi.nz) {
+      const int cell = gi.cell0 + (cx * gi.ny + cy) * gi.nz + cz;
+      beg = cstart[cell];
+      cnt = cstart[cell + 1] - beg;
+    }
+  }
+  const int inc = wave_incl_scan(cnt);
+  if (k < 27) { cbeg[k] = beg; cpre[k] = inc - cnt; }
+  if (k == 26) cpre[27] = inc;
+  __syncthreads();
+}
+
+__device__ __forceinline__ int cell_candidate(const int* cbeg, const int* cpre, const int* __restrict__ catom,
+                                              int idx) {
+  int lo = 0, hi = 27;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cpre[mid] <= idx) lo = mid; else hi = mid;
+  }
+  return catom[cbeg[lo] + idx - cpre[lo]];
+}
+
+// Attention (_core.py:1150-1195) and coincident atoms (_core.py:1285-1293):
+// one wavefront per atom, lanes over the atom's 27-cell candidates; atoms
+// visited in cell order so that consecutive workgroups read the same cells.
+__global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s, int all_atoms,
+                                                        const GridInfo* __restrict__ grid,
+                                                        const int* __restrict__ cstart,
+                                                        const int* __restrict__ catom,
+                                                        int8_t* __restrict__ atten, int8_t* __restrict__ access,
+                                                        unsigned long long* __restrict__ coinc,
+                                                        int* __restrict__ status) {
+  __shared__ int cbeg[27], cpre[28];
+  CMS_UNIT_LOOP(sidx, b.ntot) {
+    const int g = catom[sidx];
+    const int c = b.acplx[g];
+    const int a0 = b.aoff[c];
+    const int m = b.amol[g];
+    const GridInfo gi = grid[c];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    int ix, iy, iz;
+    cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+    cells_around(gi, ix, iy, iz, cstart, cbeg, cpre);
+    const int T = cpre[27];
+    double mn = INFINITY;
+    unsigned long long ckey = ~0ull;
+    for (int idx = lane(); idx < T; idx += WAVE) {
+      const int j = cell_candidate(cbeg, cpre, catom, idx);
+      const V3 d = xi - ld3(b.atom[j]);
+      const double d2 = dot_seq(d, d);
+      if (b.amol[j] != m) {
+        mn = fmin(mn, d2);
+      } else if (j != g && d2 <= 0.0001) {
+        const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
+        ckey = ckey < key ? ckey : key;
+      }
+    }
+    mn = wave_min_double(mn);
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long t = __shfl_xor(ckey, o, WAVE);
+      ckey = ckey < t ? ckey : t;
+    }
+    if (lane() == 0) {
+      const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
+      atten[g] = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
+      access[g] = 0;
+      if (ckey != ~0ull) {
+        atomicMin(&coinc[c], ckey);
+        atomicOr(&status[c], ST_COINCIDENT);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Interface region: non-blockers and atoms within the bridge distance of a
+// same-molecule non-blocker (every atom when full != 0).
+__global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, int full,
+                                                     const GridInfo* __restrict__ grid,
+                                                     const int* __restrict__ cstart,
+                                                     const int* __restrict__ catom,
+                                                     const int8_t* __restrict__ atten, int* __restrict__ need) {
+  __shared__ int cbeg[27], cpre[28];
+  CMS_UNIT_LOOP(sidx, b.ntot) {
+    const int g = catom[sidx];
+    if (full || atten[g] > ATTEN_BLOCKER) {
+      if (lane() == 0) need[g] = 1;
+      continue;
+    }
+    const int c = b.acplx[g];
+    const int m = b.amol[g];
+    const GridInfo gi = grid[c];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    int ix, iy, iz;
+    cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+    cells_around(gi, ix, iy, iz, cstart, cbeg, cpre);
+    const int T = cpre[27];
+    bool keep = false;
+    for (int i0 = 0; i0 < T && !keep; i0 += WAVE) {
+      const int idx = i0 + lane();
+      bool hit = false;
+      if (idx < T) {
+        const int j = cell_candidate(cbeg, cpre, catom, idx);
+        if (b.amol[j] == m && atten[j] > ATTEN_BLOCKER) {
+          const double4 aj = b.atom[j];
+          const V3 d = xi - ld3(aj);
+          const double br = (ai.w + aj.w) + 2.0 * s.rp;
+          hit = dot_seq(d, d) < br * br;
+        }
+      }
+      keep = wave_any(hit);
+    }
+    if (lane() == 0) need[g] = keep ? 1 : 0;
+    __syncthreads();
+  }
+}
+
+// Thread-per-atom variants for large batches: consecutive threads take
+// consecutive cell-sorted atoms, so a wavefront reads a handful of cells
+// that stay in L1/L2 instead of every lane's own neighbourhood.
+__global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings s, int all_atoms,
+                                                           const GridInfo* __restrict__ grid,
+                                                           const int* __restrict__ cstart,
+                                                           const int* __restrict__ catom,
+                                                           int8_t* __restrict__ atten, int8_t* __restrict__ access,
+                                                           unsigned long long* __restrict__ coinc,
+                                                           int* __restrict__ status) {
+  for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
+    const int g = catom[sidx];
+    const int c = b.acplx[g];
+    const int a0 = b.aoff[c];
+    const int m = b.amol[g];
+    const GridInfo gi = grid[c];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    int ix, iy, iz;
+    cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+    double mn = INFINITY;
+    unsigned long long ckey = ~0ull;
+    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cell) {
+      for (int t = cstart[cell]; t < cstart[cell + 1]; ++t) {
+        const int j = catom[t];
+        const V3 d = xi - ld3(b.atom[j]);
+        const double d2 = dot_seq(d, d);
+        if (b.amol[j] != m) {
+          mn = fmin(mn, d2);
+        } else if (j != g && d2 <= 0.0001) {
+          const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
+          ckey = ckey < key ? ckey : key;
+        }
+      }
+    }
+    const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
+    atten[g] = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
+    access[g] = 0;
+    if (ckey != ~0ull) {
+      atomicMin(&coinc[c], ckey);
+      atomicOr(&status[c], ST_COINCIDENT);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s, int full,
+                                                        const GridInfo* __restrict__ grid,
+                                                        const int* __restrict__ cstart,
+                                                        const int* __restrict__ catom,
+                                                        const int8_t* __restrict__ atten, int* __restrict__ need) {
+  for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
+    const int g = catom[sidx];
+    int keep = full || atten[g] > ATTEN_BLOCKER;
+    if (!keep) {
+      const int c = b.acplx[g];
+      const int m = b.amol[g];
+      const GridInfo gi = grid[c];
+      const double4 ai = b.atom[g];
+      const V3 xi = ld3(ai);
+      int ix, iy, iz;
+      cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+      FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cell) {
+        for (int t = cstart[cell]; t < cstart[cell + 1] && !keep; ++t) {
+          const int j = catom[t];
+          if (b.amol[j] != m || atten[j] <= ATTEN_BLOCKER) continue;
+          const double4 aj = b.atom[j];
+          const V3 d = xi - ld3(aj);
+          const double br = (ai.w + aj.w) + 2.0 * s.rp;
+          if (dot_seq(d, d) < br * br) keep = 1;
+        }
+      }
+    }
+    need[g] = keep;
+  }
+}
+
+// Neighbour rows (_core.py:1251-1349) from the 27 cells: one wavefront per
+// region atom; the cells' atom ranges are flattened across lanes; rows are
+// sorted by (d^2, j), buried rows by j.
+__global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s, Work w,
+                                                        const GridInfo* __restrict__ grid,
+                                                        const int* __restrict__ cstart,
+                                                        const int* __restrict__ catom) {
+  __shared__ int cj[KCAP];
+  __shared__ double cd[KCAP];
+  __shared__ int bj[BCAP];
+  __shared__ int cbeg[27], cpre[28];
+  const int nR = *w.nR;
+  CMS_UNIT_LOOP(r, nR) {
+    const int g = w.ratom[r];
+    const int c = b.acplx[g];
+    const int m = b.amol[g];
+    const GridInfo gi = grid[c];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    int ix, iy, iz;
+    cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+    cells_around(gi, ix, iy, iz, cstart, cbeg, cpre);
+    const int T = cpre[27];
+    int cnt = 0, cntb = 0, nbmax = -1;
+    for (int i0 = 0; i0 < T; i0 += WAVE) {
+      const int idx = i0 + lane();
+      bool isn = false, isb = false;
+      double d2 = 0.0;
+      int j = -1;
+      if (idx < T) {
+        j = cell_candidate(cbeg, cpre, catom, idx);
+        const double4 aj = b.atom[j];
+        const V3 d = xi - ld3(aj);
+        d2 = dot_seq(d, d);
+        const double br = (ai.w + aj.w) + 2.0 * s.rp;
+        const bool inb = d2 < br * br;
+        if (b.amol[j] == m) {
+          isn = inb && j != g;
+          if (isn && w.atten[j] > ATTEN_BLOCKER) nbmax = max(nbmax, j);
+        } else {
+          isb = inb && w.atten[j] >= ATTEN_BURIED;
+        }
+      }
+      const uint64_t mn = ballot(isn), mb = ballot(isb);
+      if (isn) {
+        const int p = cnt + lanes_below(mn);
+        if (p < KCAP) { cj[p] = j; cd[p] = d2; }
+      }
+      if (isb) {
+        const int p = cntb + lanes_below(mb);
+        if (p < BCAP) bj[p] = j;
+      }
+      cnt += __popcll(mn);
+      cntb += __popcll(mb);
+    }
+    __syncthreads();
+    if (cnt > KCAP || cntb > BCAP) {
+      if (lane() == 0) atomicOr(&w.status[c], cnt > KCAP ? ST_NBR_CAP : ST_BUR_CAP);
+      cnt = min(cnt, KCAP);
+      cntb = min(cntb, BCAP);
+    }
+    int* row = w.nbr + (int64_t)r * KCAP;
+    for (int p = lane(); p < cnt; p += WAVE) {
+      const double dp = cd[p];
+      const int jp = cj[p];
+      int rank = 0;
+      for (int t = 0; t < cnt; ++t) {
+        const double dt = cd[t];
+        rank += (dt < dp) || (dt == dp && cj[t] < jp);
+      }
+      row[rank] = jp;
+    }
+    int* brow = w.bur + (int64_t)r * BCAP;
+    for (int p = lane(); p < cntb; p += WAVE) {
+      const int jp = bj[p];
+      int rank = 0;
+      for (int t = 0; t < cntb; ++t) rank += bj[t] < jp;
+      brow[rank] = jp;
+    }
+    for (int o = 32; o > 0; o >>= 1) nbmax = max(nbmax, __shfl_xor(nbmax, o, WAVE));
+    if (lane() == 0) {
+      w.nn[r] = cnt;
+      w.nb[r] = cntb;
+      w.nbmax[r] = nbmax;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace cms

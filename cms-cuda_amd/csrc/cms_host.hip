@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/cms_hip.h"
+#include "cms_cells.h"
 #include "cms_device.h"
 #include "cms_internal.h"
 #include "cms_readout.h"
@@ -72,7 +73,8 @@ struct cms_ctx {
   int nR = 0, nP = 0, nQ = 0, nD = 0;
   std::vector<int> h_dgrp, h_prgrp;
   DBuf atten, access, status, coinc, flag, nbscan, nbl, need, rpos, ratom, part;
-  DBuf nn, nbr, nb, bur;
+  DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom;
+  DBuf nn, nbr, nb, bur, nbmax;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
   DBuf ccnt, cscan, t1cnt, t1scan, t2cnt, t2scan, tflag, kcnt, kscan, dbase, dgrp;
@@ -84,6 +86,8 @@ struct cms_ctx {
   DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm;
   DBuf bacc, trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
+  DBuf frame, bxyz32, axyz32, txyz32, qpack32, rpack32, nresc;
+  bool float_lists = false;
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
   struct Ev { std::string name; hipEvent_t a, b; };
@@ -176,6 +180,7 @@ static Work work(cms_ctx* ctx) {
   w.nbr = ctx->nbr.as<int>();
   w.nb = ctx->nb.as<int>();
   w.bur = ctx->bur.as<int>();
+  w.nbmax = ctx->nbmax.as<int>();
   w.pcnt = ctx->pcnt.as<int>();
   w.pscan = ctx->pscan.as<int>();
   w.nP = ctx->pscan.as<int>() + ctx->nR;
@@ -241,6 +246,18 @@ int cms_create(int device, void* stream, cms_ctx** out) {
   cms_ctx* ctx = new cms_ctx();
   ctx->device = device;
   ctx->stream = static_cast<hipStream_t>(stream);
+  if (ctx->nresc.ensure(sizeof(unsigned long long) * 2) != hipSuccess) {
+    g_create_err = "cms_create: out of device memory";
+    delete ctx;
+    return CMS_ERR_HIP;
+  }
+  fill_u64(ctx->nresc.as<unsigned long long>(), 2, 0ull, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    g_create_err = "cms_create: stream synchronisation failed";
+    ctx->nresc.release();
+    delete ctx;
+    return CMS_ERR_HIP;
+  }
   *out = ctx;
   return CMS_OK;
 }
@@ -251,8 +268,9 @@ int cms_destroy(cms_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DBuf* bufs[] = {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
                   &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
-                  &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->need, &ctx->rpos, &ctx->ratom,
-                  &ctx->part, &ctx->nn, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
+                  &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
+                  &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->need, &ctx->rpos, &ctx->ratom,
+                  &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
@@ -264,7 +282,8 @@ int cms_destroy(cms_ctx* ctx) {
                   &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
                   &ctx->bacc, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
-                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx};
+                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
+                  &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc};
   for (DBuf* b : bufs) b->release();
   delete ctx;
   return CMS_OK;
@@ -334,11 +353,14 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
         ctx->atom.as<double4>(), ctx->dens.as<double>(), ctx->acplx.as<int>(), ctx->amol.as<int8_t>());
     CKL();
   }
+  CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
+  fill_u64(ctx->nresc.as<unsigned long long>(), 2, 0ull, ctx->stream);
   // host arrays above are copied asynchronously: wait before returning
   CK(hipStreamSynchronize(ctx->stream));
   ctx->generated = false;
   ctx->gen_all_atoms = -1;
   ctx->lists_ready = false;
+  ctx->float_lists = false;
   return CMS_OK;
 }
 
@@ -359,6 +381,7 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   BatchView b = view(ctx);
   const Settings s = ctx->s;
   ctx->lists_ready = false;
+  ctx->float_lists = false;
 
   CK(ctx->atten.ensure(n + 1));
   CK(ctx->access.ensure(n + 1));
@@ -381,19 +404,47 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   fill_int(ctx->status.as<int>(), B, 0, st);
   fill_u64(ctx->coinc.as<unsigned long long>(), B, ~0ull, st);
   if (n > 0) {
-    TIMED("attention", k_attention<<<grid_for(n, 1), 64, 0, st>>>(b, s, all_atoms, ctx->atten.as<int8_t>(),
-                                              ctx->access.as<int8_t>(),
-                                              ctx->coinc.as<unsigned long long>(),
-                                              ctx->status.as<int>()));
-    CKL();
-    k_flag_nonblocker<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->atten.as<int8_t>(),
-                                                               ctx->flag.as<int>());
-    scan_excl(ctx->flag.as<int>(), ctx->nbscan.as<int>(), n, part, st);
-    k_compact_index<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->flag.as<int>(),
-                                                             ctx->nbscan.as<int>(), ctx->nbl.as<int>());
-    TIMED("region", k_region<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, s, full_state || all_atoms,
-                                                      ctx->atten.as<int8_t>(), ctx->nbscan.as<int>(),
-                                                      ctx->nbl.as<int>(), ctx->need.as<int>()));
+    // cell lists: per-complex grids, atoms sorted by cell
+    const int64_t cell_cap = 8ll * n + 64ll * B;
+    CK(ctx->grid.ensure(sizeof(GridInfo) * (B + 1)));
+    CK(ctx->gframe.ensure(sizeof(double4) * (B + 1)));
+    CK(ctx->ncellc.ensure(sizeof(int) * (B + 1)));
+    CK(ctx->ncscan.ensure(sizeof(int) * (B + 1)));
+    CK(ctx->acell.ensure(sizeof(int) * (n + 1)));
+    CK(ctx->catom.ensure(sizeof(int) * (n + 1)));
+    CK(ctx->ccount.ensure(sizeof(int) * (cell_cap + 1)));
+    CK(ctx->cstart.ensure(sizeof(int) * (cell_cap + 1)));
+    CK(ctx->ccur.ensure(sizeof(int) * (cell_cap + 1)));
+    GridInfo* grid = ctx->grid.as<GridInfo>();
+    TIMED("grid", k_grid_setup<<<grid_for(B, 1, 4096), 256, 0, st>>>(B, ctx->aoff.as<int>(), ctx->atom.as<double4>(), s,
+                                                                      grid, ctx->gframe.as<double4>(),
+                                                                      ctx->ncellc.as<int>()));
+    scan_excl(ctx->ncellc.as<int>(), ctx->ncscan.as<int>(), B, part, st);
+    k_grid_offsets<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->ncscan.as<int>(), grid);
+    fill_int(ctx->ccount.as<int>(), cell_cap, 0, st);
+    fill_int(ctx->ccur.as<int>(), cell_cap, 0, st);
+    k_cell_count<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
+    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cell_cap, part, st);
+    k_cell_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
+                                                            ctx->ccur.as<int>(), ctx->catom.as<int>());
+    // small batches: one wavefront per atom (parallelism); large batches:
+    // one thread per cell-sorted atom (cell data shared within a wavefront)
+    const bool per_thread = n >= 200000;
+    const int* cst = ctx->cstart.as<int>();
+    const int* cat = ctx->catom.as<int>();
+    if (per_thread) {
+      TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
+          b, s, all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
+          ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
+      TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
+          b, s, full_state || all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+    } else {
+      TIMED("attention", k_attention_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
+          b, s, all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
+          ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
+      TIMED("region", k_region_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
+          b, s, full_state || all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+    }
     scan_excl(ctx->need.as<int>(), ctx->rpos.as<int>(), n, part, st);
     k_compact_index<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->need.as<int>(),
                                                              ctx->rpos.as<int>(), ctx->ratom.as<int>());
@@ -406,6 +457,7 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   const int nR = ctx->nR;
   CK(ctx->nn.ensure(sizeof(int) * (nR + 1)));
   CK(ctx->nb.ensure(sizeof(int) * (nR + 1)));
+  CK(ctx->nbmax.ensure(sizeof(int) * (nR + 1)));
   CK(ctx->nbr.ensure(sizeof(int) * ((int64_t)nR * KCAP + 1)));
   CK(ctx->bur.ensure(sizeof(int) * ((int64_t)nR * BCAP + 1)));
   CK(ctx->pcnt.ensure(sizeof(int) * (nR + 1)));
@@ -414,7 +466,8 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   CK(ctx->cscan.ensure(sizeof(int) * (nR + 1)));
   Work w = work(ctx);
   if (nR > 0) {
-    TIMED("neighbors", k_neighbors<<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
+    TIMED("neighbors", k_neighbors_cells<<<grid_for(nR, 1), 64, 0, st>>>(b, s, w, ctx->grid.as<GridInfo>(),
+                                                                          ctx->cstart.as<int>(), ctx->catom.as<int>()));
     TIMED("pairs_count", k_pairs<false><<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
     CKL();
   }
@@ -582,9 +635,9 @@ static int prepare_lists(cms_ctx* ctx) {
 // nearest search of query groups qgrp over reference groups rgrp (g ^ xmask).
 // Small query sets split every group's references over several workgroups
 // so that a single complex still spreads over the GPU.
-static int launch_nearest(cms_ctx* ctx, const double4* Q, const int* qgrp, int nq, int nr_total,
-                          const double4* R, const int* rgrp, int xmask, int G, double* out_d2,
-                          int* out_idx) {
+static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const float4* Q32, const int* qgrp,
+                          int nq, int nr_total, const double4* R, const float4* R32, const int* rgrp, int xmask,
+                          int G, double* out_d2, int* out_idx) {
   hipStream_t st = ctx->stream;
   CK(ctx->tcnt.ensure(sizeof(int) * (G + 1)));
   CK(ctx->toff.ensure(sizeof(int) * (G + 1)));
@@ -595,30 +648,60 @@ static int launch_nearest(cms_ctx* ctx, const double4* Q, const int* qgrp, int n
   int nsplit = 1;
   if (tiles_est < target) {
     nsplit = (int)std::min<int64_t>((target + tiles_est - 1) / tiles_est, 64);
-    // at least 512 references per split on average
+    // at least 128 references per split on average
     const int64_t per_group_refs = std::max<int64_t>(1, (int64_t)nr_total / std::max(G, 1));
-    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, per_group_refs / 512));
+    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, per_group_refs / 128));
   }
   const int grid = grid_for(tiles_est * nsplit, 1, 16384);
-  if (nsplit == 1) {
-    if (out_idx)
-      TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, 1, 0, out_d2, out_idx));
-    else
-      TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, 1, 0, out_d2, nullptr));
-  } else {
-    const int64_t ps = nq;
+  const int64_t ps = nsplit == 1 ? 0 : nq;
+  double* d2o = out_d2;
+  int* ido = out_idx;
+  if (nsplit > 1) {
     CK(ctx->npd2.ensure(sizeof(double) * ((int64_t)nsplit * ps + 1)));
     CK(ctx->npidx.ensure(sizeof(int) * ((int64_t)nsplit * ps + 1)));
-    double* pd2 = ctx->npd2.as<double>();
-    int* pidx = out_idx ? ctx->npidx.as<int>() : nullptr;
-    if (out_idx)
-      TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, nsplit, ps, pd2, pidx));
-    else
-      TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, ctx->toff.as<int>(), G, nsplit, ps, pd2, nullptr));
-    if (nq > 0)
-      k_nearest_combine<<<grid_for(nq, 256, 4096), 256, 0, st>>>(nq, nsplit, ps, pd2, pidx, out_d2, out_idx);
+    d2o = ctx->npd2.as<double>();
+    ido = out_idx ? ctx->npidx.as<int>() : nullptr;
   }
+  const int* toff = ctx->toff.as<int>();
+  if (precision == CMS_PREC_FP32_RESCORE) {
+    unsigned long long* nr = ctx->nresc.as<unsigned long long>();
+    if (ido)
+      TIMED("nearest32", k_nearest32<true><<<grid, 256, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+                                                                 ctx->frame.as<double4>(), d2o, ido, nr));
+    else
+      TIMED("nearest32", k_nearest32<false><<<grid, 256, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+                                                                  ctx->frame.as<double4>(), d2o, nullptr, nr));
+  } else {
+    if (ido)
+      TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, ido));
+    else
+      TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, nullptr));
+  }
+  if (nsplit > 1 && nq > 0)
+    k_nearest_combine<<<grid_for(nq, 256, 4096), 256, 0, st>>>(nq, nsplit, ps, d2o, ido, out_d2, out_idx);
   CKL();
+  return CMS_OK;
+}
+
+// float, recentred copies of the buried / accessible dot lists
+static int prepare_float_lists(cms_ctx* ctx) {
+  if (ctx->float_lists) return CMS_OK;
+  hipStream_t st = ctx->stream;
+  const int B = ctx->B, G = 2 * B;
+  CK(ctx->frame.ensure(sizeof(double4) * (B + 1)));
+  CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
+  CK(ctx->bxyz32.ensure(sizeof(float4) * (ctx->nB + 1)));
+  CK(ctx->axyz32.ensure(sizeof(float4) * (ctx->nA + 1)));
+  k_complex_frame<<<grid_for(B, 1, 4096), 256, 0, st>>>(B, ctx->aoff.as<int>(), ctx->atom.as<double4>(), ctx->s.rp,
+                                                         ctx->frame.as<double4>());
+  if (ctx->nB > 0)
+    k_to_float<<<grid_for(ctx->nB, 256, 4096), 256, 0, st>>>(ctx->nB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, 1,
+                                                              ctx->frame.as<double4>(), ctx->bxyz32.as<float4>());
+  if (ctx->nA > 0)
+    k_to_float<<<grid_for(ctx->nA, 256, 4096), 256, 0, st>>>(ctx->nA, ctx->axyz.as<double4>(), ctx->agrpd.as<int>(), G, 1,
+                                                              ctx->frame.as<double4>(), ctx->axyz32.as<float4>());
+  CKL();
+  ctx->float_lists = true;
   return CMS_OK;
 }
 
@@ -665,16 +748,19 @@ int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals
   if (!ctx || !per_atom || !totals) return CMS_ERR_INVALID;
   if (!ctx->generated || ctx->gen_all_atoms != 0)
     return fail(ctx, CMS_ERR_STATE, "cms_contact_ms: surface not generated in CMS mode");
-  if (precision != CMS_PREC_FP64) return fail(ctx, CMS_ERR_INVALID, "cms_contact_ms: unsupported precision");
+  if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
+    return fail(ctx, CMS_ERR_INVALID, "cms_contact_ms: unsupported precision");
   CK(hipSetDevice(ctx->device));
   int r = prepare_lists(ctx);
   if (r) return r;
+  if (precision == CMS_PREC_FP32_RESCORE && (r = prepare_float_lists(ctx))) return r;
   const int G = 2 * ctx->B, nB = ctx->nB;
   hipStream_t st = ctx->stream;
   CK(ctx->bmin.ensure(sizeof(double) * (nB + 1)));
   CK(ctx->wv.ensure(sizeof(double) * (nB + 1)));
-  r = launch_nearest(ctx, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), nB, nB, ctx->bxyz.as<double4>(),
-                     ctx->bgrp.as<int>(), 1, G, ctx->bmin.as<double>(), nullptr);
+  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), nB, nB,
+                     ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), 1, G,
+                     ctx->bmin.as<double>(), nullptr);
   if (r) return r;
   if (nB > 0)
     k_cms_weights<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bxyz.as<double4>(), ctx->bmin.as<double>(),
@@ -697,11 +783,12 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
   if (!ctx || !results) return CMS_ERR_INVALID;
   if (!ctx->generated || ctx->gen_all_atoms != 0)
     return fail(ctx, CMS_ERR_STATE, "cms_shape_complementarity: surface not generated in CMS mode");
-  if (precision != CMS_PREC_FP64)
+  if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
     return fail(ctx, CMS_ERR_INVALID, "cms_shape_complementarity: unsupported precision");
   CK(hipSetDevice(ctx->device));
   int r = prepare_lists(ctx);
   if (r) return r;
+  if (precision == CMS_PREC_FP32_RESCORE && (r = prepare_float_lists(ctx))) return r;
   hipStream_t st = ctx->stream;
   const int B = ctx->B, G = 2 * B, nB = ctx->nB;
   int* part = ctx->part.as<int>();
@@ -711,8 +798,9 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
   CK(ctx->tgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->side.ensure(sizeof(double) * 8 * G));
   // trim: buried dots of g against accessible dots of g
-  r = launch_nearest(ctx, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), nB, ctx->nA, ctx->axyz.as<double4>(),
-                     ctx->agrpd.as<int>(), 0, G, ctx->bacc.as<double>(), nullptr);
+  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), nB,
+                     ctx->nA, ctx->axyz.as<double4>(), ctx->axyz32.as<float4>(), ctx->agrpd.as<int>(), 0, G,
+                     ctx->bacc.as<double>(), nullptr);
   if (r) return r;
   if (nB > 0)
     k_trim_flag<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bacc.as<double>(), ctx->s.band * ctx->s.band,
@@ -738,8 +826,15 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
                                                              ctx->dnml.as<double4>(), ctx->tlist.as<int>(),
                                                              ctx->txyz.as<double4>(), ctx->tnml.as<double4>());
   CKL();
-  r = launch_nearest(ctx, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), nT, nT, ctx->txyz.as<double4>(),
-                     ctx->tgrp.as<int>(), 1, G, ctx->tmin.as<double>(), ctx->tidx.as<int>());
+  if (precision == CMS_PREC_FP32_RESCORE) {
+    CK(ctx->txyz32.ensure(sizeof(float4) * (nT + 1)));
+    if (nT > 0)
+      k_to_float<<<grid_for(nT, 256, 4096), 256, 0, st>>>(nT, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), G, 1,
+                                                           ctx->frame.as<double4>(), ctx->txyz32.as<float4>());
+  }
+  r = launch_nearest(ctx, precision, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), nT, nT,
+                     ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), 1, G,
+                     ctx->tmin.as<double>(), ctx->tidx.as<int>());
   if (r) return r;
   k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->tgof.as<int>());
   if (nT > 0)
@@ -882,11 +977,37 @@ __global__ void k_pack3(int n, const double* xyz, double4* out) {
     out[i] = make_double4(xyz[3 * (int64_t)i], xyz[3 * (int64_t)i + 1], xyz[3 * (int64_t)i + 2], 0.0);
 }
 
+__global__ void k_points_frame(int nq, const double4* q, int nr, const double4* r, double4* frame) {
+  __shared__ double s[6][256];
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < nq + nr; i += blockDim.x) {
+    const double4 a = i < nq ? q[i] : r[i - nq];
+    mn[0] = fmin(mn[0], a.x); mn[1] = fmin(mn[1], a.y); mn[2] = fmin(mn[2], a.z);
+    mx[0] = fmax(mx[0], a.x); mx[1] = fmax(mx[1], a.y); mx[2] = fmax(mx[2], a.z);
+  }
+  for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = mn[k]; s[3 + k][threadIdx.x] = mx[k]; }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + o]);
+      for (int k = 3; k < 6; ++k) s[k][threadIdx.x] = fmax(s[k][threadIdx.x], s[k][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double cx = 0.5 * (s[0][0] + s[3][0]), cy = 0.5 * (s[1][0] + s[4][0]), cz = 0.5 * (s[2][0] + s[5][0]);
+    double half = fmax(fmax(s[3][0] - cx, s[4][0] - cy), s[5][0] - cz);
+    if (!(half >= 0.0)) { cx = cy = cz = 0.0; half = 0.0; }
+    frame[0] = make_double4(cx, cy, cz, half * 1.0000001 + 1e-6);
+  }
+}
+
 int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r, int64_t nr,
                 double* out_d2, int32_t* out_idx) {
   if (!ctx || nq < 0 || nr < 0 || (nq > 0 && (!q || !out_d2)) || (nr > 0 && !r)) return CMS_ERR_INVALID;
   if (nq > (1ll << 30) || nr > (1ll << 30)) return fail(ctx, CMS_ERR_INVALID, "cms_nearest: too many points");
-  if (precision != CMS_PREC_FP64) return fail(ctx, CMS_ERR_INVALID, "cms_nearest: unsupported precision");
+  if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
+    return fail(ctx, CMS_ERR_INVALID, "cms_nearest: unsupported precision");
   CK(hipSetDevice(ctx->device));
   if (nq == 0) return CMS_OK;
   hipStream_t st = ctx->stream;
@@ -894,14 +1015,41 @@ int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const 
   CK(ctx->rpack.ensure(sizeof(double4) * (nr + 1)));
   CK(ctx->grp1.ensure(sizeof(int) * 4));
   CK(ctx->part.ensure(sizeof(int) * 1040));
+  CK(ctx->frame.ensure(sizeof(double4) * 2));
+  CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
   k_pack3<<<grid_for(nq, 256, 4096), 256, 0, st>>>((int)nq, q, ctx->qpack.as<double4>());
   if (nr > 0) k_pack3<<<grid_for(nr, 256, 4096), 256, 0, st>>>((int)nr, r, ctx->rpack.as<double4>());
   int g[4] = {0, (int)nq, 0, (int)nr};
   CK(hipMemcpyAsync(ctx->grp1.p, g, sizeof g, hipMemcpyHostToDevice, st));
-  int rc = launch_nearest(ctx, ctx->qpack.as<double4>(), ctx->grp1.as<int>(), (int)nq, (int)nr, ctx->rpack.as<double4>(),
+  if (precision == CMS_PREC_FP32_RESCORE) {
+    CK(ctx->qpack32.ensure(sizeof(float4) * (nq + 1)));
+    CK(ctx->rpack32.ensure(sizeof(float4) * (nr + 1)));
+    k_points_frame<<<1, 256, 0, st>>>((int)nq, ctx->qpack.as<double4>(), (int)nr, ctx->rpack.as<double4>(),
+                                      ctx->frame.as<double4>());
+    k_to_float<<<grid_for(nq, 256, 4096), 256, 0, st>>>((int)nq, ctx->qpack.as<double4>(), nullptr, 1, 0,
+                                                         ctx->frame.as<double4>(), ctx->qpack32.as<float4>());
+    if (nr > 0)
+      k_to_float<<<grid_for(nr, 256, 4096), 256, 0, st>>>((int)nr, ctx->rpack.as<double4>(), nullptr, 1, 0,
+                                                           ctx->frame.as<double4>(), ctx->rpack32.as<float4>());
+  }
+  int rc = launch_nearest(ctx, precision, ctx->qpack.as<double4>(), ctx->qpack32.as<float4>(), ctx->grp1.as<int>(),
+                          (int)nq, (int)nr, ctx->rpack.as<double4>(), ctx->rpack32.as<float4>(),
                           ctx->grp1.as<int>() + 2, 0, 1, out_d2, out_idx);
   if (rc) return rc;
   CK(hipStreamSynchronize(st));
+  return CMS_OK;
+}
+
+int cms_search_stats(cms_ctx* ctx, uint64_t* rescored) {
+  if (!ctx || !rescored) return CMS_ERR_INVALID;
+  *rescored = 0;
+  if (!ctx->nresc.p) return CMS_OK;
+  unsigned long long v = 0;
+  CK(hipMemcpyAsync(&v, ctx->nresc.p, sizeof v, hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  *rescored = v;
+  fill_u64(ctx->nresc.as<unsigned long long>(), 1, 0ull, ctx->stream);
+  CK(hipStreamSynchronize(ctx->stream));
   return CMS_OK;
 }
 

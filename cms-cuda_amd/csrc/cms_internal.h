@@ -48,6 +48,7 @@ struct Work {
   int* nbr;
   int* nb;
   int* bur;
+  int* nbmax;             // largest non-blocker neighbour index per row (-1: none)
   // pairs
   int* pcnt;
   int* pscan;
@@ -93,11 +94,7 @@ template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_toroid(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);
-__global__ void k_attention(BatchView, Settings, int, int8_t*, int8_t*, unsigned long long*, int*);
-__global__ void k_flag_nonblocker(int, const int8_t*, int*);
-__global__ void k_region(BatchView, Settings, int, const int8_t*, const int*, const int*, int*);
 __global__ void k_compact_index(int, const int*, const int*, int*);
-__global__ void k_neighbors(BatchView, Settings, Work);
 __global__ void k_near(BatchView, Settings, Work);
 
 }  // namespace cms

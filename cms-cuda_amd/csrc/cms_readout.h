@@ -34,4 +34,13 @@ __global__ void k_sc_sides(int G, const int* tgrp, const double4* txyz, const do
                            const double* rr, const int* dgrp, Settings s, double* side);
 __global__ void k_sc_combine(int B, const double* side, double* res);
 
+template <bool IDX>
+__global__ void k_nearest32(const float4* Q32, const double4* Q64, const int* qgrp, const float4* R32,
+                            const double4* R64, const int* rgrp, int xmask, const int* toff, int G, int nsplit,
+                            int64_t pstride, const double4* frame, double* out_d2, int* out_idx,
+                            unsigned long long* nrescored);
+__global__ void k_complex_frame(int B, const int* aoff, const double4* atom, double rp, double4* frame);
+__global__ void k_to_float(int n, const double4* in, const int* grp, int G, int percomplex,
+                           const double4* frame, float4* out);
+
 }  // namespace cms

@@ -222,21 +222,21 @@ constexpr int PW_MAXTOK = 2048;
 // thread 0 evaluates the postfix.  Returns the sum in every thread.
 __device__ double block_np_sum(const double* a, int64_t stride, int n) {
   __shared__ int tok_lo[PW_MAXTOK];
-  __shared__ int tok_n[PW_MAXTOK];   // > 0 leaf of that length, 0 = add
+  __shared__ int tok_n[PW_MAXTOK];   // > 0 leaf of that length, 0 = add, -1 empty leaf
   __shared__ double leaf[PW_MAXTOK];
-  __shared__ int ntok, nleaf, overflow;
+  __shared__ int stk_n[64], stk_lo[64], stk_v[64];
+  __shared__ double vstk[64];
+  __shared__ int ntok, overflow;
   __shared__ double result;
   if (threadIdx.x == 0) {
-    int stk_n[64], stk_lo[64], stk_v[64];
-    int sp = 0, nt = 0, nl = 0;
-    int of = 0;
+    int sp = 0, nt = 0, of = 0;
     stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
-    while (sp >= 0 && !of) {
+    while (sp >= 0) {
       const int m = stk_n[sp], lo = stk_lo[sp], v = stk_v[sp];
       --sp;
-      if (nt >= PW_MAXTOK) { of = 1; break; }
+      if (nt >= PW_MAXTOK || sp > 60) { of = 1; break; }
       if (m <= 128) {
-        tok_lo[nt] = lo; tok_n[nt] = m > 0 ? m : -1; ++nt; ++nl;
+        tok_lo[nt] = lo; tok_n[nt] = m > 0 ? m : -1; ++nt;
       } else if (!v) {
         int n2 = m / 2;
         n2 -= n2 % 8;
@@ -247,14 +247,12 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
         tok_lo[nt] = 0; tok_n[nt] = 0; ++nt;
       }
     }
-    ntok = nt; nleaf = nl; overflow = of;
+    ntok = nt; overflow = of;
   }
   __syncthreads();
   if (overflow) {
-    // very long arrays: a single thread follows the same tree recursively
+    // very long arrays: one thread follows the same tree
     if (threadIdx.x == 0) {
-      double vstk[64];
-      int stk_n[64], stk_lo[64], stk_v[64];
       int sp = 0, vp = 0;
       stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
       while (sp >= 0) {
@@ -277,7 +275,9 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
       result = vstk[0];
     }
     __syncthreads();
-    return result;
+    const double r = result;
+    __syncthreads();
+    return r;
   }
   for (int t = threadIdx.x; t < ntok; t += blockDim.x) {
     const int m = tok_n[t];
@@ -285,7 +285,6 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double vstk[64];
     int vp = 0;
     for (int t = 0; t < ntok; ++t) {
       if (tok_n[t] != 0) {

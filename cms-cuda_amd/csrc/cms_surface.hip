@@ -17,160 +17,10 @@
 
 namespace cms {
 
-// ------------------------------------------------------------ attention
-// _core.py:1150-1195: atten = 1 (blocker) if the nearest atom of the other
-// molecule is >= sep away, else 5.  The same sweep checks coincident
-// same-molecule atoms (d^2 <= 1e-4, _core.py:1285-1293) and records the
-// first such (i, j) pair in row-major order per complex.
-__global__ __launch_bounds__(64) void k_attention(BatchView b, Settings s, int all_atoms,
-                                                  int8_t* __restrict__ atten,
-                                                  int8_t* __restrict__ access,
-                                                  unsigned long long* __restrict__ coinc,
-                                                  int* __restrict__ status) {
-  CMS_UNIT_LOOP(g, b.ntot) {
-    const int c = b.acplx[g];
-    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
-    const int m = b.amol[g];
-    const V3 xi = ld3(b.atom[g]);
-    double mn = INFINITY;
-    unsigned long long ckey = ~0ull;
-    for (int j0 = a0; j0 < a1; j0 += WAVE) {
-      const int j = j0 + lane();
-      if (j < a1) {
-        const V3 d = xi - ld3(b.atom[j]);
-        const double d2 = dot_seq(d, d);
-        if (b.amol[j] != m) {
-          mn = fmin(mn, d2);
-        } else if (j != g && d2 <= 0.0001) {
-          unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
-          ckey = ckey < key ? ckey : key;
-        }
-      }
-    }
-    mn = wave_min_double(mn);
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long t = __shfl_xor(ckey, o, WAVE);
-      ckey = ckey < t ? ckey : t;
-    }
-    if (lane() == 0) {
-      double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
-      atten[g] = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
-      access[g] = 0;
-      if (ckey != ~0ull) {
-        atomicMin(&coinc[c], ckey);
-        atomicOr(&status[c], ST_COINCIDENT);
-      }
-    }
-  }
-}
-
-__global__ void k_flag_nonblocker(int n, const int8_t* __restrict__ atten, int* __restrict__ flag) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x)
-    flag[g] = atten[g] > ATTEN_BLOCKER ? 1 : 0;
-}
-
-// Interface region: atoms that are non-blockers or lie within bridge
-// distance (r_i + r_j + 2rp) of a same-molecule non-blocker.  Only these
-// atoms can own dots, appear in a toroid-queue pair or in a probe triple
-// with a non-blocker, so restricting neighbour rows, pairs and probes to them
-// leaves every dot, probe and readout unchanged.  With full != 0 every atom
-// is kept (full run-state introspection, max-CMS mode).
-__global__ void k_region(BatchView b, Settings s, int full, const int8_t* __restrict__ atten,
-                         const int* __restrict__ nb_scan, const int* __restrict__ nbl,
-                         int* __restrict__ need) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < b.ntot; g += gridDim.x * blockDim.x) {
-    int keep = full || atten[g] > ATTEN_BLOCKER;
-    if (!keep) {
-      const int c = b.acplx[g];
-      const int m = b.amol[g];
-      const double4 ai = b.atom[g];
-      const V3 xi = ld3(ai);
-      const int lo = nb_scan[b.aoff[c]], hi = nb_scan[b.aoff[c + 1]];
-      for (int t = lo; t < hi && !keep; ++t) {
-        const int j = nbl[t];
-        if (b.amol[j] != m) continue;
-        const double4 aj = b.atom[j];
-        const V3 d = xi - ld3(aj);
-        const double d2 = dot_seq(d, d);
-        const double br = (ai.w + aj.w) + 2.0 * s.rp;
-        if (d2 < br * br) keep = 1;
-      }
-    }
-    need[g] = keep;
-  }
-}
-
 __global__ void k_compact_index(int n, const int* __restrict__ flag, const int* __restrict__ scan,
                                 int* __restrict__ out) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (flag[i]) out[scan[i]] = i;
-}
-
-// ------------------------------------------------------------ neighbours
-// _core.py:1251-1349.  Row of atom i: same-molecule atoms j != i with
-// d^2 < (r_i + r_j + 2rp)^2 sorted by (d^2, j); buried row: other-molecule
-// atoms with atten >= 5 inside the same bridge, ascending j.
-__global__ __launch_bounds__(64) void k_neighbors(BatchView b, Settings s, Work w) {
-  __shared__ int cj[KCAP];
-  __shared__ double cd[KCAP];
-  const int nR = *w.nR;
-  CMS_UNIT_LOOP(r, nR) {
-    const int g = w.ratom[r];
-    const int c = b.acplx[g];
-    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
-    const int m = b.amol[g];
-    const double4 ai = b.atom[g];
-    const V3 xi = ld3(ai);
-    int cnt = 0, cntb = 0;
-    int* brow = w.bur + (int64_t)r * BCAP;
-    for (int j0 = a0; j0 < a1; j0 += WAVE) {
-      const int j = j0 + lane();
-      bool isn = false, isb = false;
-      double d2 = 0.0;
-      if (j < a1) {
-        const double4 aj = b.atom[j];
-        const V3 d = xi - ld3(aj);
-        d2 = dot_seq(d, d);
-        const double br = (ai.w + aj.w) + 2.0 * s.rp;
-        const bool inb = d2 < br * br;
-        if (b.amol[j] == m) isn = inb && j != g;
-        else isb = inb && w.atten[j] >= ATTEN_BURIED;
-      }
-      const uint64_t mn = ballot(isn), mb = ballot(isb);
-      if (isn) {
-        const int p = cnt + lanes_below(mn);
-        if (p < KCAP) { cj[p] = j; cd[p] = d2; }
-      }
-      if (isb) {
-        const int p = cntb + lanes_below(mb);
-        if (p < BCAP) brow[p] = j;
-      }
-      cnt += __popcll(mn);
-      cntb += __popcll(mb);
-    }
-    __syncthreads();
-    if (cnt > KCAP || cntb > BCAP) {
-      if (lane() == 0) atomicOr(&w.status[c], cnt > KCAP ? ST_NBR_CAP : ST_BUR_CAP);
-      cnt = min(cnt, KCAP);
-      cntb = min(cntb, BCAP);
-    }
-    int* row = w.nbr + (int64_t)r * KCAP;
-    for (int p = lane(); p < cnt; p += WAVE) {
-      const double dp = cd[p];
-      const int jp = cj[p];
-      int rank = 0;
-      for (int t = 0; t < cnt; ++t) {
-        const double dt = cd[t];
-        rank += (dt < dp) || (dt == dp && cj[t] < jp);
-      }
-      row[rank] = jp;
-    }
-    if (lane() == 0) {
-      w.nn[r] = cnt;
-      w.nb[r] = cntb;
-    }
-    __syncthreads();
-  }
 }
 
 // ------------------------------------------------------------ pairs
@@ -186,12 +36,13 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
     const V3 xi = ld3(ai);
     const double ri = ai.w, eri = ri + s.rp;
     const int* row = w.nbr + (int64_t)r * KCAP;
+    const int nbmax = w.nbmax[r];
     int base = EMIT ? w.pscan[r] : 0;
     int total = 0;
     bool shortcut = false;
     for (int s0 = 0; s0 < nn; s0 += WAVE) {
       const int sl = s0 + lane();
-      bool ok = false;
+      bool ok = false, relevant = false;
       int j = -1;
       V3 u{0, 0, 0}, t{0, 0, 0};
       double rij = 0.0;
@@ -213,6 +64,9 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
           const double cont_sq = dsq - (ri - rj) * (ri - rj);
           ok = (far_sq > 0.0) && (cont_sq > 0.0);
           if (ok) rij = (0.5 * sqrt(far_sq)) * sqrt(cont_sq) / dij;
+          // a pair matters only for the toroid queue (a non-blocker end) or
+          // for probes (some atom of a triple i, j, k > j is a non-blocker)
+          relevant = ok && (w.atten[g] > ATTEN_BLOCKER || w.atten[j] > ATTEN_BLOCKER || nbmax > j);
         }
       }
       if (nn <= 1) {
@@ -226,8 +80,8 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
           break;
         }
       }
-      const uint64_t mk = ballot(ok);
-      if (EMIT && ok) {
+      const uint64_t mk = ballot(relevant);
+      if (EMIT && relevant) {
         const int o = base + total + lanes_below(mk);
         w.pi[o] = g;
         w.pj[o] = j;

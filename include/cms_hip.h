@@ -168,6 +168,11 @@ int cms_copy_probes(cms_ctx* ctx, int complex_index, int32_t* atoms, double* hei
 int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r,
                 int64_t nr, double* out_d2, int32_t* out_idx);
 
+/* Number of references rescored in FP64 by CMS_PREC_FP32_RESCORE searches
+   since the last call (then reset).  No reference counterpart (the
+   reference searches in FP64 only). */
+int cms_search_stats(cms_ctx* ctx, uint64_t* rescored);
+
 /* Kernel timing with HIP events on the context's stream.  filter: NULL or
    "" = off, "*" = every instrumented kernel, otherwise one kernel name
    ("attention", "neighbors", "pairs_count", "probes_count", "convex_emit",

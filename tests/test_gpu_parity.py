@@ -135,11 +135,14 @@ def test_golden_probes(cms_cuda, golden_complex, golden_oracle):
         assert rel_err(pr.point_xyz[mol == m], o.pr_p[omol == m]) <= TOL_DOT
 
 
-def test_synthetic_config1(cms_cuda):
+@pytest.mark.parametrize("which", ["config1", "config2"])
+def test_synthetic_configs(cms_cuda, which):
+    """400 + 1,000 and 1,800 + 4,000 atom synthetic complexes against the
+    oracle."""
     from cms_cuda import synth
     from oracle.cms_oracle import OracleCalculator
-    say("synthetic 400 + 1000 complex")
-    c = synth.config1(seed=1)
+    say("synthetic " + which)
+    c = getattr(synth, which)()
     args = (c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"])
     o = OracleCalculator()
     o.add_binder_and_target(*args)
@@ -252,4 +255,59 @@ def test_batch_matches_single(cms_cuda, golden_complex):
         assert out["target"][k] == cms_t and out["binder"][k] == cms_b, k
         assert np.array_equal(per[k][1], per_t) and np.array_equal(per[k][0], per_b), k
         assert tuple(sc[k, :3]) == s, k
+    bc.close()
+
+
+def test_fp32_search_matches_fp64(cms_cuda):
+    """FP32 search with FP64 rescoring returns the FP64 distances and the
+    FP64 first-index partners exactly, including exact ties and near-ties
+    closer than float resolution."""
+    import ctypes
+    import torch
+    from cms_cuda import _lib
+    say("fp32 vs fp64 nearest")
+    rng = np.random.default_rng(7)
+    q = rng.uniform(-60, 60, size=(4000, 3)) + np.array([30.0, -20.0, 10.0])
+    r = rng.uniform(-60, 60, size=(3000, 3)) + np.array([30.0, -20.0, 10.0])
+    r[2000:2100] = r[0:100]                       # exact ties
+    r[2100:2200] = r[100:200] + 1e-12             # near-ties below float resolution
+    q[:200] = r[:200] + rng.normal(scale=1e-3, size=(200, 3))
+    ctx = _lib.Context(0, torch.cuda.current_stream().cuda_stream)
+    dq = torch.from_numpy(q).cuda()
+    dr = torch.from_numpy(r).cuda()
+    res = {}
+    for prec in (_lib.PREC_FP64, _lib.PREC_FP32_RESCORE):
+        d2 = torch.empty(len(q), dtype=torch.float64, device="cuda")
+        idx = torch.empty(len(q), dtype=torch.int32, device="cuda")
+        ctx("cms_nearest", prec, ctypes.c_void_p(dq.data_ptr()), len(q), ctypes.c_void_p(dr.data_ptr()), len(r),
+            ctypes.c_void_p(d2.data_ptr()), ctypes.c_void_p(idx.data_ptr()))
+        res[prec] = (d2.cpu().numpy(), idx.cpu().numpy())
+    n = ctypes.c_uint64()
+    ctx("cms_search_stats", ctypes.byref(n))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    assert n.value < 200 * len(q)   # rescoring stays a small fraction of all pairs
+    ctx.close()
+
+
+def test_fp32_mode_batch_identical(cms_cuda, golden_complex):
+    """Batched CMS / SC in FP32-search mode equal the FP64 mode bit for bit."""
+    from cms_cuda import synth
+    from cms_cuda import _lib
+    from cms_cuda.batch import BatchCalculator
+    say("fp32 mode batch")
+    g = golden_complex
+    cs = [(g["binder_xyz"], g["binder_radii"], g["target_xyz"], g["target_radii"])]
+    for seed, (nt, nb) in enumerate([(800, 40), (1000, 400), (1200, 900)]):
+        c = synth.make_complex(nt, nb, 200 + seed, gap=3.5)
+        cs.append((c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"]))
+    bc = BatchCalculator()
+    bc.load(cs)
+    a64 = bc.contact_ms(_lib.PREC_FP64)
+    s64 = bc.shape_complementarity(_lib.PREC_FP64)
+    a32 = bc.contact_ms(_lib.PREC_FP32_RESCORE)
+    s32 = bc.shape_complementarity(_lib.PREC_FP32_RESCORE)
+    assert np.array_equal(a64["target"], a32["target"]) and np.array_equal(a64["binder"], a32["binder"])
+    assert np.array_equal(a64["per_atom"], a32["per_atom"])
+    assert np.array_equal(s64, s32)
     bc.close()
