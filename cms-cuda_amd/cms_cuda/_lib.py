@@ -56,13 +56,13 @@ _PROTOS = {
     "cms_copy_probes": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp]),
     "cms_nearest": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),
     "cms_profile": (_c.c_int, [_vp, _c.c_char_p]),
+    "cms_backward_cms": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "cms_frozen_cms": (_c.c_int, [_vp, _vp, _vp]),
+    "cms_check_dots": (_c.c_int, [_vp, _c.POINTER(_i64), _c.POINTER(_c.c_double)]),
     "cms_search_stats": (_c.c_int, [_vp, _c.POINTER(_c.c_uint64)]),
     "cms_profile_read": (_c.c_int, [_vp, _c.c_char_p, _c.POINTER(_c.c_double), _c.POINTER(_i64)]),
 }
-OPTIONAL = {
-    "cms_backward_cms": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
-    "cms_frozen_cms": (_c.c_int, [_vp, _vp, _vp, _vp]),
-}
+OPTIONAL = {}
 
 _lock = threading.Lock()
 _lib = None

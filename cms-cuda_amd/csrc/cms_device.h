@@ -25,6 +25,10 @@ constexpr int8_t ATTEN_2 = 2;
 constexpr int8_t ATTEN_BURIED = 5;
 constexpr double TWO_PI = 6.283185307179586;
 
+// Dot descriptor kinds: what re-evaluates the dot from atom coordinates
+// with its subdivision counts frozen (backward pass, frozen-topology CMS).
+constexpr int DK_CONVEX = 1, DK_ARC1 = 2, DK_ARC2 = 3, DK_CONCAVE = 4;
+
 // status bits (per complex)
 constexpr int ST_COINCIDENT = 1;
 constexpr int ST_NBR_CAP = 2;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/cms_hip.h"
+#include "cms_backward.h"
 #include "cms_cells.h"
 #include "cms_device.h"
 #include "cms_internal.h"
@@ -78,7 +79,7 @@ struct cms_ctx {
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
   DBuf ccnt, cscan, t1cnt, t1scan, t2cnt, t2scan, tflag, kcnt, kscan, dbase, dgrp;
-  DBuf dpos, dnml, dbur, dtype, datom;
+  DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
   int nB = 0, nA = 0;
@@ -87,6 +88,8 @@ struct cms_ctx {
   DBuf bacc, trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
   DBuf frame, bxyz32, axyz32, txyz32, qpack32, rpack32, nresc;
+  DBuf bidx, bgof, adj, eatom, evpos, chk;
+  bool cms_ready = false;
   bool float_lists = false;
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
@@ -214,6 +217,8 @@ static Work work(cms_ctx* ctx) {
   w.dbur = ctx->dbur.as<int8_t>();
   w.dtype = ctx->dtype.as<int8_t>();
   w.datom = ctx->datom.as<int>();
+  w.ddesc = ctx->ddesc.as<int4>();
+  w.ddesc2 = ctx->ddesc2.as<int4>();
   w.dot_cap = ctx->nD;
   return w;
 }
@@ -276,14 +281,15 @@ int cms_destroy(cms_ctx* ctx) {
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
                   &ctx->tflag, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
-                  &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->bflag,
+                  &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
                   &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
                   &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
                   &ctx->bacc, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
                   &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
-                  &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc};
+                  &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
+                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk};
   for (DBuf* b : bufs) b->release();
   delete ctx;
   return CMS_OK;
@@ -361,6 +367,7 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
   ctx->gen_all_atoms = -1;
   ctx->lists_ready = false;
   ctx->float_lists = false;
+  ctx->cms_ready = false;
   return CMS_OK;
 }
 
@@ -382,6 +389,7 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   const Settings s = ctx->s;
   ctx->lists_ready = false;
   ctx->float_lists = false;
+  ctx->cms_ready = false;
 
   CK(ctx->atten.ensure(n + 1));
   CK(ctx->access.ensure(n + 1));
@@ -539,6 +547,8 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   CK(ctx->dbur.ensure(nD + 1));
   CK(ctx->dtype.ensure(nD + 1));
   CK(ctx->datom.ensure(sizeof(int) * (nD + 1)));
+  CK(ctx->ddesc.ensure(sizeof(int4) * (nD + 1)));
+  CK(ctx->ddesc2.ensure(sizeof(int4) * (nD + 1)));
   w = work(ctx);
   if (nD > 0) {
     if (nR > 0) TIMED("convex_emit", k_convex<true><<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
@@ -757,16 +767,20 @@ int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals
   const int G = 2 * ctx->B, nB = ctx->nB;
   hipStream_t st = ctx->stream;
   CK(ctx->bmin.ensure(sizeof(double) * (nB + 1)));
+  CK(ctx->bidx.ensure(sizeof(int) * (nB + 1)));
   CK(ctx->wv.ensure(sizeof(double) * (nB + 1)));
   r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), nB, nB,
                      ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), 1, G,
-                     ctx->bmin.as<double>(), nullptr);
+                     ctx->bmin.as<double>(), ctx->bidx.as<int>());
   if (r) return r;
   if (nB > 0)
     k_cms_weights<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bxyz.as<double4>(), ctx->bmin.as<double>(),
                                                             ctx->s.weight, ctx->wv.as<double>());
   CKL();
-  return owner_sums(ctx, nB, ctx->blist.as<int>(), ctx->wv.as<double>(), 1, per_atom, totals, G, 1);
+  r = owner_sums(ctx, nB, ctx->blist.as<int>(), ctx->wv.as<double>(), 1, per_atom, totals, G, 1);
+  if (r) return r;
+  ctx->cms_ready = true;
+  return CMS_OK;
 }
 
 int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals) {
@@ -1084,5 +1098,98 @@ int cms_profile_read(cms_ctx* ctx, const char* name, double* total_ms, int64_t* 
   return CMS_OK;
 }
 
+// ------------------------------------------------------------ backward
+
+static int buried_groups(cms_ctx* ctx) {
+  const int G = 2 * ctx->B;
+  CK(ctx->bgof.ensure(sizeof(int) * (ctx->nB + 1)));
+  k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, ctx->stream>>>(G, ctx->bgrp.as<int>(), ctx->bgof.as<int>());
+  CKL();
+  return CMS_OK;
+}
+
+int cms_backward_cms(cms_ctx* ctx, const double* g_totals, const double* g_per_atom, double* grad_xyz) {
+  if (!ctx || !grad_xyz) return CMS_ERR_INVALID;
+  if (!ctx->cms_ready)
+    return fail(ctx, CMS_ERR_STATE, "cms_backward_cms: run cms_contact_ms on the current surface first");
+  CK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int nD = ctx->nD, nB = ctx->nB;
+  int r = buried_groups(ctx);
+  if (r) return r;
+  CK(ctx->adj.ensure(sizeof(double) * 4 * ((int64_t)nD + 1)));
+  fill_double(ctx->adj.as<double>(), 4 * (int64_t)nD, 0.0, st);
+  fill_double(grad_xyz, 3 * (int64_t)ctx->ntot, 0.0, st);
+  if (nB > 0)
+    TIMED("cms_adjoint", k_cms_adjoint<<<grid_for(nB, 256, 4096), 256, 0, st>>>(
+        nB, ctx->blist.as<int>(), ctx->bgof.as<int>(), ctx->bgrp.as<int>(), ctx->bidx.as<int>(),
+        ctx->bmin.as<double>(), ctx->dpos.as<double4>(), ctx->datom.as<int>(), g_totals, g_per_atom,
+        ctx->s.weight, ctx->adj.as<double>()));
+  if (nD > 0)
+    TIMED("dot_grad", k_dot_grad<<<grid_for(nD, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, nD,
+                                                                          ctx->ddesc.as<int4>(), ctx->ddesc2.as<int4>(),
+                                                                          ctx->adj.as<double>(), grad_xyz));
+  CKL();
+  return CMS_OK;
+}
+
+__global__ void k_pack_eval(int n, const double* xyz, const double4* atom, double4* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = make_double4(xyz[3 * (int64_t)i], xyz[3 * (int64_t)i + 1], xyz[3 * (int64_t)i + 2], atom[i].w);
+}
+
+int cms_frozen_cms(cms_ctx* ctx, const double* xyz_eval, double* totals) {
+  if (!ctx || !xyz_eval || !totals) return CMS_ERR_INVALID;
+  if (!ctx->cms_ready)
+    return fail(ctx, CMS_ERR_STATE, "cms_frozen_cms: run cms_contact_ms on the current surface first");
+  CK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->ntot, nB = ctx->nB, G = 2 * ctx->B;
+  int r = buried_groups(ctx);
+  if (r) return r;
+  CK(ctx->eatom.ensure(sizeof(double4) * (n + 1)));
+  CK(ctx->evpos.ensure(sizeof(double4) * (nB + 1)));
+  if (n > 0)
+    k_pack_eval<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, xyz_eval, ctx->atom.as<double4>(), ctx->eatom.as<double4>());
+  CK(ctx->wv.ensure(sizeof(double) * (nB + 1)));
+  if (nB > 0) {
+    k_frozen_eval<<<grid_for(nB, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, ctx->eatom.as<double4>(), nB,
+                                                           ctx->blist.as<int>(), ctx->ddesc.as<int4>(),
+                                                           ctx->ddesc2.as<int4>(), ctx->evpos.as<double4>());
+    k_frozen_cms<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bgof.as<int>(), ctx->bgrp.as<int>(),
+                                                           ctx->bidx.as<int>(), ctx->evpos.as<double4>(),
+                                                           ctx->s.weight, ctx->wv.as<double>());
+  }
+  // per-group sums in buried-dot order: deterministic, so finite differences
+  // see no summation-order noise
+  k_group_np_sum<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->bgrp.as<int>(), ctx->wv.as<double>(), 1, totals);
+  CKL();
+  return CMS_OK;
+}
+
+int cms_check_dots(cms_ctx* ctx, int64_t* mismatches, double* maxdev) {
+  if (!ctx || !mismatches || !maxdev) return CMS_ERR_INVALID;
+  if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_check_dots: no surface");
+  CK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  CK(ctx->chk.ensure(16));
+  fill_u64(ctx->chk.as<unsigned long long>(), 2, 0ull, st);
+  if (ctx->nD > 0)
+    k_check_dots<<<grid_for(ctx->nD, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, ctx->nD, ctx->dpos.as<double4>(),
+                                                               ctx->ddesc.as<int4>(), ctx->ddesc2.as<int4>(),
+                                                               ctx->chk.as<unsigned long long>(),
+                                                               reinterpret_cast<double*>(ctx->chk.as<unsigned long long>() + 1));
+  CKL();
+  unsigned long long h[2] = {0, 0};
+  CK(hipMemcpyAsync(h, ctx->chk.p, 16, hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  *mismatches = (int64_t)h[0];
+  double d;
+  memcpy(&d, &h[1], sizeof d);
+  *maxdev = d;
+  return CMS_OK;
+}
+
 }  // extern "C"
+
 

@@ -86,6 +86,8 @@ struct Work {
   int8_t* dbur;
   int8_t* dtype;
   int* datom;
+  int4* ddesc;            // (kind | mm << 4, atom a, atom b, atom c)
+  int4* ddesc2;           // (latitude or torus point, point, count 1, count 2)
   int64_t dot_cap;
 };
 

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64) void k_near(BatchView b, Settings s, Work w) {
 // (_core.py:2337-2377), then the store.
 __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s, const Work& w,
                                           int64_t o, V3 pt, V3 pc, double area, int owner,
-                                          int rowner, int8_t type) {
+                                          int rowner, int8_t type, int4 desc, int4 desc2) {
   V3 nml = (s.rp <= 0.0) ? pt - ld3(b.atom[owner]) : (pc - pt) / s.rp;
   // every dot owner has a neighbour row; a missing one would be a bug, so
   // it is flagged (pool status) instead of read out of bounds
@@ -320,6 +320,8 @@ __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s,
   w.dbur[o] = buried;
   w.dtype[o] = type;
   w.datom[o] = owner;
+  w.ddesc[o] = desc;
+  w.ddesc2[o] = desc2;
 }
 
 // ------------------------------------------------------------ convex
@@ -432,7 +434,8 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
         const uint64_t mk = ballot(keep);
         if (EMIT && keep) {
           const int64_t o = obase + total + lanes_below(mk);
-          store_dot(b, s, w, o, pt, pc, lat_ps[t] * cs, g, r, 1);
+          store_dot(b, s, w, o, pt, pc, lat_ps[t] * cs, g, r, 1, make_int4(DK_CONVEX, g, row[0], -1),
+                    make_int4(t, q - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
         }
         total += __popcll(mk);
       }
@@ -557,7 +560,8 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
               const V3 pt = subdiv_point(ts_.p, ts_.pi, ts_.y1, s.rp, adelta, k);
               const double dist = dist_to_line(tij, uij, pt);
               const double area = ps * ts * dist / rij;
-              store_dot(b, s, w, o0 + k, pt, ts_.p, area, i, ri_, 2);
+              store_dot(b, s, w, o0 + k, pt, ts_.p, area, i, ri_, 2, make_int4(DK_ARC1, i, j, -1),
+                        make_int4(m, k, M, ts_.n1));
             }
           }
           if (ts_.n2 > 0) {
@@ -567,7 +571,8 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
               const V3 pt = subdiv_point(ts_.p, ts_.pqj, ts_.y2, s.rp, adelta, k);
               const double dist = dist_to_line(tij, uij, pt);
               const double area = ps * ts * dist / rij;
-              store_dot(b, s, w, o0 + k, pt, ts_.p, area, j, rj_, 2);
+              store_dot(b, s, w, o0 + k, pt, ts_.p, area, j, rj_, 2, make_int4(DK_ARC2, i, j, -1),
+                        make_int4(m, k, M, ts_.n2));
             }
           }
         }
@@ -698,7 +703,9 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
         }
         const int owner = aid[mc];
         const int64_t o = obase + total + lanes_below(mk);
-        store_dot(b, s, w, o, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3);
+        store_dot(b, s, w, o, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
+                  make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
+                  make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
       }
       total += __popcll(mk);
     }

@@ -168,6 +168,26 @@ int cms_copy_probes(cms_ctx* ctx, int complex_index, int32_t* atoms, double* hei
 int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r,
                 int64_t nr, double* out_d2, int32_t* out_idx);
 
+/* Backward pass of cms_contact_ms at frozen topology (dot existence,
+   burial, nearest partners and every subdivision count kept from the
+   forward pass; dot positions and areas differentiated through the one to
+   three atoms that generate each dot, recomputed here rather than stored).
+   g_totals: device, 2B upstream gradients of the totals (or NULL);
+   g_per_atom: device, ntot upstream gradients of the per-atom CMS (or
+   NULL); grad_xyz: device, 3*ntot, overwritten with dL/dxyz.  Requires the
+   last call on the context to have been cms_contact_ms on the current
+   surface.  No reference counterpart (the reference has no gradient). */
+int cms_backward_cms(cms_ctx* ctx, const double* g_totals, const double* g_per_atom, double* grad_xyz);
+
+/* CMS totals (device, 2B) at other atom coordinates xyz_eval (device,
+   3*ntot) with the topology of the current surface frozen -- the function
+   cms_backward_cms differentiates; used for finite-difference checks. */
+int cms_frozen_cms(cms_ctx* ctx, const double* xyz_eval, double* totals);
+
+/* Re-evaluates every dot from its descriptor at the forward coordinates
+   and counts those that differ from the stored dot (expected 0). */
+int cms_check_dots(cms_ctx* ctx, int64_t* mismatches, double* maxdev);
+
 /* Number of references rescored in FP64 by CMS_PREC_FP32_RESCORE searches
    since the last call (then reset).  No reference counterpart (the
    reference searches in FP64 only). */
