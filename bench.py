@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--kernel", default="attention", help="kernel timed for the roofline")
+    ap.add_argument("--no-graph", action="store_true", help="enqueue every step instead of replaying a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the CMS/SC path shards by complex across GPUs "
@@ -107,19 +108,27 @@ def main():
     prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_FP32_RESCORE
 
     def step():
+        # pack the resident coordinates, then surface + CMS (both sides) + SC
+        # in one call: one host wait per step, a graph replay once warm
         ctx("cms_load", B, nb_c, nt_c, P(d_xyz), P(d_rad), None, ctypes.byref(st))
-        ctx("cms_generate", 0, 0)
-        ctx("cms_contact_ms", prec, P(per_atom), P(totals))
-        ctx("cms_shape_complementarity", prec, P(scres))
+        ctx("cms_run", prec, P(per_atom), P(totals), P(scres))
 
+    use_graph = 0 if args.no_graph else 1
+    ctx("cms_set_graph", use_graph)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    # which kernels take the time (one profiled step, all kernels)
+    # which kernels take the time (one profiled step, enqueued eagerly: a
+    # graph replay records no events)
+    ctx("cms_set_graph", 0)
     ctx("cms_profile", b"*")
     step()
     torch.cuda.synchronize()
+    ctx("cms_profile", b"")
+    ctx("cms_set_graph", use_graph)
+    for _ in range(2):
+        step()
     names = ["attention", "region", "neighbors", "pairs_count", "pairs_emit", "probes_count",
              "probes_emit", "near", "convex_count", "convex_emit", "toroid_count", "toroid_emit",
              "concave_count", "concave_emit", "nearest", "nearest32", "owner_sums", "sc_sides"]
@@ -132,15 +141,23 @@ def main():
             breakdown[nm] = round(ms.value * 1e3, 2)
     log("per-kernel us in one step: %s" % json.dumps(breakdown))
 
-    # timed region, HIP events on the roofline kernel only
-    ctx("cms_profile", args.kernel.encode())
+    # timed region
     torch.cuda.synchronize()
-    log("timing %d steps" % args.steps)
+    log("timing %d steps (graph %s)" % (args.steps, "on" if use_graph else "off"))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    stats = (ctypes.c_int64 * 4)()
+    ctx("cms_run_stats", stats)
+    # the roofline kernel's launch duration: HIP events around that kernel
+    # on the context's stream, over the same number of eagerly enqueued
+    # steps (events cannot be recorded inside a graph replay)
+    ctx("cms_set_graph", 0)
+    ctx("cms_profile", args.kernel.encode())
+    for _ in range(args.steps):
+        step()
     ms = ctypes.c_double()
     cnt = ctypes.c_int64()
     ctx("cms_profile_read", args.kernel.encode(), ctypes.byref(ms), ctypes.byref(cnt))
@@ -214,6 +231,8 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_oracle_max_rel_err": max_err,
         "kernel_us_per_step": breakdown,
+        "graph": {"enabled": bool(use_graph), "captures": stats[0], "replays": stats[1],
+                  "exact_runs": stats[2], "overflow_reruns": stats[3]},
         "counts_per_complex": counts[0].tolist(),
     }
     print(json.dumps(out), flush=True)

@@ -49,6 +49,9 @@ _PROTOS = {
     "cms_contact_ms": (_c.c_int, [_vp, _c.c_int, _vp, _vp]),
     "cms_max_contact_ms": (_c.c_int, [_vp, _vp, _vp]),
     "cms_shape_complementarity": (_c.c_int, [_vp, _c.c_int, _vp]),
+    "cms_run": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp]),
+    "cms_set_graph": (_c.c_int, [_vp, _c.c_int]),
+    "cms_run_stats": (_c.c_int, [_vp, _vp]),
     "cms_dot_count": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.POINTER(_i64)]),
     "cms_copy_dots": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cms_copy_atom_state": (_c.c_int, [_vp, _vp, _vp]),

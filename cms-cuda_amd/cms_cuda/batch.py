@@ -91,6 +91,35 @@ class BatchCalculator:
         t = totals.cpu().numpy().reshape(self.B, 2)
         return dict(target=t[:, 1].copy(), binder=t[:, 0].copy(), per_atom=per_atom[:self.ntot].cpu().numpy())
 
+    def run(self, precision=_lib.PREC_FP64, sc=True, graph=False, out=None):
+        """One step of the hot path on the loaded batch (cms_run): surface,
+        CMS of both sides and, with sc=True, SC, with one host wait; with
+        graph=True repeated steps on the same layout replay a captured HIP
+        graph.  Returns dict(target, binder, per_atom, sc) as numpy arrays,
+        or None when out=(per_atom, totals, sc_records) device tensors are
+        given (sc_records may be None)."""
+        import torch
+        self.ctx("cms_set_graph", 1 if graph else 0)
+        if out is not None:
+            per_atom, totals, res = out
+        else:
+            per_atom = torch.empty(max(self.ntot, 1), dtype=torch.float64, device=self.dev)
+            totals = torch.empty(2 * self.B, dtype=torch.float64, device=self.dev)
+            res = torch.empty(self.B * _lib.SC_NRES, dtype=torch.float64, device=self.dev) if sc else None
+        self.ctx("cms_run", int(precision), _ptr(per_atom), _ptr(totals), _ptr(res))
+        self.mode = False
+        if out is not None:
+            return None
+        t = totals.cpu().numpy().reshape(self.B, 2)
+        return dict(target=t[:, 1].copy(), binder=t[:, 0].copy(), per_atom=per_atom[:self.ntot].cpu().numpy(),
+                    sc=res.cpu().numpy().reshape(self.B, _lib.SC_NRES) if res is not None else None)
+
+    def run_stats(self):
+        """(graphs captured, graph replays, exact runs, overflow reruns)."""
+        out = (ctypes.c_int64 * 4)()
+        self.ctx("cms_run_stats", out)
+        return tuple(out)
+
     def per_atom_split(self, per_atom):
         """Split a batch per-atom array into [(binder (N0,), target (N1,))]."""
         out = []

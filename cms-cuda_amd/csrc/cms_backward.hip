@@ -382,9 +382,10 @@ __device__ void eval_dot(const BatchView& b, const Settings& s, const double4* _
 
 // Forward dots re-evaluated from their descriptors must equal the stored
 // ones bit for bit; counts mismatches and the largest deviation.
-__global__ void k_check_dots(BatchView b, Settings s, int nD, const double4* __restrict__ dpos,
+__global__ void k_check_dots(BatchView b, Settings s, Count cnD, const double4* __restrict__ dpos,
                              const int4* __restrict__ dd1, const int4* __restrict__ dd2,
                              unsigned long long* __restrict__ nbad, double* __restrict__ maxdev) {
+  const int nD = cnD.get();
   double dev = 0.0;
   unsigned long long bad = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nD; i += gridDim.x * blockDim.x) {
@@ -407,11 +408,12 @@ __global__ void k_check_dots(BatchView b, Settings s, int nD, const double4* __r
 // Adjoint of every buried query dot and its partner:
 //   W = g_total[side] + g_atom[owner];  e = exp(-w d^2)
 //   dL/dA_q = W e,  dL/dP_q = -2 w W A_q e (P_q - P_r),  dL/dP_r -= the same.
-__global__ void k_cms_adjoint(int nB, const int* __restrict__ blist, const int* __restrict__ bgrp_of,
+__global__ void k_cms_adjoint(Count cnB, const int* __restrict__ blist, const int* __restrict__ bgrp_of,
                               const int* __restrict__ bgrp, const int* __restrict__ bidx,
                               const double* __restrict__ bmin, const double4* __restrict__ dpos,
                               const int* __restrict__ datom, const double* __restrict__ g_total,
                               const double* __restrict__ g_atom, double weight, double* __restrict__ adj) {
+  const int nB = cnB.get();
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nB; k += gridDim.x * blockDim.x) {
     const int ri = bidx[k];
     if (ri < 0) continue;  // no buried dot on the other side: contribution 0
@@ -436,9 +438,10 @@ __global__ void k_cms_adjoint(int nB, const int* __restrict__ blist, const int* 
 }
 
 // Vector-Jacobian product per dot: one forward-mode pass per generating atom.
-__global__ __launch_bounds__(64) void k_dot_grad(BatchView b, Settings s, int nD, const int4* __restrict__ dd1,
+__global__ __launch_bounds__(64) void k_dot_grad(BatchView b, Settings s, Count cnD, const int4* __restrict__ dd1,
                                                  const int4* __restrict__ dd2, const double* __restrict__ adj,
                                                  double* __restrict__ grad) {
+  const int nD = cnD.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nD; i += gridDim.x * blockDim.x) {
     const double ax = adj[4 * (int64_t)i], ay = adj[4 * (int64_t)i + 1], az = adj[4 * (int64_t)i + 2],
                  aa = adj[4 * (int64_t)i + 3];
@@ -462,9 +465,10 @@ __global__ __launch_bounds__(64) void k_dot_grad(BatchView b, Settings s, int nD
 
 // Frozen-topology CMS at other coordinates: positions and areas of every
 // buried dot re-evaluated, partners kept.
-__global__ void k_frozen_eval(BatchView b, Settings s, const double4* __restrict__ coords, int nB,
+__global__ void k_frozen_eval(BatchView b, Settings s, const double4* __restrict__ coords, Count cnB,
                               const int* __restrict__ blist, const int4* __restrict__ dd1,
                               const int4* __restrict__ dd2, double4* __restrict__ out) {
+  const int nB = cnB.get();
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nB; k += gridDim.x * blockDim.x) {
     const int d = blist[k];
     V3T<double> P;
@@ -474,9 +478,10 @@ __global__ void k_frozen_eval(BatchView b, Settings s, const double4* __restrict
   }
 }
 
-__global__ void k_frozen_cms(int nB, const int* __restrict__ bgrp_of, const int* __restrict__ bgrp,
+__global__ void k_frozen_cms(Count cnB, const int* __restrict__ bgrp_of, const int* __restrict__ bgrp,
                              const int* __restrict__ bidx, const double4* __restrict__ ev, double weight,
                              double* __restrict__ contrib) {
+  const int nB = cnB.get();
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nB; k += gridDim.x * blockDim.x) {
     const int ri = bidx[k];
     double v = 0.0;

@@ -91,6 +91,25 @@ struct cms_ctx {
   DBuf bidx, bgof, adj, eatom, evpos, chk;
   bool cms_ready = false;
   bool float_lists = false;
+  // per-run count block and capacities
+  DBuf meta;
+  int* hmeta = nullptr;             // pinned host copy: M_WORDS meta words, then B status words
+  int64_t hmeta_words = 0;
+  bool host_groups = false;         // h_dgrp / h_prgrp hold the current run's offsets
+  int64_t cap[M_WORDS] = {};        // element capacity of the buffers of stages R, P, Q, D
+  int64_t est[M_WORDS] = {};        // counts the current run's launch shapes are sized for
+  double ratio[2][M_WORDS] = {};    // largest count per atom seen (CMS mode, all-atoms mode)
+  bool learned[2] = {false, false};
+  uint64_t layout_epoch = 0;        // bumped when the batch layout changes
+  // one captured cms_run step
+  bool use_graph = false;
+  bool capturing = false;
+  hipStream_t gstream = nullptr;    // stream the graph is captured on and launched to
+  hipEvent_t gev0 = nullptr, gev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gkey, ekey;  // key of the captured graph / of the last eager run
+  int64_t n_captures = 0, n_replays = 0, n_exact = 0, n_overflow = 0;
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
   struct Ev { std::string name; hipEvent_t a, b; };
@@ -105,7 +124,7 @@ struct ProfScope {
   hipEvent_t a{}, b{};
   const char* name;
   ProfScope(cms_ctx* c, const char* n) : ctx(c), name(n) {
-    if (c->prof_filter.empty()) return;
+    if (c->prof_filter.empty() || c->capturing) return;
     if (c->prof_filter != "*" && c->prof_filter != n) return;
     if (hipEventCreate(&a) != hipSuccess) return;
     if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
@@ -176,7 +195,7 @@ static Work work(cms_ctx* ctx) {
   w.atten = ctx->atten.as<int8_t>();
   w.access = ctx->access.as<int8_t>();
   w.status = ctx->status.as<int>();
-  w.nR = ctx->rpos.as<int>() + ctx->ntot;
+  w.nR = ctx->meta.as<int>() + M_R;
   w.ratom = ctx->ratom.as<int>();
   w.rpos = ctx->rpos.as<int>();
   w.nn = ctx->nn.as<int>();
@@ -186,7 +205,7 @@ static Work work(cms_ctx* ctx) {
   w.nbmax = ctx->nbmax.as<int>();
   w.pcnt = ctx->pcnt.as<int>();
   w.pscan = ctx->pscan.as<int>();
-  w.nP = ctx->pscan.as<int>() + ctx->nR;
+  w.nP = ctx->meta.as<int>() + M_P;
   w.pi = ctx->pi.as<int>();
   w.pj = ctx->pj.as<int>();
   w.pu = ctx->pu.as<double4>();
@@ -195,7 +214,7 @@ static Work work(cms_ctx* ctx) {
   w.pgrp = ctx->pgrp.as<int>();
   w.prcnt = ctx->prcnt.as<int>();
   w.prscan = ctx->prscan.as<int>();
-  w.nQ = ctx->prscan.as<int>() + 2 * (int64_t)ctx->nP;
+  w.nQ = ctx->meta.as<int>() + M_Q;
   w.patom = ctx->patom.as<int4>();
   w.ppos = ctx->ppos.as<double4>();
   w.palt = ctx->palt.as<double4>();
@@ -219,8 +238,50 @@ static Work work(cms_ctx* ctx) {
   w.datom = ctx->datom.as<int>();
   w.ddesc = ctx->ddesc.as<int4>();
   w.ddesc2 = ctx->ddesc2.as<int4>();
-  w.dot_cap = ctx->nD;
+  w.pair_cap = ctx->cap[M_P];
+  w.probe_cap = ctx->cap[M_Q];
+  w.dot_cap = ctx->cap[M_D];
   return w;
+}
+
+static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
+  return {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
+                  &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
+                  &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
+                  &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->need, &ctx->rpos, &ctx->ratom,
+                  &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
+                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
+                  &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
+                  &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
+                  &ctx->cscan, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
+                  &ctx->tflag, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
+                  &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
+                  &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
+                  &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
+                  &ctx->bacc, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
+                  &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
+                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
+                  &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
+                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->meta};
+}
+
+// Identity of the workspace: a captured graph holds these addresses.
+static uint64_t workspace_signature(cms_ctx* ctx) {
+  uint64_t h = 1469598103934665603ull;
+  for (DBuf* b : all_bufs(ctx)) {
+    h = (h ^ reinterpret_cast<uintptr_t>(b->p)) * 1099511628211ull;
+    h = (h ^ (uint64_t)b->cap) * 1099511628211ull;
+  }
+  return h;
+}
+
+static void drop_graph(cms_ctx* ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->gkey.clear();
 }
 
 extern "C" {
@@ -251,15 +312,20 @@ int cms_create(int device, void* stream, cms_ctx** out) {
   cms_ctx* ctx = new cms_ctx();
   ctx->device = device;
   ctx->stream = static_cast<hipStream_t>(stream);
-  if (ctx->nresc.ensure(sizeof(unsigned long long) * 2) != hipSuccess) {
+  if (ctx->nresc.ensure(sizeof(unsigned long long) * 2) != hipSuccess ||
+      ctx->meta.ensure(sizeof(int) * M_WORDS) != hipSuccess) {
     g_create_err = "cms_create: out of device memory";
+    ctx->nresc.release();
+    ctx->meta.release();
     delete ctx;
     return CMS_ERR_HIP;
   }
   fill_u64(ctx->nresc.as<unsigned long long>(), 2, 0ull, ctx->stream);
+  fill_int(ctx->meta.as<int>(), M_WORDS, 0, ctx->stream);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     g_create_err = "cms_create: stream synchronisation failed";
     ctx->nresc.release();
+    ctx->meta.release();
     delete ctx;
     return CMS_ERR_HIP;
   }
@@ -271,26 +337,12 @@ int cms_destroy(cms_ctx* ctx) {
   if (!ctx) return CMS_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DBuf* bufs[] = {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
-                  &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
-                  &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
-                  &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->need, &ctx->rpos, &ctx->ratom,
-                  &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
-                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
-                  &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
-                  &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
-                  &ctx->cscan, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
-                  &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
-                  &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
-                  &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
-                  &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
-                  &ctx->bacc, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
-                  &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
-                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
-                  &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
-                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk};
-  for (DBuf* b : bufs) b->release();
+  drop_graph(ctx);
+  if (ctx->gev0) (void)hipEventDestroy(ctx->gev0);
+  if (ctx->gev1) (void)hipEventDestroy(ctx->gev1);
+  if (ctx->gstream) (void)hipStreamDestroy(ctx->gstream);
+  if (ctx->hmeta) (void)hipHostFree(ctx->hmeta);
+  for (DBuf* b : all_bufs(ctx)) b->release();
   delete ctx;
   return CMS_OK;
 }
@@ -312,26 +364,28 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
   if (!ctx) return CMS_ERR_INVALID;
   if (n_complex < 1 || !n_binder || !n_target || !settings)
     return fail(ctx, CMS_ERR_INVALID, "cms_load: invalid arguments");
+  if (ctx->capturing) return fail(ctx, CMS_ERR_STATE, "cms_load: not allowed during capture");
   CK(hipSetDevice(ctx->device));
-  ctx->B = n_complex;
-  ctx->h_aoff.assign(n_complex + 1, 0);
-  ctx->h_nb0.assign(n_complex, 0);
-  ctx->h_agrp.assign(2 * n_complex + 1, 0);
+  std::vector<int> aoff(n_complex + 1, 0), nb0(n_complex, 0), agrp(2 * n_complex + 1, 0);
   int64_t tot = 0;
   for (int c = 0; c < n_complex; ++c) {
     if (n_binder[c] < 0 || n_target[c] < 0)
       return fail(ctx, CMS_ERR_INVALID, "cms_load: negative atom count");
-    ctx->h_aoff[c] = (int)tot;
-    ctx->h_nb0[c] = n_binder[c];
-    ctx->h_agrp[2 * c] = (int)tot;
-    ctx->h_agrp[2 * c + 1] = (int)(tot + n_binder[c]);
+    aoff[c] = (int)tot;
+    nb0[c] = n_binder[c];
+    agrp[2 * c] = (int)tot;
+    agrp[2 * c + 1] = (int)(tot + n_binder[c]);
     tot += (int64_t)n_binder[c] + n_target[c];
     if (tot > (1ll << 30)) return fail(ctx, CMS_ERR_INVALID, "cms_load: too many atoms");
   }
-  ctx->h_aoff[n_complex] = (int)tot;
-  ctx->h_agrp[2 * n_complex] = (int)tot;
-  ctx->ntot = (int)tot;
+  aoff[n_complex] = (int)tot;
+  agrp[2 * n_complex] = (int)tot;
   if (tot > 0 && (!xyz || !radius)) return fail(ctx, CMS_ERR_INVALID, "cms_load: NULL coordinates");
+  // the same layout as the loaded batch (same complex sizes): the device
+  // offset tables stay, only the atoms are packed again
+  const bool same_layout = ctx->B == n_complex && aoff == ctx->h_aoff && nb0 == ctx->h_nb0 && ctx->aoff.p;
+  ctx->B = n_complex;
+  ctx->ntot = (int)tot;
   ctx->s.rp = settings->rp;
   ctx->s.band = settings->band;
   ctx->s.sep = settings->sep;
@@ -344,15 +398,21 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
   CK(ctx->dens.ensure(sizeof(double) * (n + 1)));
   CK(ctx->acplx.ensure(sizeof(int) * (n + 1)));
   CK(ctx->amol.ensure(n + 1));
-  CK(ctx->aoff.ensure(sizeof(int) * (n_complex + 1)));
-  CK(ctx->nb0.ensure(sizeof(int) * n_complex));
-  CK(ctx->agrp.ensure(sizeof(int) * (2 * n_complex + 1)));
-  CK(hipMemcpyAsync(ctx->aoff.p, ctx->h_aoff.data(), sizeof(int) * (n_complex + 1),
-                    hipMemcpyHostToDevice, ctx->stream));
-  CK(hipMemcpyAsync(ctx->nb0.p, ctx->h_nb0.data(), sizeof(int) * n_complex, hipMemcpyHostToDevice,
-                    ctx->stream));
-  CK(hipMemcpyAsync(ctx->agrp.p, ctx->h_agrp.data(), sizeof(int) * (2 * n_complex + 1),
-                    hipMemcpyHostToDevice, ctx->stream));
+  if (!same_layout) {
+    ctx->h_aoff.swap(aoff);
+    ctx->h_nb0.swap(nb0);
+    ctx->h_agrp.swap(agrp);
+    ++ctx->layout_epoch;
+    CK(ctx->aoff.ensure(sizeof(int) * (n_complex + 1)));
+    CK(ctx->nb0.ensure(sizeof(int) * n_complex));
+    CK(ctx->agrp.ensure(sizeof(int) * (2 * n_complex + 1)));
+    CK(hipMemcpyAsync(ctx->aoff.p, ctx->h_aoff.data(), sizeof(int) * (n_complex + 1),
+                      hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->nb0.p, ctx->h_nb0.data(), sizeof(int) * n_complex, hipMemcpyHostToDevice,
+                      ctx->stream));
+    CK(hipMemcpyAsync(ctx->agrp.p, ctx->h_agrp.data(), sizeof(int) * (2 * n_complex + 1),
+                      hipMemcpyHostToDevice, ctx->stream));
+  }
   if (n > 0) {
     k_pack_atoms<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(
         n, xyz, radius, density, ctx->density, ctx->aoff.as<int>(), ctx->nb0.as<int>(), n_complex,
@@ -361,43 +421,182 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
   }
   CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
   fill_u64(ctx->nresc.as<unsigned long long>(), 2, 0ull, ctx->stream);
-  // host arrays above are copied asynchronously: wait before returning
-  CK(hipStreamSynchronize(ctx->stream));
+  // host tables copied asynchronously: wait before they can change again
+  if (!same_layout) CK(hipStreamSynchronize(ctx->stream));
   ctx->generated = false;
   ctx->gen_all_atoms = -1;
   ctx->lists_ready = false;
   ctx->float_lists = false;
   ctx->cms_ready = false;
+  ctx->host_groups = false;
   return CMS_OK;
 }
 
-int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
-  if (!ctx) return CMS_ERR_INVALID;
-  if (ctx->B < 1) return fail(ctx, CMS_ERR_INVALID, "cms_generate: nothing loaded");
-  if (ctx->generated) {
-    if (ctx->gen_all_atoms != (all_atoms ? 1 : 0))
-      return fail(ctx, CMS_ERR_STATE,
-                  "This MolecularSurfaceCalculator instance already generated its molecular surface "
-                  "with all_atoms=%s, but was asked to reuse it with all_atoms=%s.",
-                  ctx->gen_all_atoms ? "True" : "False", all_atoms ? "True" : "False");
-    return CMS_OK;
+// ------------------------------------------------------------ capacities
+//
+// A run is enqueued in one of two ways.  Exact: after each stage the host
+// reads the stage total and sizes the next buffers to it (four waits).
+// Learned: every buffer is sized up front from the largest count per atom
+// seen so far for this mode, the whole run is enqueued without a host wait,
+// and k_stage_total checks each total against its buffer on the device; a
+// run that overflows is repeated exactly.  Launch shapes come from the same
+// estimates, so a learned run can be captured in a HIP graph.
+
+static int64_t headroom(int64_t v) { return v + v / 4 + 64; }
+
+static int size_rows(cms_ctx* ctx, int64_t c) {
+  CK(ctx->nn.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->nb.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->nbmax.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->nbr.ensure(sizeof(int) * (c * KCAP + 1)));
+  CK(ctx->bur.ensure(sizeof(int) * (c * BCAP + 1)));
+  CK(ctx->pcnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->pscan.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->ccnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->cscan.ensure(sizeof(int) * (c + 1)));
+  ctx->cap[M_R] = std::max(ctx->cap[M_R], c);
+  return CMS_OK;
+}
+
+static int size_pairs(cms_ctx* ctx, int64_t c) {
+  CK(ctx->pi.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->pj.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->pu.ensure(sizeof(double4) * (c + 1)));
+  CK(ctx->pt.ensure(sizeof(double4) * (c + 1)));
+  CK(ctx->pflag.ensure(c + 1));
+  CK(ctx->prcnt.ensure(sizeof(int) * (2 * c + 1)));
+  CK(ctx->prscan.ensure(sizeof(int) * (2 * c + 1)));
+  CK(ctx->t1cnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->t1scan.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->t2cnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->t2scan.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->tflag.ensure(c + 1));
+  ctx->cap[M_P] = std::max(ctx->cap[M_P], c);
+  return CMS_OK;
+}
+
+static int size_probes(cms_ctx* ctx, int64_t c) {
+  CK(ctx->patom.ensure(sizeof(int4) * (c + 1)));
+  CK(ctx->ppos.ensure(sizeof(double4) * (c + 1)));
+  CK(ctx->palt.ensure(sizeof(double4) * (c + 1)));
+  CK(ctx->nearl.ensure(sizeof(int) * (c * NEARCAP + 1)));
+  CK(ctx->nnear.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->kcnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->kscan.ensure(sizeof(int) * (c + 1)));
+  ctx->cap[M_Q] = std::max(ctx->cap[M_Q], c);
+  return CMS_OK;
+}
+
+// dots and every per-dot list of the readouts
+static int size_dots(cms_ctx* ctx, int64_t c) {
+  DBuf* d4[] = {&ctx->dpos, &ctx->dnml, &ctx->bxyz, &ctx->axyz, &ctx->txyz, &ctx->tnml};
+  for (DBuf* b : d4) CK(b->ensure(sizeof(double4) * (c + 1)));
+  DBuf* i4[] = {&ctx->ddesc, &ctx->ddesc2};
+  for (DBuf* b : i4) CK(b->ensure(sizeof(int4) * (c + 1)));
+  DBuf* f4[] = {&ctx->bxyz32, &ctx->axyz32, &ctx->txyz32};
+  for (DBuf* b : f4) CK(b->ensure(sizeof(float4) * (c + 1)));
+  DBuf* i1[] = {&ctx->datom, &ctx->bflag, &ctx->bscan, &ctx->aflag, &ctx->ascan, &ctx->blist, &ctx->alist,
+                &ctx->bidx, &ctx->operm, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->tidx, &ctx->tgof,
+                &ctx->bgof};
+  for (DBuf* b : i1) CK(b->ensure(sizeof(int) * (c + 1)));
+  DBuf* f8[] = {&ctx->bmin, &ctx->wv, &ctx->bacc, &ctx->tmin, &ctx->tdist, &ctx->trr};
+  for (DBuf* b : f8) CK(b->ensure(sizeof(double) * (c + 1)));
+  CK(ctx->dbur.ensure(c + 1));
+  CK(ctx->dtype.ensure(c + 1));
+  ctx->cap[M_D] = std::max(ctx->cap[M_D], c);
+  return CMS_OK;
+}
+
+// Read the stage total of an exact run and size the next buffers to it.
+static int exact_total(cms_ctx* ctx, int slot, int* out) {
+  CK(hipMemcpyAsync(out, ctx->meta.as<int>() + slot, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return CMS_OK;
+}
+
+// Learn from a run's counts (host copy of the meta words).
+static void learn(cms_ctx* ctx, int mode, const int* hm) {
+  const double n = std::max(ctx->ntot, 1);
+  for (int k = M_R; k <= M_T; ++k) {
+    const int v = std::max(hm[k], hm[M_RAW + k]);
+    ctx->ratio[mode][k] = std::max(ctx->ratio[mode][k], v / n);
   }
-  CK(hipSetDevice(ctx->device));
+}
+
+static int ensure_hmeta(cms_ctx* ctx) {
+  const int64_t words = M_WORDS + ctx->B;
+  if (ctx->hmeta && ctx->hmeta_words >= words) return CMS_OK;
+  if (ctx->hmeta) CK(hipHostFree(ctx->hmeta));
+  ctx->hmeta = nullptr;
+  ctx->hmeta_words = 0;
+  void* p = nullptr;
+  CK(hipHostMalloc(&p, sizeof(int) * (words + words / 4 + 64)));
+  ctx->hmeta = static_cast<int*>(p);
+  ctx->hmeta_words = words + words / 4 + 64;
+  return CMS_OK;
+}
+
+static const int CMS_RETRY = -1;  // internal: a learned run overflowed a buffer
+
+// Copy the count block and the status words to the host (one wait) and
+// evaluate them.
+static int finish_run(cms_ctx* ctx, hipStream_t st, int mode) {
+  int r = ensure_hmeta(ctx);
+  if (r) return r;
+  CK(hipMemcpyAsync(ctx->hmeta, ctx->meta.p, sizeof(int) * M_WORDS, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(ctx->hmeta + M_WORDS, ctx->status.p, sizeof(int) * ctx->B, hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  const int* hm = ctx->hmeta;
+  learn(ctx, mode, hm);
+  if (hm[M_OVF]) {
+    ++ctx->n_overflow;
+    return CMS_RETRY;
+  }
+  ctx->nR = hm[M_R];
+  ctx->nP = hm[M_P];
+  ctx->nQ = hm[M_Q];
+  ctx->nD = hm[M_D];
+  int worst = 0;
+  for (int c = 0; c < ctx->B; ++c) {
+    const int stc = hm[M_WORDS + c];
+    if (stc & ST_COINCIDENT) return fail(ctx, CMS_ERR_COINCIDENT, "coincident atoms in complex %d", c);
+    if (stc) worst = stc;
+  }
+  if (worst)
+    return fail(ctx, CMS_ERR_CAPACITY, "row capacity exceeded (status %d): more than %d neighbours, "
+                "%d buried partners or %d near low probes", worst, KCAP, BCAP, NEARCAP);
+  return CMS_OK;
+}
+
+// Launch shapes and buffer sizes of a learned run.
+static int plan_learned(cms_ctx* ctx, int mode) {
+  const int n = ctx->ntot;
+  int64_t* est = ctx->est;
+  for (int k = M_R; k <= M_T; ++k) est[k] = (int64_t)std::ceil(ctx->ratio[mode][k] * n);
+  for (int k = M_R; k <= M_D; ++k) est[k] = headroom(est[k]);
+  est[M_R] = std::min<int64_t>(est[M_R], n);
+  int r;
+  if ((r = size_rows(ctx, est[M_R])) || (r = size_pairs(ctx, est[M_P])) || (r = size_probes(ctx, est[M_Q])) ||
+      (r = size_dots(ctx, est[M_D])))
+    return r;
+  return CMS_OK;
+}
+
+static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact) {
   hipStream_t st = ctx->stream;
   const int B = ctx->B, G = 2 * B, n = ctx->ntot;
+  const int mode = all_atoms ? 1 : 0;
   BatchView b = view(ctx);
   const Settings s = ctx->s;
   ctx->lists_ready = false;
   ctx->float_lists = false;
   ctx->cms_ready = false;
+  ctx->host_groups = false;
 
   CK(ctx->atten.ensure(n + 1));
   CK(ctx->access.ensure(n + 1));
   CK(ctx->status.ensure(sizeof(int) * B));
   CK(ctx->coinc.ensure(sizeof(unsigned long long) * B));
-  CK(ctx->flag.ensure(sizeof(int) * (n + 1)));
-  CK(ctx->nbscan.ensure(sizeof(int) * (n + 1)));
-  CK(ctx->nbl.ensure(sizeof(int) * (n + 1)));
   CK(ctx->need.ensure(sizeof(int) * (n + 1)));
   CK(ctx->rpos.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ratom.ensure(sizeof(int) * (n + 1)));
@@ -407,22 +606,29 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
   CK(ctx->prgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->dgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->dbase.ensure(sizeof(int64_t) * 4 * G));
+  const int64_t cell_cap = 8ll * n + 64ll * B;
+  CK(ctx->grid.ensure(sizeof(GridInfo) * (B + 1)));
+  CK(ctx->gframe.ensure(sizeof(double4) * (B + 1)));
+  CK(ctx->ncellc.ensure(sizeof(int) * (B + 1)));
+  CK(ctx->ncscan.ensure(sizeof(int) * (B + 1)));
+  CK(ctx->acell.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->catom.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->ccount.ensure(sizeof(int) * (cell_cap + 1)));
+  CK(ctx->cstart.ensure(sizeof(int) * (cell_cap + 1)));
+  CK(ctx->ccur.ensure(sizeof(int) * (cell_cap + 1)));
   int* part = ctx->part.as<int>();
+  int* meta = ctx->meta.as<int>();
+  int64_t* est = ctx->est;
+  int r;
+  // overflow checks of an exact run never fire: it sizes after each total
+  const int64_t NOLIMIT = INT64_MAX;
+  if (!exact && (r = plan_learned(ctx, mode))) return r;
 
   fill_int(ctx->status.as<int>(), B, 0, st);
   fill_u64(ctx->coinc.as<unsigned long long>(), B, ~0ull, st);
+  fill_int(meta, M_WORDS, 0, st);
   if (n > 0) {
     // cell lists: per-complex grids, atoms sorted by cell
-    const int64_t cell_cap = 8ll * n + 64ll * B;
-    CK(ctx->grid.ensure(sizeof(GridInfo) * (B + 1)));
-    CK(ctx->gframe.ensure(sizeof(double4) * (B + 1)));
-    CK(ctx->ncellc.ensure(sizeof(int) * (B + 1)));
-    CK(ctx->ncscan.ensure(sizeof(int) * (B + 1)));
-    CK(ctx->acell.ensure(sizeof(int) * (n + 1)));
-    CK(ctx->catom.ensure(sizeof(int) * (n + 1)));
-    CK(ctx->ccount.ensure(sizeof(int) * (cell_cap + 1)));
-    CK(ctx->cstart.ensure(sizeof(int) * (cell_cap + 1)));
-    CK(ctx->ccur.ensure(sizeof(int) * (cell_cap + 1)));
     GridInfo* grid = ctx->grid.as<GridInfo>();
     TIMED("grid", k_grid_setup<<<grid_for(B, 1, 4096), 256, 0, st>>>(B, ctx->aoff.as<int>(), ctx->atom.as<double4>(), s,
                                                                       grid, ctx->gframe.as<double4>(),
@@ -456,124 +662,138 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
     scan_excl(ctx->need.as<int>(), ctx->rpos.as<int>(), n, part, st);
     k_compact_index<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->need.as<int>(),
                                                              ctx->rpos.as<int>(), ctx->ratom.as<int>());
-    CKL();
   } else {
     fill_int(ctx->rpos.as<int>(), 1, 0, st);
   }
-  int r = read_int(ctx, ctx->rpos.as<int>() + n, &ctx->nR);
-  if (r) return r;
-  const int nR = ctx->nR;
-  CK(ctx->nn.ensure(sizeof(int) * (nR + 1)));
-  CK(ctx->nb.ensure(sizeof(int) * (nR + 1)));
-  CK(ctx->nbmax.ensure(sizeof(int) * (nR + 1)));
-  CK(ctx->nbr.ensure(sizeof(int) * ((int64_t)nR * KCAP + 1)));
-  CK(ctx->bur.ensure(sizeof(int) * ((int64_t)nR * BCAP + 1)));
-  CK(ctx->pcnt.ensure(sizeof(int) * (nR + 1)));
-  CK(ctx->pscan.ensure(sizeof(int) * (nR + 1)));
-  CK(ctx->ccnt.ensure(sizeof(int) * (nR + 1)));
-  CK(ctx->cscan.ensure(sizeof(int) * (nR + 1)));
+  CKL();
+  // ---- region rows
+  k_stage_total<<<1, 256, 0, st>>>(ctx->rpos.as<int>(), host_count(n), exact ? NOLIMIT : ctx->cap[M_R], meta, M_R,
+                                    nullptr, 0);
+  CKL();
+  if (exact) {
+    int v = 0;
+    if ((r = exact_total(ctx, M_R, &v))) return r;
+    est[M_R] = v;
+    if ((r = size_rows(ctx, headroom(v)))) return r;
+  }
+  const Count cR = dev_count(meta + M_R), cP = dev_count(meta + M_P), cQ = dev_count(meta + M_Q);
   Work w = work(ctx);
-  if (nR > 0) {
-    TIMED("neighbors", k_neighbors_cells<<<grid_for(nR, 1), 64, 0, st>>>(b, s, w, ctx->grid.as<GridInfo>(),
-                                                                          ctx->cstart.as<int>(), ctx->catom.as<int>()));
-    TIMED("pairs_count", k_pairs<false><<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
+  if (est[M_R] > 0) {
+    TIMED("neighbors", k_neighbors_cells<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w, ctx->grid.as<GridInfo>(),
+                                                                                ctx->cstart.as<int>(),
+                                                                                ctx->catom.as<int>()));
+    TIMED("pairs_count", k_pairs<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
     CKL();
   }
-  scan_excl(ctx->pcnt.as<int>(), ctx->pscan.as<int>(), nR, part, st);
-  k_group_starts_from_atoms<<<grid_for(nR + 1, 256, 4096), 256, 0, st>>>(
-      nR, ctx->ratom.as<int>(), ctx->acplx.as<int>(), ctx->amol.as<int8_t>(), G, ctx->rgrp.as<int>());
+  scan_excl(ctx->pcnt.as<int>(), ctx->pscan.as<int>(), cR, ctx->cap[M_R], part, st);
+  k_group_starts_from_atoms<<<grid_for(est[M_R] + 1, 256, 4096), 256, 0, st>>>(
+      cR, ctx->ratom.as<int>(), ctx->acplx.as<int>(), ctx->amol.as<int8_t>(), G, ctx->rgrp.as<int>());
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->rgrp.as<int>(), 1,
                                                              ctx->pscan.as<int>(), ctx->pgrp.as<int>());
+  // ---- pairs
+  k_stage_total<<<1, 256, 0, st>>>(ctx->pscan.as<int>(), cR, exact ? NOLIMIT : ctx->cap[M_P], meta, M_P,
+                                    ctx->pgrp.as<int>(), G);
   CKL();
-  r = read_int(ctx, ctx->pscan.as<int>() + nR, &ctx->nP);
-  if (r) return r;
-  const int nP = ctx->nP;
-  CK(ctx->pi.ensure(sizeof(int) * (nP + 1)));
-  CK(ctx->pj.ensure(sizeof(int) * (nP + 1)));
-  CK(ctx->pu.ensure(sizeof(double4) * (nP + 1)));
-  CK(ctx->pt.ensure(sizeof(double4) * (nP + 1)));
-  CK(ctx->pflag.ensure(nP + 1));
-  CK(ctx->prcnt.ensure(sizeof(int) * (2 * (int64_t)nP + 1)));
-  CK(ctx->prscan.ensure(sizeof(int) * (2 * (int64_t)nP + 1)));
-  CK(ctx->t1cnt.ensure(sizeof(int) * (nP + 1)));
-  CK(ctx->t1scan.ensure(sizeof(int) * (nP + 1)));
-  CK(ctx->t2cnt.ensure(sizeof(int) * (nP + 1)));
-  CK(ctx->t2scan.ensure(sizeof(int) * (nP + 1)));
-  CK(ctx->tflag.ensure(nP + 1));
-  w = work(ctx);
-  if (nP > 0) {
-    TIMED("pairs_emit", k_pairs<true><<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
-    TIMED("probes_count", k_probes<false><<<grid_for(nP, 1), 64, 0, st>>>(b, s, w));
-    CKL();
+  if (exact) {
+    int v = 0;
+    if ((r = exact_total(ctx, M_P, &v))) return r;
+    est[M_P] = v;
+    if ((r = size_pairs(ctx, headroom(v)))) return r;
+    w = work(ctx);
   }
-  scan_excl(ctx->prcnt.as<int>(), ctx->prscan.as<int>(), 2 * (int64_t)nP, part, st);
+  if (est[M_R] > 0) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_P] > 0) TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  CKL();
+  scan_excl(ctx->prcnt.as<int>(), ctx->prscan.as<int>(), dev_count(meta + M_P, 2), 2 * ctx->cap[M_P], part, st);
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->pgrp.as<int>(), 2,
                                                              ctx->prscan.as<int>(), ctx->prgrp.as<int>());
+  // ---- probes
+  k_stage_total<<<1, 256, 0, st>>>(ctx->prscan.as<int>(), dev_count(meta + M_P, 2),
+                                    exact ? NOLIMIT : ctx->cap[M_Q], meta, M_Q, ctx->prgrp.as<int>(), G);
   CKL();
-  r = read_int(ctx, ctx->prscan.as<int>() + 2 * (int64_t)nP, &ctx->nQ);
-  if (r) return r;
-  const int nQ = ctx->nQ;
-  CK(ctx->patom.ensure(sizeof(int4) * (nQ + 1)));
-  CK(ctx->ppos.ensure(sizeof(double4) * (nQ + 1)));
-  CK(ctx->palt.ensure(sizeof(double4) * (nQ + 1)));
-  CK(ctx->nearl.ensure(sizeof(int) * ((int64_t)nQ * NEARCAP + 1)));
-  CK(ctx->nnear.ensure(sizeof(int) * (nQ + 1)));
-  CK(ctx->kcnt.ensure(sizeof(int) * (nQ + 1)));
-  CK(ctx->kscan.ensure(sizeof(int) * (nQ + 1)));
-  w = work(ctx);
-  if (nQ > 0) {
-    TIMED("probes_emit", k_probes<true><<<grid_for(nP, 1), 64, 0, st>>>(b, s, w));
-    TIMED("near", k_near<<<grid_for(nQ, 1), 64, 0, st>>>(b, s, w));
-    CKL();
+  if (exact) {
+    int v = 0;
+    if ((r = exact_total(ctx, M_Q, &v))) return r;
+    est[M_Q] = v;
+    if ((r = size_probes(ctx, headroom(v)))) return r;
+    w = work(ctx);
   }
-  if (nR > 0) TIMED("convex_count", k_convex<false><<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
-  if (nP > 0) TIMED("toroid_count", k_toroid<false><<<grid_for(nP, 1), 64, 0, st>>>(b, s, w));
-  if (nQ > 0 && s.rp > 0.0) TIMED("concave_count", k_concave<false><<<grid_for(nQ, 1), 64, 0, st>>>(b, s, w));
-  if (nQ > 0 && !(s.rp > 0.0)) fill_int(ctx->kcnt.as<int>(), nQ, 0, st);
+  if (est[M_P] > 0) TIMED("probes_emit", k_probes<true><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_Q] > 0) TIMED("near", k_near<<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
   CKL();
-  scan_excl(ctx->ccnt.as<int>(), ctx->cscan.as<int>(), nR, part, st);
-  scan_excl(ctx->t1cnt.as<int>(), ctx->t1scan.as<int>(), nP, part, st);
-  scan_excl(ctx->t2cnt.as<int>(), ctx->t2scan.as<int>(), nP, part, st);
-  scan_excl(ctx->kcnt.as<int>(), ctx->kscan.as<int>(), nQ, part, st);
+  // ---- dot counts
+  if (est[M_R] > 0) TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_P] > 0) TIMED("toroid_count", k_toroid<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_Q] > 0) {
+    if (s.rp > 0.0)
+      TIMED("concave_count", k_concave<false><<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
+    else
+      fill_int(ctx->kcnt.as<int>(), ctx->cap[M_Q], 0, st);
+  }
+  CKL();
+  scan_excl(ctx->ccnt.as<int>(), ctx->cscan.as<int>(), cR, ctx->cap[M_R], part, st);
+  scan_excl(ctx->t1cnt.as<int>(), ctx->t1scan.as<int>(), cP, ctx->cap[M_P], part, st);
+  scan_excl(ctx->t2cnt.as<int>(), ctx->t2scan.as<int>(), cP, ctx->cap[M_P], part, st);
+  scan_excl(ctx->kcnt.as<int>(), ctx->kscan.as<int>(), cQ, ctx->cap[M_Q], part, st);
   k_dot_bases<<<1, 256, 0, st>>>(G, ctx->rgrp.as<int>(), ctx->pgrp.as<int>(), ctx->prgrp.as<int>(),
                                  ctx->cscan.as<int>(), ctx->t1scan.as<int>(), ctx->t2scan.as<int>(),
                                  ctx->kscan.as<int>(), ctx->dbase.as<int64_t>(), ctx->dgrp.as<int>());
+  // ---- dots
+  k_stage_total<<<1, 256, 0, st>>>(ctx->dgrp.as<int>(), host_count(G), exact ? NOLIMIT : ctx->cap[M_D], meta, M_D,
+                                    ctx->dgrp.as<int>(), G);
   CKL();
-  r = read_int(ctx, ctx->dgrp.as<int>() + G, &ctx->nD);
-  if (r) return r;
-  const int nD = ctx->nD;
-  CK(ctx->dpos.ensure(sizeof(double4) * (nD + 1)));
-  CK(ctx->dnml.ensure(sizeof(double4) * (nD + 1)));
-  CK(ctx->dbur.ensure(nD + 1));
-  CK(ctx->dtype.ensure(nD + 1));
-  CK(ctx->datom.ensure(sizeof(int) * (nD + 1)));
-  CK(ctx->ddesc.ensure(sizeof(int4) * (nD + 1)));
-  CK(ctx->ddesc2.ensure(sizeof(int4) * (nD + 1)));
-  w = work(ctx);
-  if (nD > 0) {
-    if (nR > 0) TIMED("convex_emit", k_convex<true><<<grid_for(nR, 1), 64, 0, st>>>(b, s, w));
-    if (nP > 0) TIMED("toroid_emit", k_toroid<true><<<grid_for(nP, 1), 64, 0, st>>>(b, s, w));
-    if (nQ > 0 && s.rp > 0.0) TIMED("concave_emit", k_concave<true><<<grid_for(nQ, 1), 64, 0, st>>>(b, s, w));
-    CKL();
+  if (exact) {
+    int v = 0;
+    if ((r = exact_total(ctx, M_D, &v))) return r;
+    est[M_D] = v;
+    if ((r = size_dots(ctx, headroom(v)))) return r;
+    w = work(ctx);
+    // readout launch shapes: the previous run's counts per atom, or the dots
+    for (int k = M_B; k <= M_T; ++k) {
+      est[k] = (int64_t)std::ceil(ctx->ratio[mode][k] * n);
+      if (est[k] == 0) est[k] = v / 2;
+    }
   }
-  ctx->h_dgrp.assign(G + 1, 0);
-  ctx->h_prgrp.assign(G + 1, 0);
-  CK(hipMemcpyAsync(ctx->h_dgrp.data(), ctx->dgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, st));
-  CK(hipMemcpyAsync(ctx->h_prgrp.data(), ctx->prgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, st));
-  std::vector<int> stat(B);
-  CK(hipMemcpyAsync(stat.data(), ctx->status.p, sizeof(int) * B, hipMemcpyDeviceToHost, st));
-  CK(hipStreamSynchronize(st));
-  int worst = 0;
-  for (int c = 0; c < B; ++c) {
-    if (stat[c] & ST_COINCIDENT) return fail(ctx, CMS_ERR_COINCIDENT, "coincident atoms in complex %d", c);
-    if (stat[c]) worst = stat[c];
-  }
-  if (worst)
-    return fail(ctx, CMS_ERR_CAPACITY, "row capacity exceeded (status %d): more than %d neighbours, "
-                "%d buried partners or %d near low probes", worst, KCAP, BCAP, NEARCAP);
+  if (est[M_R] > 0) TIMED("convex_emit", k_convex<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_P] > 0) TIMED("toroid_emit", k_toroid<true><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_Q] > 0 && s.rp > 0.0)
+    TIMED("concave_emit", k_concave<true><<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
+  CKL();
   ctx->generated = true;
-  ctx->gen_all_atoms = all_atoms ? 1 : 0;
+  ctx->gen_all_atoms = mode;
+  if (exact) {
+    ++ctx->n_exact;
+    ctx->learned[mode] = true;
+  }
   return CMS_OK;
+}
+
+int cms_generate(cms_ctx* ctx, int all_atoms, int full_state) {
+  if (!ctx) return CMS_ERR_INVALID;
+  if (ctx->B < 1) return fail(ctx, CMS_ERR_INVALID, "cms_generate: nothing loaded");
+  if (ctx->generated) {
+    if (ctx->gen_all_atoms != (all_atoms ? 1 : 0))
+      return fail(ctx, CMS_ERR_STATE,
+                  "This MolecularSurfaceCalculator instance already generated its molecular surface "
+                  "with all_atoms=%s, but was asked to reuse it with all_atoms=%s.",
+                  ctx->gen_all_atoms ? "True" : "False", all_atoms ? "True" : "False");
+    return CMS_OK;
+  }
+  if (ctx->capturing) return fail(ctx, CMS_ERR_STATE, "cms_generate: not allowed during capture");
+  CK(hipSetDevice(ctx->device));
+  const int mode = all_atoms ? 1 : 0;
+  int r = generate_pass(ctx, all_atoms, full_state, !ctx->learned[mode]);
+  if (!r) r = finish_run(ctx, ctx->stream, mode);
+  if (r == CMS_RETRY) {
+    r = generate_pass(ctx, all_atoms, full_state, true);
+    if (!r) r = finish_run(ctx, ctx->stream, mode);
+    if (r == CMS_RETRY) r = fail(ctx, CMS_ERR_HIP, "cms_generate: exact run overflowed");
+  }
+  if (r) {
+    ctx->generated = false;
+    ctx->gen_all_atoms = -1;
+  }
+  return r;
 }
 
 int cms_status(cms_ctx* ctx, int32_t* status_host) {
@@ -603,67 +823,61 @@ int cms_coincident(cms_ctx* ctx, int c, int32_t* i, int32_t* j) {
 static int prepare_lists(cms_ctx* ctx) {
   if (ctx->lists_ready) return CMS_OK;
   hipStream_t st = ctx->stream;
-  const int G = 2 * ctx->B, nD = ctx->nD;
+  const int G = 2 * ctx->B;
+  const int64_t capD = ctx->cap[M_D], estD = ctx->est[M_D];
   int* part = ctx->part.as<int>();
-  CK(ctx->bflag.ensure(sizeof(int) * (nD + 1)));
-  CK(ctx->bscan.ensure(sizeof(int) * (nD + 1)));
-  CK(ctx->aflag.ensure(sizeof(int) * (nD + 1)));
-  CK(ctx->ascan.ensure(sizeof(int) * (nD + 1)));
+  int* meta = ctx->meta.as<int>();
+  const Count cD = dev_count(meta + M_D);
   CK(ctx->bgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->agrpd.ensure(sizeof(int) * (G + 1)));
-  if (nD > 0) {
-    k_dot_flag<<<grid_for(nD, 256, 4096), 256, 0, st>>>(nD, ctx->dbur.as<int8_t>(), 1, ctx->bflag.as<int>());
-    k_dot_flag<<<grid_for(nD, 256, 4096), 256, 0, st>>>(nD, ctx->dbur.as<int8_t>(), 0, ctx->aflag.as<int>());
+  if (estD > 0) {
+    k_dot_flag<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->dbur.as<int8_t>(), 1, ctx->bflag.as<int>());
+    k_dot_flag<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->dbur.as<int8_t>(), 0, ctx->aflag.as<int>());
   }
-  scan_excl(ctx->bflag.as<int>(), ctx->bscan.as<int>(), nD, part, st);
+  scan_excl(ctx->bflag.as<int>(), ctx->bscan.as<int>(), cD, capD, part, st);
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->dgrp.as<int>(), 1, ctx->bscan.as<int>(),
                                                              ctx->bgrp.as<int>());
-  scan_excl(ctx->aflag.as<int>(), ctx->ascan.as<int>(), nD, part, st);
+  scan_excl(ctx->aflag.as<int>(), ctx->ascan.as<int>(), cD, capD, part, st);
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->dgrp.as<int>(), 1, ctx->ascan.as<int>(),
                                                              ctx->agrpd.as<int>());
-  CKL();
-  int r = read_int(ctx, ctx->bscan.as<int>() + nD, &ctx->nB);
-  if (r) return r;
-  ctx->nA = nD - ctx->nB;
-  CK(ctx->blist.ensure(sizeof(int) * (ctx->nB + 1)));
-  CK(ctx->bxyz.ensure(sizeof(double4) * (ctx->nB + 1)));
-  CK(ctx->alist.ensure(sizeof(int) * (ctx->nA + 1)));
-  CK(ctx->axyz.ensure(sizeof(double4) * (ctx->nA + 1)));
-  if (nD > 0) {
-    k_dot_compact<<<grid_for(nD, 256, 4096), 256, 0, st>>>(nD, ctx->bflag.as<int>(), ctx->bscan.as<int>(),
-                                                            ctx->dpos.as<double4>(), ctx->blist.as<int>(),
-                                                            ctx->bxyz.as<double4>());
-    k_dot_compact<<<grid_for(nD, 256, 4096), 256, 0, st>>>(nD, ctx->aflag.as<int>(), ctx->ascan.as<int>(),
-                                                            ctx->dpos.as<double4>(), ctx->alist.as<int>(),
-                                                            ctx->axyz.as<double4>());
-    CKL();
+  k_list_totals<<<1, 64, 0, st>>>(ctx->bscan.as<int>(), meta);
+  if (estD > 0) {
+    k_dot_compact<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->bflag.as<int>(), ctx->bscan.as<int>(),
+                                                              ctx->dpos.as<double4>(), ctx->blist.as<int>(),
+                                                              ctx->bxyz.as<double4>());
+    k_dot_compact<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->aflag.as<int>(), ctx->ascan.as<int>(),
+                                                              ctx->dpos.as<double4>(), ctx->alist.as<int>(),
+                                                              ctx->axyz.as<double4>());
   }
+  CKL();
   ctx->lists_ready = true;
   return CMS_OK;
 }
 
 // nearest search of query groups qgrp over reference groups rgrp (g ^ xmask).
-// Small query sets split every group's references over several workgroups
-// so that a single complex still spreads over the GPU.
+// nq is read on the device; qcap bounds it.  Small query sets split every
+// group's references over several workgroups so that a single complex
+// still spreads over the GPU (the split comes from the expected sizes
+// nq_hint / nr_hint; any split gives the same result).
 static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const float4* Q32, const int* qgrp,
-                          int nq, int nr_total, const double4* R, const float4* R32, const int* rgrp, int xmask,
-                          int G, double* out_d2, int* out_idx) {
+                          Count nq, int64_t qcap, int64_t nq_hint, int64_t nr_hint, const double4* R,
+                          const float4* R32, const int* rgrp, int xmask, int G, double* out_d2, int* out_idx) {
   hipStream_t st = ctx->stream;
   CK(ctx->tcnt.ensure(sizeof(int) * (G + 1)));
   CK(ctx->toff.ensure(sizeof(int) * (G + 1)));
   k_tile_counts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, qgrp, 256, ctx->tcnt.as<int>());
   scan_excl(ctx->tcnt.as<int>(), ctx->toff.as<int>(), G, ctx->part.as<int>(), st);
-  const int64_t tiles_est = (int64_t)nq / 256 + G;
+  const int64_t tiles_est = nq_hint / 256 + G;
   const int64_t target = 2048;
   int nsplit = 1;
   if (tiles_est < target) {
     nsplit = (int)std::min<int64_t>((target + tiles_est - 1) / tiles_est, 64);
     // at least 128 references per split on average
-    const int64_t per_group_refs = std::max<int64_t>(1, (int64_t)nr_total / std::max(G, 1));
+    const int64_t per_group_refs = std::max<int64_t>(1, nr_hint / std::max(G, 1));
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, per_group_refs / 128));
   }
   const int grid = grid_for(tiles_est * nsplit, 1, 16384);
-  const int64_t ps = nsplit == 1 ? 0 : nq;
+  const int64_t ps = nsplit == 1 ? 0 : qcap;
   double* d2o = out_d2;
   int* ido = out_idx;
   if (nsplit > 1) {
@@ -687,8 +901,9 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
     else
       TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, nullptr));
   }
-  if (nsplit > 1 && nq > 0)
-    k_nearest_combine<<<grid_for(nq, 256, 4096), 256, 0, st>>>(nq, nsplit, ps, d2o, ido, out_d2, out_idx);
+  if (nsplit > 1)
+    k_nearest_combine<<<grid_for(std::max<int64_t>(nq_hint, 1), 256, 4096), 256, 0, st>>>(nq, nsplit, ps, d2o, ido,
+                                                                                           out_d2, out_idx);
   CKL();
   return CMS_OK;
 }
@@ -698,18 +913,19 @@ static int prepare_float_lists(cms_ctx* ctx) {
   if (ctx->float_lists) return CMS_OK;
   hipStream_t st = ctx->stream;
   const int B = ctx->B, G = 2 * B;
+  const int* meta = ctx->meta.as<int>();
   CK(ctx->frame.ensure(sizeof(double4) * (B + 1)));
   CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
-  CK(ctx->bxyz32.ensure(sizeof(float4) * (ctx->nB + 1)));
-  CK(ctx->axyz32.ensure(sizeof(float4) * (ctx->nA + 1)));
   k_complex_frame<<<grid_for(B, 1, 4096), 256, 0, st>>>(B, ctx->aoff.as<int>(), ctx->atom.as<double4>(), ctx->s.rp,
                                                          ctx->frame.as<double4>());
-  if (ctx->nB > 0)
-    k_to_float<<<grid_for(ctx->nB, 256, 4096), 256, 0, st>>>(ctx->nB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, 1,
-                                                              ctx->frame.as<double4>(), ctx->bxyz32.as<float4>());
-  if (ctx->nA > 0)
-    k_to_float<<<grid_for(ctx->nA, 256, 4096), 256, 0, st>>>(ctx->nA, ctx->axyz.as<double4>(), ctx->agrpd.as<int>(), G, 1,
-                                                              ctx->frame.as<double4>(), ctx->axyz32.as<float4>());
+  if (ctx->est[M_D] > 0) {
+    k_to_float<<<grid_for(ctx->est[M_B] + 1, 256, 4096), 256, 0, st>>>(
+        dev_count(meta + M_B), ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, 1, ctx->frame.as<double4>(),
+        ctx->bxyz32.as<float4>());
+    k_to_float<<<grid_for(ctx->est[M_A] + 1, 256, 4096), 256, 0, st>>>(
+        dev_count(meta + M_A), ctx->axyz.as<double4>(), ctx->agrpd.as<int>(), G, 1, ctx->frame.as<double4>(),
+        ctx->axyz32.as<float4>());
+  }
   CKL();
   ctx->float_lists = true;
   return CMS_OK;
@@ -721,24 +937,20 @@ __global__ void k_pick_stride(int n, const double* in, int stride, double* out) 
 }
 
 // per-atom sums in item order + per-group np.sum totals
-static int owner_sums(cms_ctx* ctx, int nitems, const int* list, const double* val, int64_t vstride,
-                      double* per_atom, double* totals, int G_tot, int gstride) {
+static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* list, const double* val,
+                      int64_t vstride, double* per_atom, double* totals, int G_tot, int gstride) {
   hipStream_t st = ctx->stream;
   const int n = ctx->ntot;
   CK(ctx->ocnt.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ostart.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ocur.ensure(sizeof(int) * (n + 1)));
-  CK(ctx->operm.ensure(sizeof(int) * (nitems + 1)));
   fill_int(ctx->ocnt.as<int>(), n, 0, st);
   fill_int(ctx->ocur.as<int>(), n, 0, st);
-  if (nitems > 0)
-    k_owner_count<<<grid_for(nitems, 256, 4096), 256, 0, st>>>(nitems, list, ctx->datom.as<int>(),
-                                                                ctx->ocnt.as<int>());
+  const int ig = grid_for(std::max<int64_t>(est_items, 1), 256, 4096);
+  k_owner_count<<<ig, 256, 0, st>>>(nitems, list, ctx->datom.as<int>(), ctx->ocnt.as<int>());
   scan_excl(ctx->ocnt.as<int>(), ctx->ostart.as<int>(), n, ctx->part.as<int>(), st);
-  if (nitems > 0)
-    k_owner_scatter<<<grid_for(nitems, 256, 4096), 256, 0, st>>>(nitems, list, ctx->datom.as<int>(),
-                                                                  ctx->ostart.as<int>(), ctx->ocur.as<int>(),
-                                                                  ctx->operm.as<int>());
+  k_owner_scatter<<<ig, 256, 0, st>>>(nitems, list, ctx->datom.as<int>(), ctx->ostart.as<int>(),
+                                      ctx->ocur.as<int>(), ctx->operm.as<int>());
   if (n > 0)
     TIMED("owner_sums", k_owner_sums<<<grid_for(n, 1, 16384), 64, 0, st>>>(n, ctx->ostart.as<int>(), ctx->operm.as<int>(), val,
                                                           vstride, per_atom));
@@ -754,6 +966,28 @@ static int owner_sums(cms_ctx* ctx, int nitems, const int* list, const double* v
   return CMS_OK;
 }
 
+static int contact_ms_pass(cms_ctx* ctx, int precision, double* per_atom, double* totals) {
+  int r = prepare_lists(ctx);
+  if (r) return r;
+  if (precision == CMS_PREC_FP32_RESCORE && (r = prepare_float_lists(ctx))) return r;
+  const int G = 2 * ctx->B;
+  hipStream_t st = ctx->stream;
+  const int* meta = ctx->meta.as<int>();
+  const Count cB = dev_count(meta + M_B);
+  const int64_t estB = ctx->est[M_B];
+  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), cB,
+                     ctx->cap[M_D], estB, estB, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(),
+                     ctx->bgrp.as<int>(), 1, G, ctx->bmin.as<double>(), ctx->bidx.as<int>());
+  if (r) return r;
+  k_cms_weights<<<grid_for(std::max<int64_t>(estB, 1), 256, 4096), 256, 0, st>>>(
+      cB, ctx->bxyz.as<double4>(), ctx->bmin.as<double>(), ctx->s.weight, ctx->wv.as<double>());
+  CKL();
+  r = owner_sums(ctx, cB, estB, ctx->blist.as<int>(), ctx->wv.as<double>(), 1, per_atom, totals, G, 1);
+  if (r) return r;
+  ctx->cms_ready = true;
+  return CMS_OK;
+}
+
 int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals) {
   if (!ctx || !per_atom || !totals) return CMS_ERR_INVALID;
   if (!ctx->generated || ctx->gen_all_atoms != 0)
@@ -761,26 +995,7 @@ int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals
   if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
     return fail(ctx, CMS_ERR_INVALID, "cms_contact_ms: unsupported precision");
   CK(hipSetDevice(ctx->device));
-  int r = prepare_lists(ctx);
-  if (r) return r;
-  if (precision == CMS_PREC_FP32_RESCORE && (r = prepare_float_lists(ctx))) return r;
-  const int G = 2 * ctx->B, nB = ctx->nB;
-  hipStream_t st = ctx->stream;
-  CK(ctx->bmin.ensure(sizeof(double) * (nB + 1)));
-  CK(ctx->bidx.ensure(sizeof(int) * (nB + 1)));
-  CK(ctx->wv.ensure(sizeof(double) * (nB + 1)));
-  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), nB, nB,
-                     ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), 1, G,
-                     ctx->bmin.as<double>(), ctx->bidx.as<int>());
-  if (r) return r;
-  if (nB > 0)
-    k_cms_weights<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bxyz.as<double4>(), ctx->bmin.as<double>(),
-                                                            ctx->s.weight, ctx->wv.as<double>());
-  CKL();
-  r = owner_sums(ctx, nB, ctx->blist.as<int>(), ctx->wv.as<double>(), 1, per_atom, totals, G, 1);
-  if (r) return r;
-  ctx->cms_ready = true;
-  return CMS_OK;
+  return contact_ms_pass(ctx, precision, per_atom, totals);
 }
 
 int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals) {
@@ -790,7 +1005,55 @@ int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals) {
   CK(hipSetDevice(ctx->device));
   // items: all dots; value: dpos[i].w (area)
   const double* val = reinterpret_cast<const double*>(ctx->dpos.as<double4>()) + 3;
-  return owner_sums(ctx, ctx->nD, nullptr, val, 4, per_atom, totals, 2 * ctx->B, 2);
+  return owner_sums(ctx, dev_count(ctx->meta.as<int>() + M_D), ctx->est[M_D], nullptr, val, 4, per_atom, totals,
+                    2 * ctx->B, 2);
+}
+
+static int sc_pass(cms_ctx* ctx, int precision, double* results) {
+  int r = prepare_lists(ctx);
+  if (r) return r;
+  if (precision == CMS_PREC_FP32_RESCORE && (r = prepare_float_lists(ctx))) return r;
+  hipStream_t st = ctx->stream;
+  const int B = ctx->B, G = 2 * B;
+  int* part = ctx->part.as<int>();
+  int* meta = ctx->meta.as<int>();
+  const Count cB = dev_count(meta + M_B), cT = dev_count(meta + M_T);
+  const int64_t capD = ctx->cap[M_D], estB = ctx->est[M_B], estA = ctx->est[M_A], estT = ctx->est[M_T];
+  CK(ctx->tgrp.ensure(sizeof(int) * (G + 1)));
+  CK(ctx->side.ensure(sizeof(double) * 8 * G));
+  // trim: buried dots of g against accessible dots of g
+  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), cB,
+                     capD, estB, estA, ctx->axyz.as<double4>(), ctx->axyz32.as<float4>(), ctx->agrpd.as<int>(), 0,
+                     G, ctx->bacc.as<double>(), nullptr);
+  if (r) return r;
+  const int gB = grid_for(std::max<int64_t>(estB, 1), 256, 4096);
+  k_trim_flag<<<gB, 256, 0, st>>>(cB, ctx->bacc.as<double>(), ctx->s.band * ctx->s.band, ctx->trflag.as<int>());
+  scan_excl(ctx->trflag.as<int>(), ctx->trscan.as<int>(), cB, capD, part, st);
+  k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->bgrp.as<int>(), 1, ctx->trscan.as<int>(),
+                                                             ctx->tgrp.as<int>());
+  k_copy_word<<<1, 64, 0, st>>>(ctx->trscan.as<int>(), cB, meta + M_T);
+  k_trim_compact<<<gB, 256, 0, st>>>(cB, ctx->trflag.as<int>(), ctx->trscan.as<int>(), ctx->blist.as<int>(),
+                                     ctx->dpos.as<double4>(), ctx->dnml.as<double4>(), ctx->tlist.as<int>(),
+                                     ctx->txyz.as<double4>(), ctx->tnml.as<double4>());
+  CKL();
+  const int gT = grid_for(std::max<int64_t>(estT, 1), 256, 4096);
+  if (precision == CMS_PREC_FP32_RESCORE)
+    k_to_float<<<gT, 256, 0, st>>>(cT, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), G, 1, ctx->frame.as<double4>(),
+                                   ctx->txyz32.as<float4>());
+  r = launch_nearest(ctx, precision, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), cT, capD,
+                     estT, estT, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), 1, G,
+                     ctx->tmin.as<double>(), ctx->tidx.as<int>());
+  if (r) return r;
+  k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->tgof.as<int>());
+  k_sc_terms<<<gT, 256, 0, st>>>(cT, ctx->tgof.as<int>(), ctx->tgrp.as<int>(), ctx->tmin.as<double>(),
+                                 ctx->tidx.as<int>(), ctx->tnml.as<double4>(), ctx->s.weight, ctx->tdist.as<double>(),
+                                 ctx->trr.as<double>());
+  TIMED("sc_sides", k_sc_sides<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->txyz.as<double4>(),
+                                                    ctx->tdist.as<double>(), ctx->trr.as<double>(),
+                                                    ctx->dgrp.as<int>(), ctx->s, ctx->side.as<double>()));
+  k_sc_combine<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->side.as<double>(), results);
+  CKL();
+  return CMS_OK;
 }
 
 int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
@@ -800,67 +1063,153 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
   if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
     return fail(ctx, CMS_ERR_INVALID, "cms_shape_complementarity: unsupported precision");
   CK(hipSetDevice(ctx->device));
-  int r = prepare_lists(ctx);
-  if (r) return r;
-  if (precision == CMS_PREC_FP32_RESCORE && (r = prepare_float_lists(ctx))) return r;
-  hipStream_t st = ctx->stream;
-  const int B = ctx->B, G = 2 * B, nB = ctx->nB;
-  int* part = ctx->part.as<int>();
-  CK(ctx->bacc.ensure(sizeof(double) * (nB + 1)));
-  CK(ctx->trflag.ensure(sizeof(int) * (nB + 1)));
-  CK(ctx->trscan.ensure(sizeof(int) * (nB + 1)));
-  CK(ctx->tgrp.ensure(sizeof(int) * (G + 1)));
-  CK(ctx->side.ensure(sizeof(double) * 8 * G));
-  // trim: buried dots of g against accessible dots of g
-  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), nB,
-                     ctx->nA, ctx->axyz.as<double4>(), ctx->axyz32.as<float4>(), ctx->agrpd.as<int>(), 0, G,
-                     ctx->bacc.as<double>(), nullptr);
-  if (r) return r;
-  if (nB > 0)
-    k_trim_flag<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bacc.as<double>(), ctx->s.band * ctx->s.band,
-                                                          ctx->trflag.as<int>());
-  scan_excl(ctx->trflag.as<int>(), ctx->trscan.as<int>(), nB, part, st);
-  k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->bgrp.as<int>(), 1, ctx->trscan.as<int>(),
-                                                             ctx->tgrp.as<int>());
-  CKL();
-  int nT = 0;
-  r = read_int(ctx, ctx->trscan.as<int>() + nB, &nT);
-  if (r) return r;
-  CK(ctx->tlist.ensure(sizeof(int) * (nT + 1)));
-  CK(ctx->txyz.ensure(sizeof(double4) * (nT + 1)));
-  CK(ctx->tnml.ensure(sizeof(double4) * (nT + 1)));
-  CK(ctx->tmin.ensure(sizeof(double) * (nT + 1)));
-  CK(ctx->tidx.ensure(sizeof(int) * (nT + 1)));
-  CK(ctx->tgof.ensure(sizeof(int) * (nT + 1)));
-  CK(ctx->tdist.ensure(sizeof(double) * (nT + 1)));
-  CK(ctx->trr.ensure(sizeof(double) * (nT + 1)));
-  if (nB > 0)
-    k_trim_compact<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->trflag.as<int>(), ctx->trscan.as<int>(),
-                                                             ctx->blist.as<int>(), ctx->dpos.as<double4>(),
-                                                             ctx->dnml.as<double4>(), ctx->tlist.as<int>(),
-                                                             ctx->txyz.as<double4>(), ctx->tnml.as<double4>());
-  CKL();
-  if (precision == CMS_PREC_FP32_RESCORE) {
-    CK(ctx->txyz32.ensure(sizeof(float4) * (nT + 1)));
-    if (nT > 0)
-      k_to_float<<<grid_for(nT, 256, 4096), 256, 0, st>>>(nT, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), G, 1,
-                                                           ctx->frame.as<double4>(), ctx->txyz32.as<float4>());
+  return sc_pass(ctx, precision, results);
+}
+
+// ------------------------------------------------------------ one step
+
+static int readouts_pass(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc) {
+  int r = contact_ms_pass(ctx, precision, per_atom, totals);
+  if (!r && sc) r = sc_pass(ctx, precision, sc);
+  return r;
+}
+
+// Everything the launches of a learned step depend on besides device data.
+static std::vector<uint64_t> run_key(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc) {
+  std::vector<uint64_t> k = {(uint64_t)ctx->B,        (uint64_t)ctx->ntot,     (uint64_t)precision,
+                             (uint64_t)(uintptr_t)per_atom, (uint64_t)(uintptr_t)totals,
+                             (uint64_t)(uintptr_t)sc,  ctx->layout_epoch,       workspace_signature(ctx)};
+  for (int i = M_R; i <= M_T; ++i) k.push_back((uint64_t)ctx->est[i]);
+  const double sv[6] = {ctx->s.rp, ctx->s.band, ctx->s.sep, ctx->s.weight, ctx->s.binwidth_dist,
+                        ctx->s.binwidth_norm};
+  for (double v : sv) {
+    uint64_t u;
+    memcpy(&u, &v, sizeof u);
+    k.push_back(u);
   }
-  r = launch_nearest(ctx, precision, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), nT, nT,
-                     ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), 1, G,
-                     ctx->tmin.as<double>(), ctx->tidx.as<int>());
-  if (r) return r;
-  k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->tgof.as<int>());
-  if (nT > 0)
-    k_sc_terms<<<grid_for(nT, 256, 4096), 256, 0, st>>>(nT, ctx->tgof.as<int>(), ctx->tgrp.as<int>(),
-                                                         ctx->tmin.as<double>(), ctx->tidx.as<int>(),
-                                                         ctx->tnml.as<double4>(), ctx->s.weight,
-                                                         ctx->tdist.as<double>(), ctx->trr.as<double>());
-  TIMED("sc_sides", k_sc_sides<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->txyz.as<double4>(),
-                                                    ctx->tdist.as<double>(), ctx->trr.as<double>(),
-                                                    ctx->dgrp.as<int>(), ctx->s, ctx->side.as<double>()));
-  k_sc_combine<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->side.as<double>(), results);
-  CKL();
+  return k;
+}
+
+static int ensure_gstream(cms_ctx* ctx) {
+  if (!ctx->gstream) CK(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+  if (!ctx->gev0) CK(hipEventCreateWithFlags(&ctx->gev0, hipEventDisableTiming));
+  if (!ctx->gev1) CK(hipEventCreateWithFlags(&ctx->gev1, hipEventDisableTiming));
+  return CMS_OK;
+}
+
+// Capture one learned step on the graph stream.  Any failure (an
+// allocation or a wait that capture refuses) leaves no graph behind and the
+// caller runs the step eagerly.
+static bool capture_step(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc,
+                         const std::vector<uint64_t>& key) {
+  drop_graph(ctx);
+  if (ensure_gstream(ctx)) return false;
+  hipStream_t saved = ctx->stream;
+  ctx->stream = ctx->gstream;
+  ctx->capturing = true;
+  int r = hipStreamBeginCapture(ctx->gstream, hipStreamCaptureModeThreadLocal) == hipSuccess ? CMS_OK : CMS_ERR_HIP;
+  const bool began = r == CMS_OK;
+  if (!r) r = generate_pass(ctx, 0, 0, false);
+  if (!r) r = readouts_pass(ctx, precision, per_atom, totals, sc);
+  hipGraph_t g = nullptr;
+  hipError_t e = began ? hipStreamEndCapture(ctx->gstream, &g) : hipErrorUnknown;
+  ctx->capturing = false;
+  ctx->stream = saved;
+  (void)hipGetLastError();
+  const bool same_ws = run_key(ctx, precision, per_atom, totals, sc) == key;
+  if (r || e != hipSuccess || !g || !same_ws) {
+    if (g) (void)hipGraphDestroy(g);
+    return false;
+  }
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess || !ex) {
+    (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return false;
+  }
+  ctx->graph = g;
+  ctx->gexec = ex;
+  ctx->gkey = key;
+  ++ctx->n_captures;
+  return true;
+}
+
+int cms_run(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc_results) {
+  if (!ctx || !per_atom || !totals) return CMS_ERR_INVALID;
+  if (ctx->B < 1) return fail(ctx, CMS_ERR_INVALID, "cms_run: nothing loaded");
+  if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
+    return fail(ctx, CMS_ERR_INVALID, "cms_run: unsupported precision");
+  if (ctx->capturing) return fail(ctx, CMS_ERR_STATE, "cms_run: not allowed during capture");
+  CK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  ctx->generated = false;
+  int r;
+  if (!ctx->learned[0]) {
+    r = generate_pass(ctx, 0, 0, true);
+    if (!r) r = readouts_pass(ctx, precision, per_atom, totals, sc_results);
+    if (!r) r = finish_run(ctx, st, 0);
+  } else {
+    if ((r = plan_learned(ctx, 0))) return r;
+    const std::vector<uint64_t> key = run_key(ctx, precision, per_atom, totals, sc_results);
+    bool replay = false;
+    if (ctx->use_graph) {
+      if (ctx->gexec && key == ctx->gkey) replay = true;
+      else if (key == ctx->ekey) replay = capture_step(ctx, precision, per_atom, totals, sc_results, key);
+    }
+    hipStream_t fin = st;
+    if (replay) {
+      CK(hipEventRecord(ctx->gev0, st));
+      CK(hipStreamWaitEvent(ctx->gstream, ctx->gev0, 0));
+      CK(hipGraphLaunch(ctx->gexec, ctx->gstream));
+      ++ctx->n_replays;
+      fin = ctx->gstream;
+      ctx->generated = true;
+      ctx->gen_all_atoms = 0;
+      ctx->lists_ready = true;
+      ctx->float_lists = precision == CMS_PREC_FP32_RESCORE;
+      ctx->cms_ready = true;
+      ctx->host_groups = false;
+    } else {
+      r = generate_pass(ctx, 0, 0, false);
+      if (!r) r = readouts_pass(ctx, precision, per_atom, totals, sc_results);
+      if (r) {
+        ctx->generated = false;
+        return r;
+      }
+      ctx->ekey = key;
+    }
+    r = finish_run(ctx, fin, 0);
+    if (r == CMS_RETRY) {
+      // a buffer was too small for this batch: repeat exactly (learns the
+      // new sizes; the next step captures again)
+      drop_graph(ctx);
+      ctx->ekey.clear();
+      r = generate_pass(ctx, 0, 0, true);
+      if (!r) r = readouts_pass(ctx, precision, per_atom, totals, sc_results);
+      if (!r) r = finish_run(ctx, st, 0);
+    }
+  }
+  if (r == CMS_RETRY) r = fail(ctx, CMS_ERR_HIP, "cms_run: exact run overflowed");
+  if (r) {
+    ctx->generated = false;
+    ctx->gen_all_atoms = -1;
+  }
+  return r;
+}
+
+int cms_set_graph(cms_ctx* ctx, int enable) {
+  if (!ctx) return CMS_ERR_INVALID;
+  ctx->use_graph = enable != 0;
+  if (!ctx->use_graph) drop_graph(ctx);
+  return CMS_OK;
+}
+
+int cms_run_stats(cms_ctx* ctx, int64_t* out) {
+  if (!ctx || !out) return CMS_ERR_INVALID;
+  out[0] = ctx->n_captures;
+  out[1] = ctx->n_replays;
+  out[2] = ctx->n_exact;
+  out[3] = ctx->n_overflow;
   return CMS_OK;
 }
 
@@ -881,9 +1230,24 @@ __global__ void k_unpack_dots(int n, int lo, int aoff, const double4* dpos, cons
   }
 }
 
+// Host copies of the dot and probe group offsets, fetched on first use.
+static int ensure_host_groups(cms_ctx* ctx) {
+  if (ctx->host_groups) return CMS_OK;
+  const int G = 2 * ctx->B;
+  ctx->h_dgrp.assign(G + 1, 0);
+  ctx->h_prgrp.assign(G + 1, 0);
+  CK(hipMemcpyAsync(ctx->h_dgrp.data(), ctx->dgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipMemcpyAsync(ctx->h_prgrp.data(), ctx->prgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->host_groups = true;
+  return CMS_OK;
+}
+
 int cms_dot_count(cms_ctx* ctx, int c, int mol, int64_t* n) {
   if (!ctx || !n || c < 0 || c >= ctx->B || mol < 0 || mol > 1) return CMS_ERR_INVALID;
   if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_dot_count: no surface");
+  int r = ensure_host_groups(ctx);
+  if (r) return r;
   const int g = 2 * c + mol;
   *n = ctx->h_dgrp[g + 1] - ctx->h_dgrp[g];
   return CMS_OK;
@@ -928,6 +1292,8 @@ __global__ void k_unpack_probes(int n, int lo, int aoff, const int4* patom, cons
 int cms_probe_count(cms_ctx* ctx, int c, int64_t* n) {
   if (!ctx || !n || c < 0 || c >= ctx->B) return CMS_ERR_INVALID;
   if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_probe_count: no surface");
+  int r = ensure_host_groups(ctx);
+  if (r) return r;
   *n = ctx->h_prgrp[2 * c + 2] - ctx->h_prgrp[2 * c];
   return CMS_OK;
 }
@@ -950,6 +1316,7 @@ int cms_counts(cms_ctx* ctx, int64_t* out) {
   const int B = ctx->B, G = 2 * B;
   hipStream_t st = ctx->stream;
   int r = prepare_lists(ctx);
+  if (!r) r = ensure_host_groups(ctx);
   if (r) return r;
   std::vector<int> bg(G + 1), rg(G + 1), pg(G + 1), prg(G + 1);
   std::vector<int> cs(ctx->nR + 1), t1(ctx->nP + 1), t2(ctx->nP + 1), ks(ctx->nQ + 1);
@@ -1040,14 +1407,14 @@ int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const 
     CK(ctx->rpack32.ensure(sizeof(float4) * (nr + 1)));
     k_points_frame<<<1, 256, 0, st>>>((int)nq, ctx->qpack.as<double4>(), (int)nr, ctx->rpack.as<double4>(),
                                       ctx->frame.as<double4>());
-    k_to_float<<<grid_for(nq, 256, 4096), 256, 0, st>>>((int)nq, ctx->qpack.as<double4>(), nullptr, 1, 0,
+    k_to_float<<<grid_for(nq, 256, 4096), 256, 0, st>>>(host_count((int)nq), ctx->qpack.as<double4>(), nullptr, 1, 0,
                                                          ctx->frame.as<double4>(), ctx->qpack32.as<float4>());
     if (nr > 0)
-      k_to_float<<<grid_for(nr, 256, 4096), 256, 0, st>>>((int)nr, ctx->rpack.as<double4>(), nullptr, 1, 0,
+      k_to_float<<<grid_for(nr, 256, 4096), 256, 0, st>>>(host_count((int)nr), ctx->rpack.as<double4>(), nullptr, 1, 0,
                                                            ctx->frame.as<double4>(), ctx->rpack32.as<float4>());
   }
   int rc = launch_nearest(ctx, precision, ctx->qpack.as<double4>(), ctx->qpack32.as<float4>(), ctx->grp1.as<int>(),
-                          (int)nq, (int)nr, ctx->rpack.as<double4>(), ctx->rpack32.as<float4>(),
+                          host_count((int)nq), nq, nq, nr, ctx->rpack.as<double4>(), ctx->rpack32.as<float4>(),
                           ctx->grp1.as<int>() + 2, 0, 1, out_d2, out_idx);
   if (rc) return rc;
   CK(hipStreamSynchronize(st));
@@ -1102,7 +1469,6 @@ int cms_profile_read(cms_ctx* ctx, const char* name, double* total_ms, int64_t* 
 
 static int buried_groups(cms_ctx* ctx) {
   const int G = 2 * ctx->B;
-  CK(ctx->bgof.ensure(sizeof(int) * (ctx->nB + 1)));
   k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, ctx->stream>>>(G, ctx->bgrp.as<int>(), ctx->bgof.as<int>());
   CKL();
   return CMS_OK;
@@ -1114,21 +1480,20 @@ int cms_backward_cms(cms_ctx* ctx, const double* g_totals, const double* g_per_a
     return fail(ctx, CMS_ERR_STATE, "cms_backward_cms: run cms_contact_ms on the current surface first");
   CK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int nD = ctx->nD, nB = ctx->nB;
+  const int* meta = ctx->meta.as<int>();
+  const int64_t capD = ctx->cap[M_D], estD = ctx->est[M_D], estB = ctx->est[M_B];
   int r = buried_groups(ctx);
   if (r) return r;
-  CK(ctx->adj.ensure(sizeof(double) * 4 * ((int64_t)nD + 1)));
-  fill_double(ctx->adj.as<double>(), 4 * (int64_t)nD, 0.0, st);
+  CK(ctx->adj.ensure(sizeof(double) * 4 * (capD + 1)));
+  fill_double(ctx->adj.as<double>(), 4 * capD, 0.0, st);
   fill_double(grad_xyz, 3 * (int64_t)ctx->ntot, 0.0, st);
-  if (nB > 0)
-    TIMED("cms_adjoint", k_cms_adjoint<<<grid_for(nB, 256, 4096), 256, 0, st>>>(
-        nB, ctx->blist.as<int>(), ctx->bgof.as<int>(), ctx->bgrp.as<int>(), ctx->bidx.as<int>(),
-        ctx->bmin.as<double>(), ctx->dpos.as<double4>(), ctx->datom.as<int>(), g_totals, g_per_atom,
-        ctx->s.weight, ctx->adj.as<double>()));
-  if (nD > 0)
-    TIMED("dot_grad", k_dot_grad<<<grid_for(nD, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, nD,
-                                                                          ctx->ddesc.as<int4>(), ctx->ddesc2.as<int4>(),
-                                                                          ctx->adj.as<double>(), grad_xyz));
+  TIMED("cms_adjoint", k_cms_adjoint<<<grid_for(std::max<int64_t>(estB, 1), 256, 4096), 256, 0, st>>>(
+      dev_count(meta + M_B), ctx->blist.as<int>(), ctx->bgof.as<int>(), ctx->bgrp.as<int>(), ctx->bidx.as<int>(),
+      ctx->bmin.as<double>(), ctx->dpos.as<double4>(), ctx->datom.as<int>(), g_totals, g_per_atom,
+      ctx->s.weight, ctx->adj.as<double>()));
+  TIMED("dot_grad", k_dot_grad<<<grid_for(std::max<int64_t>(estD, 1), 64, 16384), 64, 0, st>>>(
+      view(ctx), ctx->s, dev_count(meta + M_D), ctx->ddesc.as<int4>(), ctx->ddesc2.as<int4>(),
+      ctx->adj.as<double>(), grad_xyz));
   CKL();
   return CMS_OK;
 }
@@ -1144,22 +1509,21 @@ int cms_frozen_cms(cms_ctx* ctx, const double* xyz_eval, double* totals) {
     return fail(ctx, CMS_ERR_STATE, "cms_frozen_cms: run cms_contact_ms on the current surface first");
   CK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int n = ctx->ntot, nB = ctx->nB, G = 2 * ctx->B;
+  const int n = ctx->ntot, G = 2 * ctx->B;
+  const Count cB = dev_count(ctx->meta.as<int>() + M_B);
+  const int64_t estB = std::max<int64_t>(ctx->est[M_B], 1);
   int r = buried_groups(ctx);
   if (r) return r;
   CK(ctx->eatom.ensure(sizeof(double4) * (n + 1)));
-  CK(ctx->evpos.ensure(sizeof(double4) * (nB + 1)));
+  CK(ctx->evpos.ensure(sizeof(double4) * (ctx->cap[M_D] + 1)));
   if (n > 0)
     k_pack_eval<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, xyz_eval, ctx->atom.as<double4>(), ctx->eatom.as<double4>());
-  CK(ctx->wv.ensure(sizeof(double) * (nB + 1)));
-  if (nB > 0) {
-    k_frozen_eval<<<grid_for(nB, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, ctx->eatom.as<double4>(), nB,
+  k_frozen_eval<<<grid_for(estB, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, ctx->eatom.as<double4>(), cB,
                                                            ctx->blist.as<int>(), ctx->ddesc.as<int4>(),
                                                            ctx->ddesc2.as<int4>(), ctx->evpos.as<double4>());
-    k_frozen_cms<<<grid_for(nB, 256, 4096), 256, 0, st>>>(nB, ctx->bgof.as<int>(), ctx->bgrp.as<int>(),
+  k_frozen_cms<<<grid_for(estB, 256, 4096), 256, 0, st>>>(cB, ctx->bgof.as<int>(), ctx->bgrp.as<int>(),
                                                            ctx->bidx.as<int>(), ctx->evpos.as<double4>(),
                                                            ctx->s.weight, ctx->wv.as<double>());
-  }
   // per-group sums in buried-dot order: deterministic, so finite differences
   // see no summation-order noise
   k_group_np_sum<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->bgrp.as<int>(), ctx->wv.as<double>(), 1, totals);
@@ -1175,7 +1539,7 @@ int cms_check_dots(cms_ctx* ctx, int64_t* mismatches, double* maxdev) {
   CK(ctx->chk.ensure(16));
   fill_u64(ctx->chk.as<unsigned long long>(), 2, 0ull, st);
   if (ctx->nD > 0)
-    k_check_dots<<<grid_for(ctx->nD, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, ctx->nD, ctx->dpos.as<double4>(),
+    k_check_dots<<<grid_for(ctx->nD, 64, 16384), 64, 0, st>>>(view(ctx), ctx->s, host_count(ctx->nD), ctx->dpos.as<double4>(),
                                                                ctx->ddesc.as<int4>(), ctx->ddesc2.as<int4>(),
                                                                ctx->chk.as<unsigned long long>(),
                                                                reinterpret_cast<double*>(ctx->chk.as<unsigned long long>() + 1));

@@ -21,6 +21,24 @@
 
 namespace cms {
 
+// Item count of a launch: a host value, or a device word (times mul) that
+// an earlier kernel of the same stream wrote.  Kernels size their loops with
+// get(); the host sizes the grid from a capacity that bounds the count.
+struct Count {
+  int n;
+  const int* p;
+  int mul;
+  __host__ __device__ __forceinline__ int get() const { return p ? *p * mul : n; }
+};
+inline Count host_count(int n) { return Count{n, nullptr, 1}; }
+inline Count dev_count(const int* p, int mul = 1) { return Count{0, p, mul}; }
+
+// Device words of the per-run count block ("meta"): clamped stage counts
+// that kernels loop over, the raw totals, and the overflow bits (one per
+// stage whose total exceeded the buffer capacity it was enqueued with).
+constexpr int M_R = 0, M_P = 1, M_Q = 2, M_D = 3, M_B = 4, M_A = 5, M_T = 6;
+constexpr int M_RAW = 8, M_OVF = 15, M_WORDS = 16;
+
 struct Settings {
   double rp, band, sep, weight, binwidth_dist, binwidth_norm;
 };
@@ -88,6 +106,10 @@ struct Work {
   int* datom;
   int4* ddesc;            // (kind | mm << 4, atom a, atom b, atom c)
   int4* ddesc2;           // (latitude or torus point, point, count 1, count 2)
+  // buffer capacities: writes at or beyond them are dropped (the stage
+  // total then exceeds the capacity and the run is repeated, larger)
+  int64_t pair_cap;
+  int64_t probe_cap;
   int64_t dot_cap;
 };
 

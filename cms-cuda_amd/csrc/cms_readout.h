@@ -6,28 +6,28 @@
 
 namespace cms {
 
-__global__ void k_dot_flag(int n, const int8_t* bur, int want, int* flag);
-__global__ void k_dot_compact(int n, const int* flag, const int* scan, const double4* dpos, int* list,
+__global__ void k_dot_flag(Count n, const int8_t* bur, int want, int* flag);
+__global__ void k_dot_compact(Count n, const int* flag, const int* scan, const double4* dpos, int* list,
                               double4* xyz);
 __global__ void k_tile_counts(int G, const int* qgrp, int tq, int* cnt);
 template <bool IDX>
 __global__ void k_nearest64(const double4* Q, const int* qgrp, const double4* R, const int* rgrp,
                             int xmask, const int* toff, int G, int nsplit, int64_t pstride,
                             double* out_d2, int* out_idx);
-__global__ void k_nearest_combine(int nq, int nsplit, int64_t pstride, const double* pd2, const int* pidx,
+__global__ void k_nearest_combine(Count nq, int nsplit, int64_t pstride, const double* pd2, const int* pidx,
                                   double* out_d2, int* out_idx);
-__global__ void k_cms_weights(int n, const double4* bxyz, const double* d2, double weight, double* w);
-__global__ void k_owner_count(int n, const int* list, const int* datom, int* cnt);
-__global__ void k_owner_scatter(int n, const int* list, const int* datom, const int* start, int* cur,
+__global__ void k_cms_weights(Count n, const double4* bxyz, const double* d2, double weight, double* w);
+__global__ void k_owner_count(Count n, const int* list, const int* datom, int* cnt);
+__global__ void k_owner_scatter(Count n, const int* list, const int* datom, const int* start, int* cur,
                                 int* perm);
 __global__ void k_owner_sums(int natoms, const int* start, int* perm, const double* val,
                              int64_t vstride, double* out);
 __global__ void k_group_np_sum(int G, const int* start, const double* vals, int64_t stride, double* out);
-__global__ void k_trim_flag(int n, const double* d2acc, double band2, int* flag);
-__global__ void k_trim_compact(int n, const int* flag, const int* scan, const int* blist,
+__global__ void k_trim_flag(Count n, const double* d2acc, double band2, int* flag);
+__global__ void k_trim_compact(Count n, const int* flag, const int* scan, const int* blist,
                                const double4* dpos, const double4* dnml, int* tlist, double4* txyz,
                                double4* tnml);
-__global__ void k_sc_terms(int n, const int* tgrp_of, const int* tgrp, const double* d2, const int* idx,
+__global__ void k_sc_terms(Count n, const int* tgrp_of, const int* tgrp, const double* d2, const int* idx,
                            const double4* tnml, double weight, double* dist, double* rr);
 __global__ void k_group_of_items(int G, const int* start, int* gof);
 __global__ void k_sc_sides(int G, const int* tgrp, const double4* txyz, const double* dist,
@@ -40,7 +40,7 @@ __global__ void k_nearest32(const float4* Q32, const double4* Q64, const int* qg
                             int64_t pstride, const double4* frame, double* out_d2, int* out_idx,
                             unsigned long long* nrescored);
 __global__ void k_complex_frame(int B, const int* aoff, const double4* atom, double rp, double4* frame);
-__global__ void k_to_float(int n, const double4* in, const int* grp, int G, int percomplex,
+__global__ void k_to_float(Count n, const double4* in, const int* grp, int G, int percomplex,
                            const double4* frame, float4* out);
 
 }  // namespace cms

@@ -8,15 +8,17 @@
 namespace cms {
 
 // ------------------------------------------------------------ lists
-__global__ void k_dot_flag(int n, const int8_t* __restrict__ bur, int want, int* __restrict__ flag) {
+__global__ void k_dot_flag(Count cnt, const int8_t* __restrict__ bur, int want, int* __restrict__ flag) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     flag[i] = (bur[i] != 0) == (want != 0) ? 1 : 0;
 }
 
 // list[scan[i]] = i for flagged i; also gathers the dot position record
-__global__ void k_dot_compact(int n, const int* __restrict__ flag, const int* __restrict__ scan,
+__global__ void k_dot_compact(Count cnt, const int* __restrict__ flag, const int* __restrict__ scan,
                               const double4* __restrict__ dpos, int* __restrict__ list,
                               double4* __restrict__ xyz) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (flag[i]) {
       const int o = scan[i];
@@ -94,9 +96,10 @@ __global__ __launch_bounds__(NT) void k_nearest64(const double4* __restrict__ Q,
   }
 }
 
-__global__ void k_nearest_combine(int nq, int nsplit, int64_t pstride, const double* __restrict__ pd2,
+__global__ void k_nearest_combine(Count cnt, int nsplit, int64_t pstride, const double* __restrict__ pd2,
                                   const int* __restrict__ pidx, double* __restrict__ out_d2,
                                   int* __restrict__ out_idx) {
+  const int nq = cnt.get();
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
     double best = INFINITY;
     int bi = -1;
@@ -114,8 +117,9 @@ __global__ void k_nearest_combine(int nq, int nsplit, int64_t pstride, const dou
 
 // ------------------------------------------------------------ CMS weights
 // per buried dot: area * exp(-d^2 * weight)   (_core.py:995)
-__global__ void k_cms_weights(int n, const double4* __restrict__ bxyz, const double* __restrict__ d2,
+__global__ void k_cms_weights(Count cnt, const double4* __restrict__ bxyz, const double* __restrict__ d2,
                               double weight, double* __restrict__ w) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     w[i] = bxyz[i].w * exp(-d2[i] * weight);
 }
@@ -125,14 +129,16 @@ __global__ void k_cms_weights(int n, const double4* __restrict__ bxyz, const dou
 // order starting from 0.0.  Counting sort by owner (integer atomics), then
 // each atom sorts its own (short) segment by item index and sums in order,
 // so the result is deterministic and bit-identical to bincount.
-__global__ void k_owner_count(int n, const int* __restrict__ list, const int* __restrict__ datom,
+__global__ void k_owner_count(Count items, const int* __restrict__ list, const int* __restrict__ datom,
                               int* __restrict__ cnt) {
+  const int n = items.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     atomicAdd(&cnt[datom[list ? list[i] : i]], 1);
 }
-__global__ void k_owner_scatter(int n, const int* __restrict__ list, const int* __restrict__ datom,
+__global__ void k_owner_scatter(Count cnt, const int* __restrict__ list, const int* __restrict__ datom,
                                 const int* __restrict__ start, int* __restrict__ cur,
                                 int* __restrict__ perm) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int a = datom[list ? list[i] : i];
     perm[start[a] + atomicAdd(&cur[a], 1)] = i;
@@ -315,14 +321,16 @@ __global__ __launch_bounds__(256) void k_group_np_sum(int G, const int* __restri
 }
 
 // ------------------------------------------------------------ trim
-__global__ void k_trim_flag(int n, const double* __restrict__ d2acc, double band2, int* __restrict__ flag) {
+__global__ void k_trim_flag(Count cnt, const double* __restrict__ d2acc, double band2, int* __restrict__ flag) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     flag[i] = d2acc[i] > band2 ? 1 : 0;
 }
-__global__ void k_trim_compact(int n, const int* __restrict__ flag, const int* __restrict__ scan,
+__global__ void k_trim_compact(Count cnt, const int* __restrict__ flag, const int* __restrict__ scan,
                                const int* __restrict__ blist, const double4* __restrict__ dpos,
                                const double4* __restrict__ dnml, int* __restrict__ tlist,
                                double4* __restrict__ txyz, double4* __restrict__ tnml) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     if (flag[i]) {
       const int o = scan[i];
@@ -335,10 +343,11 @@ __global__ void k_trim_compact(int n, const int* __restrict__ flag, const int* _
 
 // Per trimmed dot: distance to the nearest trimmed dot of the other side and
 // r = clip((n . n_other) * exp(-dist^2 * weight), +-0.999)  (_core.py:1134-1141)
-__global__ void k_sc_terms(int n, const int* __restrict__ tgrp_of, const int* __restrict__ tgrp,
+__global__ void k_sc_terms(Count cnt, const int* __restrict__ tgrp_of, const int* __restrict__ tgrp,
                            const double* __restrict__ d2, const int* __restrict__ idx,
                            const double4* __restrict__ tnml, double weight,
                            double* __restrict__ dist, double* __restrict__ rr) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (idx[i] < 0) {  // the other side has no trimmed dots: statistics are skipped
       dist[i] = 0.0;

@@ -140,8 +140,9 @@ __global__ __launch_bounds__(256) void k_complex_frame(int B, const int* __restr
 // float copy of a double4 point list, recentred on its complex's frame;
 // group of item i found from grp (G+1 starts), complex = group / 2 (or 0
 // when percomplex == 0).
-__global__ void k_to_float(int n, const double4* __restrict__ in, const int* __restrict__ grp, int G,
+__global__ void k_to_float(Count cnt, const double4* __restrict__ in, const int* __restrict__ grp, int G,
                            int percomplex, const double4* __restrict__ frame, float4* __restrict__ out) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int c = 0;
     if (percomplex) {

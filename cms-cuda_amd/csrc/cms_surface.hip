@@ -81,7 +81,7 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
         }
       }
       const uint64_t mk = ballot(relevant);
-      if (EMIT && relevant) {
+      if (EMIT && relevant && base + total + lanes_below(mk) < w.pair_cap) {
         const int o = base + total + lanes_below(mk);
         w.pi[o] = g;
         w.pj[o] = j;
@@ -224,13 +224,13 @@ __global__ __launch_bounds__(64) void k_probes(BatchView b, Settings s, Work w) 
           if (okp) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
           if (okm) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
         } else {
-          if (okp) {
+          if (okp && ob_plus + np_plus + lanes_below(mp) < w.probe_cap) {
             const int o = ob_plus + np_plus + lanes_below(mp);
             w.patom[o] = make_int4(i, j, k, 0);
             w.ppos[o] = make_double4(pp.x, pp.y, pp.z, h);
             w.palt[o] = make_double4(uijk.x, uijk.y, uijk.z, 0.0);
           }
-          if (okm) {
+          if (okm && ob_minus + np_minus + lanes_below(mm) < w.probe_cap) {
             const int o = ob_minus + np_minus + lanes_below(mm);
             w.patom[o] = make_int4(j, i, k, 0);
             w.ppos[o] = make_double4(pm.x, pm.y, pm.z, h);

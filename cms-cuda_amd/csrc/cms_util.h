@@ -9,6 +9,11 @@ namespace cms {
 // Exclusive scan of in[0..n) into out[0..n], out[n] = total.  part needs
 // 1025 ints of scratch.
 void scan_excl(const int* in, int* out, int64_t n, int* part, hipStream_t st);
+// Same with the count read on the device; cap bounds it and picks the grid.
+void scan_excl(const int* in, int* out, Count n, int64_t cap, int* part, hipStream_t st);
+__global__ void k_stage_total(const int* scan, Count at, int64_t cap, int* meta, int slot, int* grp, int G);
+__global__ void k_list_totals(const int* bscan, int* meta);
+__global__ void k_copy_word(const int* src, Count at, int* dst);
 void fill_int(int* p, int64_t n, int v, hipStream_t st);
 void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
 void fill_double(double* p, int64_t n, double v, hipStream_t st);
@@ -16,7 +21,7 @@ void fill_double(double* p, int64_t n, double v, hipStream_t st);
 __global__ void k_pack_atoms(int ntot, const double* xyz, const double* rad, const double* dens_in,
                              double dens_default, const int* aoff, const int* nb0, int B,
                              double4* atom, double* dens, int* acplx, int8_t* amol);
-__global__ void k_group_starts_from_atoms(int n, const int* items, const int* acplx,
+__global__ void k_group_starts_from_atoms(Count n, const int* items, const int* acplx,
                                           const int8_t* amol, int G, int* start);
 __global__ void k_gather_scan(int G, const int* idx, int mul, const int* scan, int* out);
 __global__ void k_dot_bases(int G, const int* rgrp, const int* pgrp, const int* prgrp,

@@ -24,8 +24,14 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total) {
   return off + inc - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partial(const int* __restrict__ in, int64_t n,
-                                                              int64_t chunk, int* __restrict__ part) {
+// Multi-block scan: nb blocks, each owns a contiguous chunk of
+// ceil(n / nb) items, n read on the device.
+__device__ __forceinline__ int64_t scan_chunk(int64_t n, int nb) { return (n + nb - 1) / nb; }
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partial(const int* __restrict__ in, Count cnt, int nb,
+                                                              int* __restrict__ part) {
+  const int64_t n = cnt.get();
+  const int64_t chunk = scan_chunk(n, nb);
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   int acc = 0;
@@ -50,8 +56,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int* __restrict__ par
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restrict__ in, int* __restrict__ out,
-                                                            int64_t n, int64_t chunk,
-                                                            const int* __restrict__ part, int nb) {
+                                                            Count cnt, int nb, const int* __restrict__ part) {
+  const int64_t n = cnt.get();
+  const int64_t chunk = scan_chunk(n, nb);
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   int carry = part[blockIdx.x];
@@ -71,8 +78,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restri
 constexpr int SS_THREADS = 1024;
 constexpr int64_t SS_MAX = 64 * SS_THREADS;
 __global__ __launch_bounds__(SS_THREADS) void k_scan_single(const int* __restrict__ in, int* __restrict__ out,
-                                                            int64_t n) {
+                                                            Count cnt) {
   __shared__ int wsum[SS_THREADS / WAVE];
+  const int64_t n = cnt.get();
   const int64_t per = (n + SS_THREADS - 1) / SS_THREADS;
   const int64_t lo = threadIdx.x * per;
   const int64_t hi = lo + per < n ? lo + per : n;
@@ -96,19 +104,55 @@ __global__ __launch_bounds__(SS_THREADS) void k_scan_single(const int* __restric
   if (threadIdx.x == 0) out[n] = tot;
 }
 
-void scan_excl(const int* in, int* out, int64_t n, int* part, hipStream_t st) {
-  // part must hold at least 1025 ints
-  if (n <= SS_MAX) {
+void scan_excl(const int* in, int* out, Count n, int64_t cap, int* part, hipStream_t st) {
+  // part must hold at least 1025 ints; cap bounds the count
+  if (cap <= SS_MAX) {
     k_scan_single<<<1, SS_THREADS, 0, st>>>(in, out, n);
     return;
   }
-  int nb = (int)((n + 4095) / 4096);
+  int nb = (int)((cap + 4095) / 4096);
   if (nb < 1) nb = 1;
   if (nb > 1024) nb = 1024;
-  const int64_t chunk = (n + nb - 1) / nb > 0 ? (n + nb - 1) / nb : 1;
-  k_scan_partial<<<nb, SCAN_THREADS, 0, st>>>(in, n, chunk, part);
+  k_scan_partial<<<nb, SCAN_THREADS, 0, st>>>(in, n, nb, part);
   k_scan_top<<<1, SCAN_THREADS, 0, st>>>(part, nb);
-  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(in, out, n, chunk, part, nb);
+  k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(in, out, n, nb, part);
+}
+
+void scan_excl(const int* in, int* out, int64_t n, int* part, hipStream_t st) {
+  scan_excl(in, out, host_count((int)n), n, part, st);
+}
+
+// Stage total: meta[slot] = total if it fits the capacity, else 0 with the
+// overflow bit set and the stage's group offsets zeroed, so that every
+// later kernel of the run sees an empty stage and stays inside its buffers.
+__global__ __launch_bounds__(256) void k_stage_total(const int* __restrict__ scan, Count at, int64_t cap,
+                                                     int* __restrict__ meta, int slot, int* __restrict__ grp,
+                                                     int G) {
+  __shared__ int ovf;
+  if (threadIdx.x == 0) {
+    const int v = scan[at.get()];
+    ovf = (int64_t)v > cap ? 1 : 0;
+    meta[M_RAW + slot] = v;
+    meta[slot] = ovf ? 0 : v;
+    if (ovf) meta[M_OVF] |= 1 << slot;
+  }
+  __syncthreads();
+  if (ovf && grp)
+    for (int g = threadIdx.x; g <= G; g += blockDim.x) grp[g] = 0;
+}
+
+// Buried / accessible dot totals of the run: nB = bscan[nD], nA = nD - nB.
+__global__ void k_list_totals(const int* __restrict__ bscan, int* __restrict__ meta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int nD = meta[M_D];
+    const int nB = bscan[nD];
+    meta[M_B] = nB;
+    meta[M_A] = nD - nB;
+  }
+}
+
+__global__ void k_copy_word(const int* __restrict__ src, Count at, int* __restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src[at.get()];
 }
 
 __global__ void k_fill_int(int* p, int64_t n, int v) {
@@ -165,9 +209,10 @@ __global__ void k_pack_atoms(int ntot, const double* __restrict__ xyz, const dou
 
 // First index of each group in a list sorted by group (group of item i is
 // key(i)); writes start[0..G], start[G] = n.
-__global__ void k_group_starts_from_atoms(int n, const int* __restrict__ items,
+__global__ void k_group_starts_from_atoms(Count cnt, const int* __restrict__ items,
                                           const int* __restrict__ acplx, const int8_t* __restrict__ amol,
                                           int G, int* __restrict__ start) {
+  const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
     const int k = i < n ? 2 * acplx[items[i]] + amol[items[i]] : G;
     const int kp = i > 0 ? 2 * acplx[items[i - 1]] + amol[items[i - 1]] : -1;

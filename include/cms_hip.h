@@ -137,6 +137,27 @@ int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals);
    (cms-cuda/_core.py:807-863, 1037-1147; _gpu.py:350-419, 604-686). */
 int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results);
 
+/* One step of the hot path over the loaded batch: surface (CMS mode), CMS
+   of both sides into per_atom / totals (as cms_contact_ms) and, when
+   sc_results is not NULL, SC (as cms_shape_complementarity).  The same
+   results as cms_generate(ctx, 0, 0) + cms_contact_ms +
+   cms_shape_complementarity, issued without host waits once the context
+   has seen a batch of this kind (buffer sizes learned per atom; a batch
+   that outgrows them is repeated exactly), and replayed from a captured
+   HIP graph when cms_set_graph(ctx, 1) is on and the batch layout, output
+   pointers and sizes are those of the previous step.  Returns after one
+   synchronisation, with the same error codes as cms_generate.  The
+   batched, graph-captured entry point the reference's per-complex
+   _ensure_surfaces_generated + calc_contact_molecular_surface +
+   CalcLoadedSC sequence (cms-cuda/_gpu.py:421-461, 515-571, 350-419)
+   becomes. */
+int cms_run(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc_results);
+/* Enable (1) or disable (0) graph capture / replay in cms_run. */
+int cms_set_graph(cms_ctx* ctx, int enable);
+/* out[4]: graphs captured, graph replays, exact (host-sized) surface runs,
+   learned runs repeated after a buffer overflow. */
+int cms_run_stats(cms_ctx* ctx, int64_t* out);
+
 /* Number of dots of (complex, molecule) and a copy of them into device
    buffers (coor, nml: 3n float64; area: n; buried, type: n int8; atom: n
    int32 local atom index).  Any pointer may be NULL.  Counterpart of

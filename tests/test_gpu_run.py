@@ -1,0 +1,157 @@
+"""GPU: the one-step entry point cms_run (surface + CMS both sides + SC with
+one host wait), its learned buffer sizes, the exact rerun after an
+overflow, and HIP-graph capture / replay.
+
+Bar: every result bit-identical to the separate entry points
+(cms_generate + cms_contact_ms + cms_shape_complementarity) on a fresh
+context, for every step and every replay.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _batch(golden, sizes, seed0):
+    from cms_cuda import synth
+    g = golden
+    cs = [(g["binder_xyz"], g["binder_radii"], g["target_xyz"], g["target_radii"])]
+    for k, (nt, nb) in enumerate(sizes):
+        c = synth.make_complex(nt, nb, seed0 + k, gap=3.5)
+        cs.append((c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"]))
+    return cs
+
+
+def _reference(cs, precision=0):
+    """Separate entry points on a fresh context (exact sizing)."""
+    from cms_cuda.batch import BatchCalculator
+    bc = BatchCalculator()
+    bc.load(cs)
+    out = bc.contact_ms(precision)
+    out["sc"] = bc.shape_complementarity(precision)
+    bc.close()
+    return out
+
+
+def _same(a, b):
+    return (np.array_equal(a["target"], b["target"]) and np.array_equal(a["binder"], b["binder"])
+            and np.array_equal(a["per_atom"], b["per_atom"]) and np.array_equal(a["sc"], b["sc"]))
+
+
+def _flat(cs):
+    nb = np.array([len(c[1]) for c in cs], dtype=np.int32)
+    nt = np.array([len(c[3]) for c in cs], dtype=np.int32)
+    xyz = np.concatenate([np.concatenate([c[0], c[2]]) for c in cs])
+    rad = np.concatenate([np.concatenate([c[1], c[3]]) for c in cs])
+    return nb, nt, xyz, rad
+
+
+def test_run_matches_separate_calls(golden_complex):
+    from cms_cuda.batch import BatchCalculator
+    cs = _batch(golden_complex, [(400, 40), (700, 200), (300, 300)], 300)
+    say("cms_run vs separate entry points, %d complexes" % len(cs))
+    ref = _reference(cs)
+    bc = BatchCalculator()
+    bc.load(cs)
+    for step in range(3):
+        say("  step %d" % step)
+        got = bc.run(graph=False)
+        assert _same(got, ref), step
+    cap, rep, exact, ovf = bc.run_stats()
+    assert exact == 1 and ovf == 0 and cap == 0 and rep == 0
+    bc.close()
+
+
+def test_run_fp32_search_identical(golden_complex):
+    from cms_cuda import _lib
+    from cms_cuda.batch import BatchCalculator
+    cs = _batch(golden_complex, [(900, 60), (1100, 500)], 320)
+    say("cms_run fp32 search vs fp64")
+    bc = BatchCalculator()
+    bc.load(cs)
+    a = bc.run(_lib.PREC_FP64)
+    b = bc.run(_lib.PREC_FP32_RESCORE)
+    c = bc.run(_lib.PREC_FP32_RESCORE, graph=True)
+    d = bc.run(_lib.PREC_FP32_RESCORE, graph=True)
+    assert _same(a, b) and _same(a, c) and _same(a, d)
+    bc.close()
+
+
+def test_overflow_repeats_exactly(golden_complex):
+    """A context that learned from a batch without any surface (molecules
+    40 A apart) under-sizes every buffer for the next batch: the learned
+    run overflows, is detected on the device and repeated exactly."""
+    from cms_cuda.batch import BatchCalculator
+    g = golden_complex
+    far = [(g["binder_xyz"], g["binder_radii"], g["target_xyz"] + 40.0, g["target_radii"])]
+    cs = _batch(golden_complex, [(500, 100)], 340)
+    say("overflow rerun")
+    ref = _reference(cs)
+    bc = BatchCalculator()
+    bc.load(far)
+    z = bc.run()
+    assert not z["target"].any() and not z["binder"].any()
+    bc.load(cs)
+    got = bc.run()
+    assert _same(got, ref)
+    cap, rep, exact, ovf = bc.run_stats()
+    assert ovf >= 1 and exact == 2
+    # the learned sizes now fit: no further overflow
+    got = bc.run()
+    assert _same(got, ref) and bc.run_stats()[3] == ovf
+    bc.close()
+
+
+def test_graph_replays_match_eager(golden_complex):
+    """Graph capture / replay on one layout while the coordinates change in
+    place between steps; eager torch work and synchronisations between the
+    replays; every replay checked against a fresh eager run on the same
+    coordinates."""
+    import torch
+    from cms_cuda.batch import BatchCalculator
+    cs = _batch(golden_complex, [(600, 80), (800, 300), (350, 350)], 360)
+    nb, nt, xyz0, rad = _flat(cs)
+    rng = np.random.default_rng(5)
+    xyz1 = xyz0 + rng.normal(scale=0.02, size=xyz0.shape)
+    dev = torch.device("cuda", 0)
+    d_xyz = torch.from_numpy(xyz0.copy()).to(dev)
+    d_rad = torch.from_numpy(rad.copy()).to(dev)
+    refs = {}
+    for key, x in (("a", xyz0), ("b", xyz1)):
+        off = np.concatenate([[0], np.cumsum(nb + nt)])
+        cs_k = []
+        for c in range(len(cs)):
+            xc = x[off[c]:off[c + 1]]
+            rc = rad[off[c]:off[c + 1]]
+            cs_k.append((xc[:nb[c]], rc[:nb[c]], xc[nb[c]:], rc[nb[c]:]))
+        refs[key] = _reference(cs_k)
+    bc = BatchCalculator()
+    per_atom = torch.empty(len(rad), dtype=torch.float64, device=dev)
+    totals = torch.empty(2 * len(cs), dtype=torch.float64, device=dev)
+    scr = torch.empty(len(cs) * 24, dtype=torch.float64, device=dev)
+    seq = ["a", "a", "a", "b", "a", "b", "b", "a"]
+    for step, key in enumerate(seq):
+        say("  graph step %d coords %s" % (step, key))
+        d_xyz.copy_(torch.from_numpy(xyz0 if key == "a" else xyz1))
+        bc.load_device(nb, nt, d_xyz, d_rad)
+        bc.run(graph=True, out=(per_atom, totals, scr))
+        # eager work of the program between replays
+        junk = torch.randn(4096, device=dev).sum()
+        torch.cuda.synchronize()
+        t = totals.cpu().numpy().reshape(len(cs), 2)
+        got = dict(target=t[:, 1], binder=t[:, 0], per_atom=per_atom.cpu().numpy(),
+                   sc=scr.cpu().numpy().reshape(len(cs), 24))
+        assert _same(got, refs[key]), (step, key)
+        del junk
+    cap, rep, exact, ovf = bc.run_stats()
+    say("  captures %d replays %d exact %d overflow %d" % (cap, rep, exact, ovf))
+    assert cap >= 1 and rep >= 3
+    bc.close()
