@@ -84,7 +84,7 @@ def poisson_disk(n, axes, rng, spacing=MIN_SPACING, max_rounds=5000):
         need = n - len(pts)
         if need <= 0:
             break
-        m = max(64, 2 * need)
+        m = max(2048, 2 * need)
         cand = rng.uniform(-1.0, 1.0, size=(m, 3))
         cand = cand[np.sum(cand * cand, axis=1) <= 1.0] * axes
         if tree is not None and len(cand):
@@ -128,7 +128,10 @@ def point_ellipsoid_distance(p, axes, iters=40):
         qq = q * q
         f = qq.sum(axis=1) - 1.0
         df = -2.0 * (qq / (t[:, None] + a2)).sum(axis=1)
-        t = t - f / df
+        step = f / df
+        t = t - step
+        if np.all(np.abs(step) <= 1e-13 * (1.0 + np.abs(t))):
+            break
     close = a2 * p / (t[:, None] + a2)
     return np.linalg.norm(p - close, axis=1)
 
@@ -143,7 +146,7 @@ def _surface_points(axes, n=3000):
 
 def place_binder(t_axes, b_axes, rot, direction, gap):
     """Offset along direction at which the binder ellipsoid (rotated by rot)
-    is gap away from the target ellipsoid: bisection (to 1e-9 A) on the
+    is gap away from the target ellipsoid: Brent's method (to 1e-9 A) on the
     minimum over the binder-surface sample facing the target of the exact
     point-to-ellipsoid distance."""
     surf = _surface_points(b_axes) @ rot.T
@@ -156,16 +159,11 @@ def place_binder(t_axes, b_axes, rot, direction, gap):
             return -1.0
         return float(point_ellipsoid_distance(p, t_axes).min())
 
+    from scipy.optimize import brentq
     lo, hi = 0.0, float(np.max(t_axes) + np.max(b_axes) + gap + 1.0)
     while surface_gap(hi) < gap:
         hi *= 2.0
-    while hi - lo > 1e-9:
-        mid = 0.5 * (lo + hi)
-        if surface_gap(mid) < gap:
-            lo = mid
-        else:
-            hi = mid
-    return hi
+    return brentq(lambda off: surface_gap(off) - gap, lo, hi, xtol=1e-9, rtol=1e-15, maxiter=200)
 
 
 def make_molecule(n, rng, density=0.05, ratio=1.3, ligand=False):

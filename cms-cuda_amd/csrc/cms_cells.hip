@@ -6,8 +6,11 @@
 // non-adjacent cells), so every partner an atom can have -- other-molecule
 // atoms closer than sep, same-molecule atoms inside the bridge distance
 // r_i + r_j + 2rp, coincident atoms -- lies in the 27 cells around it.
-// Results do not depend on the order atoms are visited in: minima, "any"
-// tests, atomic minima of (i, j) keys, and rows sorted afterwards.
+// Each cell holds two atom ranges, molecule 0 then molecule 1 (slot
+// 2 * cell + molecule of the count / start arrays), so a sweep reads only
+// the molecule it needs.  Results do not depend on the order atoms are
+// visited in: minima, "any" tests, atomic minima of (i, j) keys, idempotent
+// flags, and rows sorted afterwards.
 #include "cms_cells.h"
 #include "cms_device.h"
 #include "cms_internal.h"
@@ -97,7 +100,7 @@ __global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int
     const double4 a = b.atom[g];
     int ix, iy, iz;
     cell_coords(gi, a.x, a.y, a.z, &ix, &iy, &iz);
-    const int cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+    const int cell = 2 * (gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz) + b.amol[g];
     acell[g] = cell;
     atomicAdd(&ccount[cell], 1);
   }
@@ -111,29 +114,60 @@ __global__ void k_cell_scatter(int n, const int* __restrict__ acell, const int* 
   }
 }
 
-// Visits the (up to) 27 cells around cell (ix, iy, iz) of grid gi.
-#define FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cellvar)                                   \
-  for (int dx_ = -1; dx_ <= 1; ++dx_)                                                \
-    for (int dy_ = -1; dy_ <= 1; ++dy_)                                              \
-      for (int dz_ = -1; dz_ <= 1; ++dz_)                                            \
-        if ((ix) + dx_ >= 0 && (ix) + dx_ < (gi).nx && (iy) + dy_ >= 0 &&            \
-            (iy) + dy_ < (gi).ny && (iz) + dz_ >= 0 && (iz) + dz_ < (gi).nz)         \
-          for (int cellvar = (gi).cell0 + (((ix) + dx_) * (gi).ny + (iy) + dy_) * (gi).nz + (iz) + dz_, \
-                   once_ = 1; once_; once_ = 0)
+// Molecule selector of a sweep: 0 / 1 one molecule's range, 2 both.
+__device__ __forceinline__ int sel_lo(int sel) { return sel == 2 ? 0 : sel; }
+__device__ __forceinline__ int sel_hi(int sel) { return sel == 2 ? 2 : sel + 1; }
 
-// Flattened candidate list of the 27 cells around (ix, iy, iz): cbeg[k] /
-// cpre[k] are the start in catom and the prefix count of cell k; cpre[27]
-// is the total.  Wave-cooperative (lanes 0..26 each own a cell).
+// Cell k of the 3x3x3 block around (ix, iy, iz) (k = 9 dx + 3 dy + dz with
+// offsets -1..1), or -1 outside the grid.
+__device__ __forceinline__ int cell_of(const GridInfo& gi, int ix, int iy, int iz, int k) {
+  const int cx = ix + k / 9 - 1, cy = iy + (k / 3) % 3 - 1, cz = iz + k % 3 - 1;
+  if (cx < 0 || cx >= gi.nx || cy < 0 || cy >= gi.ny || cz < 0 || cz >= gi.nz) return -1;
+  return gi.cell0 + (cx * gi.ny + cy) * gi.nz + cz;
+}
+
+// Cells that can hold an atom within COINC_R of x: the atom's own cell,
+// and a neighbour only across a face, edge or corner closer than COINC_R
+// (bit k of the mask = cell k of the 3x3x3 block).
+constexpr double COINC_R = 0.0101;  // > 0.01, the coincident-atom distance
+__device__ __forceinline__ uint32_t coincident_cells(const GridInfo& gi, double x, double y, double z, int ix,
+                                                     int iy, int iz) {
+  const double fx = x - (gi.x0 + ix * gi.h), fy = y - (gi.y0 + iy * gi.h), fz = z - (gi.z0 + iz * gi.h);
+  const bool lx = fx <= COINC_R, hx = gi.h - fx <= COINC_R;
+  const bool ly = fy <= COINC_R, hy = gi.h - fy <= COINC_R;
+  const bool lz = fz <= COINC_R, hz = gi.h - fz <= COINC_R;
+  uint32_t mask = 0;
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+    const bool okx = dx == 0 || (dx < 0 ? lx : hx);
+    const bool oky = dy == 0 || (dy < 0 ? ly : hy);
+    const bool okz = dz == 0 || (dz < 0 ? lz : hz);
+    if (okx && oky && okz) mask |= 1u << k;
+  }
+  return mask;
+}
+
+// Visits the cells of the 3x3x3 block around (ix, iy, iz) whose bit is set
+// in mask.
+#define FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, mask, cellvar)                             \
+  for (int k_ = 0; k_ < 27; ++k_)                                                    \
+    if (((mask) >> k_) & 1u)                                                         \
+      for (int cellvar = cell_of((gi), (ix), (iy), (iz), k_), once_ = 1; once_ && cellvar >= 0; once_ = 0)
+
+// Flattened candidate list of the cells around (ix, iy, iz) selected by
+// mask, molecule range sel: cbeg[k] / cpre[k] are the start in catom and the
+// prefix count of cell k; cpre[27] is the total.  Wave-cooperative (lanes
+// 0..26 each own a cell).
 __device__ __forceinline__ void cells_around(const GridInfo& gi, int ix, int iy, int iz,
-                                             const int* __restrict__ cstart, int* cbeg, int* cpre) {
+                                             const int* __restrict__ cstart, int sel, uint32_t mask, int* cbeg,
+                                             int* cpre) {
   const int k = lane();
   int cnt = 0, beg = 0;
-  if (k < 27) {
-    const int cx = ix + k / 9 - 1, cy = iy + (k / 3) % 3 - 1, cz = iz + k % 3 - 1;
-    if (cx >= 0 && cx < gi.nx && cy >= 0 && cy < gi.ny && cz >= 0 && cz < gi.nz) {
-      const int cell = gi.cell0 + (cx * gi.ny + cy) * gi.nz + cz;
-      beg = cstart[cell];
-      cnt = cstart[cell + 1] - beg;
+  if (k < 27 && ((mask >> k) & 1u)) {
+    const int cell = cell_of(gi, ix, iy, iz, k);
+    if (cell >= 0) {
+      beg = cstart[2 * cell + sel_lo(sel)];
+      cnt = cstart[2 * cell + sel_hi(sel)] - beg;
     }
   }
   const int inc = wave_incl_scan(cnt);
@@ -153,13 +187,32 @@ __device__ __forceinline__ int cell_candidate(const int* cbeg, const int* cpre, 
 }
 
 // Attention (_core.py:1150-1195) and coincident atoms (_core.py:1285-1293):
-// one wavefront per atom, lanes over the atom's 27-cell candidates; atoms
-// visited in cell order so that consecutive workgroups read the same cells.
-__global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s, int all_atoms,
+// one wavefront per atom.  The minimum distance to the other molecule reads
+// only that molecule's atoms in the 27 cells; coincident partners (same
+// molecule, within 0.01 A) are looked for in the atom's own cell and in the
+// neighbour cells within 0.01 A of it.  Also seeds the interface-region
+// flag (non-blockers; every atom when full != 0).  Atoms are visited in
+// cell order so that consecutive workgroups read the same cells.
+__device__ __forceinline__ void attention_store(int g, int c, double mn, unsigned long long ckey, int all_atoms,
+                                                int full, const Settings& s, int8_t* atten, int8_t* access,
+                                                int* need, unsigned long long* coinc, int* status) {
+  const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
+  const int8_t at = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
+  atten[g] = at;
+  access[g] = 0;
+  need[g] = (full || at > ATTEN_BLOCKER) ? 1 : 0;
+  if (ckey != ~0ull) {
+    atomicMin(&coinc[c], ckey);
+    atomicOr(&status[c], ST_COINCIDENT);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s, int all_atoms, int full,
                                                         const GridInfo* __restrict__ grid,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
                                                         int8_t* __restrict__ atten, int8_t* __restrict__ access,
+                                                        int* __restrict__ need,
                                                         unsigned long long* __restrict__ coinc,
                                                         int* __restrict__ status) {
   __shared__ int cbeg[27], cpre[28];
@@ -173,53 +226,49 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    cells_around(gi, ix, iy, iz, cstart, cbeg, cpre);
-    const int T = cpre[27];
     double mn = INFINITY;
+    if (!all_atoms) {
+      cells_around(gi, ix, iy, iz, cstart, 1 - m, 0x7ffffffu, cbeg, cpre);
+      const int T = cpre[27];
+      for (int idx = lane(); idx < T; idx += WAVE) {
+        const int j = cell_candidate(cbeg, cpre, catom, idx);
+        const V3 d = xi - ld3(b.atom[j]);
+        mn = fmin(mn, dot_seq(d, d));
+      }
+      mn = wave_min_double(mn);
+      __syncthreads();
+    }
     unsigned long long ckey = ~0ull;
+    cells_around(gi, ix, iy, iz, cstart, m, coincident_cells(gi, ai.x, ai.y, ai.z, ix, iy, iz), cbeg, cpre);
+    const int T = cpre[27];
     for (int idx = lane(); idx < T; idx += WAVE) {
       const int j = cell_candidate(cbeg, cpre, catom, idx);
       const V3 d = xi - ld3(b.atom[j]);
-      const double d2 = dot_seq(d, d);
-      if (b.amol[j] != m) {
-        mn = fmin(mn, d2);
-      } else if (j != g && d2 <= 0.0001) {
+      if (j != g && dot_seq(d, d) <= 0.0001) {
         const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
         ckey = ckey < key ? ckey : key;
       }
     }
-    mn = wave_min_double(mn);
     for (int o = 32; o > 0; o >>= 1) {
       const unsigned long long t = __shfl_xor(ckey, o, WAVE);
       ckey = ckey < t ? ckey : t;
     }
-    if (lane() == 0) {
-      const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
-      atten[g] = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
-      access[g] = 0;
-      if (ckey != ~0ull) {
-        atomicMin(&coinc[c], ckey);
-        atomicOr(&status[c], ST_COINCIDENT);
-      }
-    }
+    if (lane() == 0) attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
     __syncthreads();
   }
 }
 
-// Interface region: non-blockers and atoms within the bridge distance of a
-// same-molecule non-blocker (every atom when full != 0).
-__global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, int full,
-                                                     const GridInfo* __restrict__ grid,
+// Interface region: every non-blocker marks the same-molecule atoms within
+// the bridge distance r_i + r_j + 2 rp of it (the relation is symmetric, so
+// this is the set of atoms within the bridge distance of a non-blocker).
+__global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, const GridInfo* __restrict__ grid,
                                                      const int* __restrict__ cstart,
                                                      const int* __restrict__ catom,
                                                      const int8_t* __restrict__ atten, int* __restrict__ need) {
   __shared__ int cbeg[27], cpre[28];
   CMS_UNIT_LOOP(sidx, b.ntot) {
     const int g = catom[sidx];
-    if (full || atten[g] > ATTEN_BLOCKER) {
-      if (lane() == 0) need[g] = 1;
-      continue;
-    }
+    if (atten[g] <= ATTEN_BLOCKER) continue;
     const int c = b.acplx[g];
     const int m = b.amol[g];
     const GridInfo gi = grid[c];
@@ -227,24 +276,15 @@ __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, in
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    cells_around(gi, ix, iy, iz, cstart, cbeg, cpre);
+    cells_around(gi, ix, iy, iz, cstart, m, 0x7ffffffu, cbeg, cpre);
     const int T = cpre[27];
-    bool keep = false;
-    for (int i0 = 0; i0 < T && !keep; i0 += WAVE) {
-      const int idx = i0 + lane();
-      bool hit = false;
-      if (idx < T) {
-        const int j = cell_candidate(cbeg, cpre, catom, idx);
-        if (b.amol[j] == m && atten[j] > ATTEN_BLOCKER) {
-          const double4 aj = b.atom[j];
-          const V3 d = xi - ld3(aj);
-          const double br = (ai.w + aj.w) + 2.0 * s.rp;
-          hit = dot_seq(d, d) < br * br;
-        }
-      }
-      keep = wave_any(hit);
+    for (int idx = lane(); idx < T; idx += WAVE) {
+      const int j = cell_candidate(cbeg, cpre, catom, idx);
+      const double4 aj = b.atom[j];
+      const V3 d = xi - ld3(aj);
+      const double br = (ai.w + aj.w) + 2.0 * s.rp;
+      if (dot_seq(d, d) < br * br) need[j] = 1;
     }
-    if (lane() == 0) need[g] = keep ? 1 : 0;
     __syncthreads();
   }
 }
@@ -252,11 +292,12 @@ __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, in
 // Thread-per-atom variants for large batches: consecutive threads take
 // consecutive cell-sorted atoms, so a wavefront reads a handful of cells
 // that stay in L1/L2 instead of every lane's own neighbourhood.
-__global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings s, int all_atoms,
+__global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings s, int all_atoms, int full,
                                                            const GridInfo* __restrict__ grid,
                                                            const int* __restrict__ cstart,
                                                            const int* __restrict__ catom,
                                                            int8_t* __restrict__ atten, int8_t* __restrict__ access,
+                                                           int* __restrict__ need,
                                                            unsigned long long* __restrict__ coinc,
                                                            int* __restrict__ status) {
   for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
@@ -270,58 +311,53 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     double mn = INFINITY;
+    if (!all_atoms) {
+      FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, 0x7ffffffu, cell) {
+        const int lo = cstart[2 * cell + 1 - m], hi = cstart[2 * cell + 2 - m];
+        for (int t = lo; t < hi; ++t) {
+          const V3 d = xi - ld3(b.atom[catom[t]]);
+          mn = fmin(mn, dot_seq(d, d));
+        }
+      }
+    }
     unsigned long long ckey = ~0ull;
-    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cell) {
-      for (int t = cstart[cell]; t < cstart[cell + 1]; ++t) {
+    const uint32_t cmask = coincident_cells(gi, ai.x, ai.y, ai.z, ix, iy, iz);
+    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cmask, cell) {
+      for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
         const int j = catom[t];
         const V3 d = xi - ld3(b.atom[j]);
-        const double d2 = dot_seq(d, d);
-        if (b.amol[j] != m) {
-          mn = fmin(mn, d2);
-        } else if (j != g && d2 <= 0.0001) {
+        if (j != g && dot_seq(d, d) <= 0.0001) {
           const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
           ckey = ckey < key ? ckey : key;
         }
       }
     }
-    const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
-    atten[g] = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
-    access[g] = 0;
-    if (ckey != ~0ull) {
-      atomicMin(&coinc[c], ckey);
-      atomicOr(&status[c], ST_COINCIDENT);
-    }
+    attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
   }
 }
 
-__global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s, int full,
-                                                        const GridInfo* __restrict__ grid,
+__global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s, const GridInfo* __restrict__ grid,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
                                                         const int8_t* __restrict__ atten, int* __restrict__ need) {
   for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
     const int g = catom[sidx];
-    int keep = full || atten[g] > ATTEN_BLOCKER;
-    if (!keep) {
-      const int c = b.acplx[g];
-      const int m = b.amol[g];
-      const GridInfo gi = grid[c];
-      const double4 ai = b.atom[g];
-      const V3 xi = ld3(ai);
-      int ix, iy, iz;
-      cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-      FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cell) {
-        for (int t = cstart[cell]; t < cstart[cell + 1] && !keep; ++t) {
-          const int j = catom[t];
-          if (b.amol[j] != m || atten[j] <= ATTEN_BLOCKER) continue;
-          const double4 aj = b.atom[j];
-          const V3 d = xi - ld3(aj);
-          const double br = (ai.w + aj.w) + 2.0 * s.rp;
-          if (dot_seq(d, d) < br * br) keep = 1;
-        }
+    if (atten[g] <= ATTEN_BLOCKER) continue;
+    const int m = b.amol[g];
+    const GridInfo gi = grid[b.acplx[g]];
+    const double4 ai = b.atom[g];
+    const V3 xi = ld3(ai);
+    int ix, iy, iz;
+    cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, 0x7ffffffu, cell) {
+      for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
+        const int j = catom[t];
+        const double4 aj = b.atom[j];
+        const V3 d = xi - ld3(aj);
+        const double br = (ai.w + aj.w) + 2.0 * s.rp;
+        if (dot_seq(d, d) < br * br) need[j] = 1;
       }
     }
-    need[g] = keep;
   }
 }
 
@@ -346,7 +382,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    cells_around(gi, ix, iy, iz, cstart, cbeg, cpre);
+    cells_around(gi, ix, iy, iz, cstart, 2, 0x7ffffffu, cbeg, cpre);
     const int T = cpre[27];
     int cnt = 0, cntb = 0, nbmax = -1;
     for (int i0 = 0; i0 < T; i0 += WAVE) {

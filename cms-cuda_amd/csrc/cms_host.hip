@@ -606,7 +606,8 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CK(ctx->prgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->dgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->dbase.ensure(sizeof(int64_t) * 4 * G));
-  const int64_t cell_cap = 8ll * n + 64ll * B;
+  // two slots (molecule 0, molecule 1) per cell
+  const int64_t cell_cap = 2 * (8ll * n + 64ll * B);
   CK(ctx->grid.ensure(sizeof(GridInfo) * (B + 1)));
   CK(ctx->gframe.ensure(sizeof(double4) * (B + 1)));
   CK(ctx->ncellc.ensure(sizeof(int) * (B + 1)));
@@ -638,7 +639,8 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     fill_int(ctx->ccount.as<int>(), cell_cap, 0, st);
     fill_int(ctx->ccur.as<int>(), cell_cap, 0, st);
     k_cell_count<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
-    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cell_cap, part, st);
+    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), dev_count(ctx->ncscan.as<int>() + B, 2), cell_cap, part,
+              st);
     k_cell_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
                                                             ctx->ccur.as<int>(), ctx->catom.as<int>());
     // small batches: one wavefront per atom (parallelism); large batches:
@@ -646,18 +648,22 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     const bool per_thread = n >= 200000;
     const int* cst = ctx->cstart.as<int>();
     const int* cat = ctx->catom.as<int>();
+    // every atom is in the region in all-atoms / full-state runs
+    const int full = (full_state || all_atoms) ? 1 : 0;
     if (per_thread) {
       TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
-          b, s, all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
-          ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
-      TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
-          b, s, full_state || all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+          b, s, all_atoms, full, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
+          ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
+      if (!full)
+        TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
+            b, s, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
     } else {
       TIMED("attention", k_attention_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
-          b, s, all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
-          ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
-      TIMED("region", k_region_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
-          b, s, full_state || all_atoms, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+          b, s, all_atoms, full, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
+          ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
+      if (!full)
+        TIMED("region", k_region_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
+            b, s, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
     }
     scan_excl(ctx->need.as<int>(), ctx->rpos.as<int>(), n, part, st);
     k_compact_index<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->need.as<int>(),

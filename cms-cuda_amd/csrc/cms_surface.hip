@@ -30,6 +30,7 @@ template <bool EMIT>
 __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
   const int nR = *w.nR;
   CMS_UNIT_LOOP(r, nR) {
+    if (EMIT && w.pcnt[r] == 0) continue;   // the count pass found nothing here
     const int g = w.ratom[r];
     const int nn = w.nn[r];
     const double4 ai = b.atom[g];
@@ -108,12 +109,18 @@ __device__ __forceinline__ int group_of_atom(const BatchView& b, int g) {
 }
 
 template <bool EMIT>
-__global__ __launch_bounds__(64) void k_probes(BatchView b, Settings s, Work w) {
+__global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
   __shared__ double4 lx[KCAP];
   const int nP = *w.nP;
   CMS_UNIT_LOOP(p, nP) {
     const int i = w.pi[p], j = w.pj[p];
+    if (EMIT) {
+      // pairs without surviving probes (most of them) are done
+      const int grp = group_of_atom(b, i);
+      const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
+      if (w.prcnt[2ll * g0 + (p - g0)] == 0 && w.prcnt[2ll * g0 + (g1 - g0) + (p - g0)] == 0) continue;
+    }
     const int r = w.rpos[i];
     const int nn = w.nn[r];
     const int* row = w.nbr + (int64_t)r * KCAP;
@@ -336,6 +343,7 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
   __shared__ int lat_start[MAX_SUBDIV + 1];
   const int nR = *w.nR;
   CMS_UNIT_LOOP(r, nR) {
+    if (EMIT && w.ccnt[r] == 0) continue;
     const int g = w.ratom[r];
     const int nn = w.nn[r];
     int total = 0;
@@ -463,6 +471,7 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
   __shared__ double4 lx[KCAP];
   const int nP = *w.nP;
   CMS_UNIT_LOOP(p, nP) {
+    if (EMIT && w.t1cnt[p] == 0 && w.t2cnt[p] == 0) continue;
     int tot1 = 0, tot2 = 0;
     bool any_live = false;
     if (w.pflag[p] & 2) {
@@ -602,6 +611,7 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
   const int nQ = *w.nQ;
   const double rp = s.rp, rp2 = s.rp * s.rp;
   CMS_UNIT_LOOP(q, nQ) {
+    if (EMIT && w.kcnt[q] == 0) continue;
     const int4 at = w.patom[q];
     const double4 pq = w.ppos[q];
     const V3 p = ld3(pq);
