@@ -56,6 +56,19 @@ __device__ __forceinline__ double dot_alt(V3 a, V3 b) {
   return (p0 + p2) + p1;
 }
 __device__ __forceinline__ double norm_seq(V3 a) { return sqrt(dot_seq(a, a)); }
+// norm_seq(a) < lim, decided without the square root unless q = |a|^2 lies
+// within a relative 2^-40 of lim^2.  Outside that band the rounded square
+// root provably sits on the same side of lim: q < fl(fl(lim^2)(1 - 2^-40))
+// implies sqrt(q) < lim (1 - 2^-42), whose rounding stays below lim, and
+// q > fl(fl(lim^2)(1 + 2^-40)) implies sqrt(q) > lim (1 + 2^-43).  NaN, 0
+// and infinite cases fall into the exact branch or agree with it.
+__device__ __forceinline__ bool norm_lt(V3 a, double lim) {
+  const double q = dot_seq(a, a);
+  const double l2 = lim * lim;
+  if (q < l2 * (1.0 - 0x1p-40)) return true;
+  if (q > l2 * (1.0 + 0x1p-40)) return false;
+  return sqrt(q) < lim;
+}
 // a x b with NumPy's component formulas
 __device__ __forceinline__ V3 cross(V3 a, V3 b) {
   return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
@@ -147,7 +160,18 @@ __device__ __forceinline__ double dist_to_line(V3 cen, V3 axis, V3 p) {
   return d2 < 0.0 ? 0.0 : sqrt(d2);
 }
 
-// grid-stride helpers: every kernel reads its unit count from device memory
-#define CMS_UNIT_LOOP(u, n) for (int64_t u = blockIdx.x; u < (int64_t)(n); u += gridDim.x)
+// Unit loop of the one-wavefront-per-unit kernels (unit count read from
+// device memory).  XCD-aware: workgroup w is dispatched to XCD w % 8, and
+// XCD x walks the x-th contiguous eighth of the units, so neighbouring units
+// (which share atoms and neighbour rows) meet in the same L2.  With fewer
+// than 8 workgroups each one takes a contiguous share.  Every unit is
+// visited exactly once.
+#define CMS_UNIT_LOOP(u, n)                                                                      \
+  for (int64_t u##_n = (int64_t)(n), u##_nx = gridDim.x < 8 ? gridDim.x : 8,                    \
+               u##_x = blockIdx.x % u##_nx, u##_per = (u##_n + u##_nx - 1) / u##_nx,            \
+               u##_hi = (u##_x + 1) * u##_per < u##_n ? (u##_x + 1) * u##_per : u##_n,          \
+               u##_step = (gridDim.x - u##_x + u##_nx - 1) / u##_nx,                            \
+               u = u##_x * u##_per + blockIdx.x / u##_nx;                                       \
+       u < u##_hi; u += u##_step)
 
 }  // namespace cms

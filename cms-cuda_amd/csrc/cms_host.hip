@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/cms_hip.h"
+#include "cms_hip.h"
 #include "cms_backward.h"
 #include "cms_cells.h"
 #include "cms_device.h"

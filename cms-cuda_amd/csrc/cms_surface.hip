@@ -108,19 +108,40 @@ __device__ __forceinline__ int group_of_atom(const BatchView& b, int g) {
   return 2 * b.acplx[g] + b.amol[g];
 }
 
+// Uniform copy of lane l's value (v_readlane on both halves).
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(bits & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(bits >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Triangle tests of slot k of atom i's row for the pair (i, j)
+// (_core.py:1478-1500): k > j, d(j, k) < erj + erk, d(i, k) < eri + erk, not
+// all three atoms blockers.
+__device__ __forceinline__ bool probe_triple_valid(int k, int j, V3 xk, double erk, V3 xi, V3 xj, double eri,
+                                                   double erj, bool bij, const int8_t* atten, double* dik) {
+  if (k <= j) return false;
+  if (!norm_lt(xk - xj, erj + erk)) return false;
+  if (!norm_lt(xk - xi, eri + erk)) return false;
+  *dik = norm_seq(xk - xi);
+  return !(bij && atten[k] <= ATTEN_BLOCKER);
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
   __shared__ double4 lx[KCAP];
+  constexpr int NCH = KCAP / WAVE;
   const int nP = *w.nP;
   CMS_UNIT_LOOP(p, nP) {
     const int i = w.pi[p], j = w.pj[p];
-    if (EMIT) {
-      // pairs without surviving probes (most of them) are done
-      const int grp = group_of_atom(b, i);
-      const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
-      if (w.prcnt[2ll * g0 + (p - g0)] == 0 && w.prcnt[2ll * g0 + (g1 - g0) + (p - g0)] == 0) continue;
-    }
+    const int grp = group_of_atom(b, i);
+    const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
+    const int64_t vp = 2ll * g0 + (p - g0);
+    const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
+    // pairs without surviving probes (most of them) are done
+    if (EMIT && w.prcnt[vp] == 0 && w.prcnt[vm] == 0) continue;
     const int r = w.rpos[i];
     const int nn = w.nn[r];
     const int* row = w.nbr + (int64_t)r * KCAP;
@@ -136,100 +157,118 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
     const double4 pu = w.pu[p], ptt = w.pt[p];
     const V3 uij = ld3(pu), tij = ld3(ptt);
     const double rij = pu.w;
-    const bool bi = w.atten[i] <= ATTEN_BLOCKER, bj = w.atten[j] <= ATTEN_BLOCKER;
+    const bool bij = w.atten[i] <= ATTEN_BLOCKER && w.atten[j] <= ATTEN_BLOCKER;
+    const int nch = (nn + WAVE - 1) / WAVE;
 
-    // pass 1: does a degenerate triple kill the pair?
+    // pass 1: valid, non-degenerate triples of every 64-slot chunk, and
+    // whether a degenerate triple kills the pair (_core.py:1505-1543).
+    // Degenerate means dt >= 1, dt <= -1, w <= 0 or sin w <= 0 with
+    // w = arccos(clip(dt)); for |dt| < 1 - 1e-6, w lies in
+    // (1.4e-3, pi - 1.4e-3), so none of these holds and arccos / sin are
+    // evaluated only near |dt| = 1.
+    uint64_t vmask[NCH];
     bool killed = false;
-    for (int s0 = 0; s0 < nn; s0 += WAVE) {
-      const int sl = s0 + lane();
-      bool kill = false;
-      if (sl < nn) {
-        const int k = lj[sl];
-        const V3 xk = ld3(lx[sl]);
-        const double erk = lx[sl].w + s.rp;
-        bool valid = k > j;
-        if (valid) valid = norm_seq(xk - xj) < erj + erk;
-        double dik = 0.0;
-        if (valid) { dik = norm_seq(xk - xi); valid = dik < eri + erk; }
-        if (valid) valid = !(bi && bj && w.atten[k] <= ATTEN_BLOCKER);
-        if (valid) {
-          const V3 uik = (xk - xi) / dik;
-          const double dt = dot_alt(uij, uik);
-          const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
-          const double sw = sin(wk);
-          const bool deg = (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
-          if (deg) {
-            const V3 dk = tij - xk;
-            kill = dot_seq(dk, dk) < erk * erk - rij * rij;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      vmask[c] = 0;
+      if (c < nch) {
+        const int sl = c * WAVE + lane();
+        bool cand = false, kill = false;
+        if (sl < nn) {
+          const V3 xk = ld3(lx[sl]);
+          const double erk = lx[sl].w + s.rp;
+          double dik = 0.0;
+          if (probe_triple_valid(lj[sl], j, xk, erk, xi, xj, eri, erj, bij, w.atten, &dik)) {
+            const V3 uik = (xk - xi) / dik;
+            const double dt = dot_alt(uij, uik);
+            bool deg = false;
+            if (!(fabs(dt) < 1.0 - 1e-6)) {
+              const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+              const double sw = sin(wk);
+              deg = (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
+            }
+            if (deg) {
+              const V3 dk = tij - xk;
+              kill = dot_seq(dk, dk) < erk * erk - rij * rij;
+            }
+            cand = !deg;
           }
         }
+        vmask[c] = ballot(cand);
+        killed = killed || wave_any(kill);
       }
-      killed = killed || wave_any(kill);
     }
+
+    // pass 2: probe positions of the candidates, then their collision with
+    // i's other neighbours (_core.py:1545-1623): one candidate at a time,
+    // the lanes spread over the neighbour row.
     int np_plus = 0, np_minus = 0;
     int ob_plus = 0, ob_minus = 0;
     if (EMIT) {
-      const int grp = group_of_atom(b, i);
-      const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
-      const int64_t vp = 2ll * g0 + (p - g0);
-      const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
       ob_plus = w.prscan[vp];
       ob_minus = w.prscan[vm];
     }
     if (!killed) {
-      for (int s0 = 0; s0 < nn; s0 += WAVE) {
-        const int sl = s0 + lane();
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        if (c >= nch || vmask[c] == 0) continue;
+        const int sl = c * WAVE + lane();
+        const bool mine = (vmask[c] >> lane()) & 1ull;
         bool okp = false, okm = false;
         int k = -1;
         V3 pp{0, 0, 0}, pm{0, 0, 0}, uijk{0, 0, 0};
         double h = 0.0;
-        if (sl < nn) {
+        if (mine) {
           k = lj[sl];
           const V3 xk = ld3(lx[sl]);
           const double erk = lx[sl].w + s.rp;
-          bool valid = k > j;
-          if (valid) valid = norm_seq(xk - xj) < erj + erk;
-          double dik = 0.0;
-          if (valid) { dik = norm_seq(xk - xi); valid = dik < eri + erk; }
-          if (valid) valid = !(bi && bj && w.atten[k] <= ATTEN_BLOCKER);
-          if (valid) {
-            const V3 uik = (xk - xi) / dik;
-            const double dt = dot_alt(uij, uik);
-            const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
-            const double sw = sin(wk);
-            const bool deg = (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
-            if (!deg) {
-              uijk = cross(uij, uik) / sw;
-              const V3 utb = cross(uijk, uij);
-              const double asymm = (eri * eri - erk * erk) / dik;
-              const V3 tik = (xi + xk) * 0.5 + uik * (asymm * 0.5);
-              const double dtb = dot_alt(uik, tik - tij);
-              const V3 bijk = tij + utb * (dtb / sw);
-              const V3 bi_ = bijk - xi;
-              const double hsq = eri * eri - dot_alt(bi_, bi_);
-              if (hsq > 0.0) {
-                h = sqrt(hsq);
-                pp = bijk + uijk * h;
-                pm = bijk + uijk * (-h);
-                okp = okm = true;
-                for (int t = 0; t < nn && (okp || okm); ++t) {
-                  const int n = lj[t];
-                  if (n == j || n == k) continue;
-                  const double rr = lx[t].w + s.rp;
-                  const double lim = rr * rr;
-                  const V3 xs = ld3(lx[t]);
-                  const V3 dp = pp - xs, dm = pm - xs;
-                  if (dot_alt(dp, dp) <= lim) okp = false;
-                  if (dot_alt(dm, dm) <= lim) okm = false;
-                }
-              }
-            }
+          const double dik = norm_seq(xk - xi);
+          const V3 uik = (xk - xi) / dik;
+          const double dt = dot_alt(uij, uik);
+          const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+          const double sw = sin(wk);
+          uijk = cross(uij, uik) / sw;
+          const V3 utb = cross(uijk, uij);
+          const double asymm = (eri * eri - erk * erk) / dik;
+          const V3 tik = (xi + xk) * 0.5 + uik * (asymm * 0.5);
+          const double dtb = dot_alt(uik, tik - tij);
+          const V3 bijk = tij + utb * (dtb / sw);
+          const V3 bi_ = bijk - xi;
+          const double hsq = eri * eri - dot_alt(bi_, bi_);
+          if (hsq > 0.0) {
+            h = sqrt(hsq);
+            pp = bijk + uijk * h;
+            pm = bijk + uijk * (-h);
+            okp = okm = true;
+          }
+        }
+        uint64_t cm = ballot(okp);
+        while (cm) {
+          const int cl = __builtin_ctzll(cm);
+          cm &= cm - 1;
+          const V3 P = v3(readlane_d(pp.x, cl), readlane_d(pp.y, cl), readlane_d(pp.z, cl));
+          const V3 M = v3(readlane_d(pm.x, cl), readlane_d(pm.y, cl), readlane_d(pm.z, cl));
+          const int kk = __builtin_amdgcn_readlane(k, cl);
+          bool hp = false, hm = false;
+          for (int t = lane(); t < nn; t += WAVE) {
+            const int n = lj[t];
+            if (n == j || n == kk) continue;
+            const double rr = lx[t].w + s.rp;
+            const double lim = rr * rr;
+            const V3 xs = ld3(lx[t]);
+            const V3 dp = P - xs, dm = M - xs;
+            hp = hp || dot_alt(dp, dp) <= lim;
+            hm = hm || dot_alt(dm, dm) <= lim;
+          }
+          const bool anyp = wave_any(hp), anym = wave_any(hm);
+          if (lane() == cl) {
+            okp = !anyp;
+            okm = !anym;
           }
         }
         const uint64_t mp = ballot(okp), mm = ballot(okm);
         if (!EMIT) {
-          if (okp) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
-          if (okm) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
+          if (okp || okm) { w.access[i] = 1; w.access[j] = 1; w.access[k] = 1; }
         } else {
           if (okp && ob_plus + np_plus + lanes_below(mp) < w.probe_cap) {
             const int o = ob_plus + np_plus + lanes_below(mp);
@@ -249,10 +288,8 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
       }
     }
     if (!EMIT && lane() == 0) {
-      const int grp = group_of_atom(b, i);
-      const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
-      w.prcnt[2ll * g0 + (p - g0)] = np_plus;
-      w.prcnt[2ll * g0 + (g1 - g0) + (p - g0)] = np_minus;
+      w.prcnt[vp] = np_plus;
+      w.prcnt[vm] = np_minus;
     }
     __syncthreads();
   }
@@ -469,6 +506,7 @@ template <bool EMIT>
 __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
   __shared__ double4 lx[KCAP];
+  __shared__ int rl[KCAP];
   const int nP = *w.nP;
   CMS_UNIT_LOOP(p, nP) {
     if (EMIT && w.t1cnt[p] == 0 && w.t2cnt[p] == 0) continue;
@@ -508,6 +546,28 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
       const double adelta = 1.0 / (sqrt(dens) * s.rp);
       const double rp2 = s.rp * s.rp;
       const bool arc1_on = w.atten[i] >= ATTEN_2, arc2_on = w.atten[j] >= ATTEN_2;
+      // Neighbours (other than j) that come within r + rp of the torus
+      // circle (centre tij, axis uij, radius rij): only they can collide
+      // with a torus point.  The 1e-4 A margin covers the rounding of the
+      // circle distance (cancellation in the radial term near the axis)
+      // and of the torus points, so the pruned set gives the same
+      // collision results.
+      int nrel = 0;
+      for (int s0 = 0; s0 < nn; s0 += WAVE) {
+        const int sl = s0 + lane();
+        bool rel = false;
+        if (sl < nn && lj[sl] != j) {
+          const V3 v = ld3(lx[sl]) - tij;
+          const double ax = dot_seq(v, uij);
+          const double rad = sqrt(fmax(dot_seq(v, v) - ax * ax, 0.0)) - rij;
+          const double reach = (lx[sl].w + s.rp) + 1e-4;
+          rel = rad * rad + ax * ax < reach * reach;
+        }
+        const uint64_t mk = ballot(rel);
+        if (rel) rl[nrel + lanes_below(mk)] = sl;
+        nrel += __popcll(mk);
+      }
+      __syncthreads();
       int64_t ob1 = 0, ob2 = 0;
       if (EMIT) {
         const int grp = group_of_atom(b, i);
@@ -522,8 +582,8 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
         if (m < M) {
           const V3 pm = subdiv_point(tij, cx, cy, rij, tdelta, m);
           live = true;
-          for (int sl = 0; sl < nn; ++sl) {
-            if (lj[sl] == j) continue;
+          for (int t = 0; t < nrel; ++t) {
+            const int sl = rl[t];
             const V3 d = pm - ld3(lx[sl]);
             const double lim = lx[sl].w + s.rp;
             if (dot_alt(d, d) < lim * lim) { live = false; break; }
