@@ -21,7 +21,7 @@ CMS_ERR_STATE = 5
 PREC_FP64 = 0
 PREC_FP32_RESCORE = 1
 
-NCOUNTS = 12
+NCOUNTS = 15
 SC_NRES = 24
 
 

@@ -224,15 +224,42 @@ def _make_one(args):
     return make_complex(nt, nb, seed, gap=gap)
 
 
-def mixed_batch(n, seed=3, workers=1, **size_kw):
-    """n complexes with batch_sizes(); workers > 1 generates them in a
+def generate(jobs, workers=1):
+    """Complexes for jobs (n_target, n_binder, seed, gap); workers > 1 uses a
     process pool (spawned, so it is safe after GPU initialisation)."""
-    nt, nb, gaps = batch_sizes(n, seed, **size_kw)
-    jobs = [(int(nt[i]), int(nb[i]), seed * 100003 + i, float(gaps[i])) for i in range(n)]
-    if workers <= 1 or n < 4:
+    jobs = list(jobs)
+    if workers <= 1 or len(jobs) < 4:
         return [_make_one(j) for j in jobs]
     import concurrent.futures
     import multiprocessing
     ctx = multiprocessing.get_context("spawn")
     with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
-        return list(ex.map(_make_one, jobs, chunksize=max(1, n // (4 * workers))))
+        return list(ex.map(_make_one, jobs, chunksize=max(1, len(jobs) // (4 * workers))))
+
+
+def mixed_jobs(n, seed=3, **size_kw):
+    nt, nb, gaps = batch_sizes(n, seed, **size_kw)
+    return [(int(nt[i]), int(nb[i]), seed * 100003 + i, float(gaps[i])) for i in range(n)]
+
+
+def mixed_batch(n, seed=3, workers=1, **size_kw):
+    """n complexes with batch_sizes()."""
+    return generate(mixed_jobs(n, seed, **size_kw), workers)
+
+
+def c3_batch(seed=3, workers=1, n=2048):
+    """2,048 complexes: binders log-uniform 20-1,800 atoms (ligand-like up to
+    60), targets uniform 800-6,000, gaps uniform 3.5-5.0 A."""
+    return mixed_batch(n, seed, workers)
+
+
+def c4_batch(seed=3, workers=1, n=2048, n_small=64):
+    """The c3 batch plus 64 complexes of the 400 + 1,000-atom size (gap 4 A)."""
+    jobs = mixed_jobs(n, seed) + [(1000, 400, seed * 100003 + n + i, 4.0) for i in range(n_small)]
+    return generate(jobs, workers)
+
+
+def c5_batch(seed=5, workers=1, n=256):
+    """256 complexes: 1,500-atom targets, binders log-uniform 80-600 atoms,
+    gaps uniform 3.5-5.0 A (the differentiation workload)."""
+    return mixed_batch(n, seed, workers, binder_lo=80, binder_hi=600, target_lo=1500, target_hi=1500)

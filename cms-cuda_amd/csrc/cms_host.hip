@@ -1326,6 +1326,7 @@ int cms_counts(cms_ctx* ctx, int64_t* out) {
   if (r) return r;
   std::vector<int> bg(G + 1), rg(G + 1), pg(G + 1), prg(G + 1);
   std::vector<int> cs(ctx->nR + 1), t1(ctx->nP + 1), t2(ctx->nP + 1), ks(ctx->nQ + 1);
+  std::vector<int> hnn(ctx->nR + 1), hnb(ctx->nR + 1), hpc(ctx->nR + 1);
   std::vector<int8_t> att(ctx->ntot + 1);
   CK(hipMemcpyAsync(bg.data(), ctx->bgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, st));
   CK(hipMemcpyAsync(rg.data(), ctx->rgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, st));
@@ -1335,6 +1336,11 @@ int cms_counts(cms_ctx* ctx, int64_t* out) {
   CK(hipMemcpyAsync(t1.data(), ctx->t1scan.p, sizeof(int) * (ctx->nP + 1), hipMemcpyDeviceToHost, st));
   CK(hipMemcpyAsync(t2.data(), ctx->t2scan.p, sizeof(int) * (ctx->nP + 1), hipMemcpyDeviceToHost, st));
   CK(hipMemcpyAsync(ks.data(), ctx->kscan.p, sizeof(int) * (ctx->nQ + 1), hipMemcpyDeviceToHost, st));
+  if (ctx->nR > 0) {
+    CK(hipMemcpyAsync(hnn.data(), ctx->nn.p, sizeof(int) * ctx->nR, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(hnb.data(), ctx->nb.p, sizeof(int) * ctx->nR, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(hpc.data(), ctx->pcnt.p, sizeof(int) * ctx->nR, hipMemcpyDeviceToHost, st));
+  }
   if (ctx->ntot > 0) CK(hipMemcpyAsync(att.data(), ctx->atten.p, ctx->ntot, hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
   for (int c = 0; c < B; ++c) {
@@ -1355,6 +1361,15 @@ int cms_counts(cms_ctx* ctx, int64_t* out) {
     for (int a = ctx->h_agrp[g1]; a < ctx->h_agrp[g2]; ++a) bl1 += att[a] == ATTEN_BLOCKER;
     o[10] = bl0;
     o[11] = bl1;
+    int64_t en = 0, eb = 0, pn = 0;
+    for (int rr = rg[g0]; rr < rg[g2]; ++rr) {
+      en += hnn[rr];
+      eb += hnb[rr];
+      pn += (int64_t)hpc[rr] * hnn[rr];
+    }
+    o[12] = en;
+    o[13] = eb;
+    o[14] = pn;
   }
   return CMS_OK;
 }

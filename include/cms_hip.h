@@ -40,10 +40,12 @@ extern "C" {
                                    within the proven error margin                    */
 
 /* per-complex counters written by cms_counts() */
-#define CMS_NCOUNTS 12
+#define CMS_NCOUNTS 15
 /* [0] dots mol 0, [1] dots mol 1, [2] buried dots mol 0, [3] buried dots mol 1,
    [4] convex, [5] toroidal, [6] concave, [7] probes, [8] region atoms,
-   [9] pairs, [10] blocked atoms mol 0, [11] blocked atoms mol 1            */
+   [9] pairs, [10] blocked atoms mol 0, [11] blocked atoms mol 1,
+   [12] neighbour-row entries, [13] buried-row entries, [14] sum over pairs
+   (i, j) of the length of atom i's neighbour row                          */
 
 /* per-complex SC record written by cms_shape_complementarity() */
 #define CMS_SC_NRES 24
