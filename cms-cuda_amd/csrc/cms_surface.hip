@@ -128,6 +128,36 @@ __device__ __forceinline__ bool probe_triple_valid(int k, int j, V3 xk, double e
   return !(bij && atten[k] <= ATTEN_BLOCKER);
 }
 
+// Torus-circle containment.  The circle of the pair (i, j) -- centre t,
+// axis u, radius rho -- holds every probe of the pair (a probe touches i
+// and j) and every torus point.  If one of i's row neighbours s other than
+// j holds the whole circle inside its sphere shrunk by COVER_MARGIN -- the
+// farthest circle point from x_s, at distance sqrt((d + rho)^2 + a^2) with
+// a and d the axial and radial offsets of x_s, lies within
+// r_s + rp - COVER_MARGIN -- then every torus point collides with s, and so
+// does every probe: s is neither j nor the probe's own third atom k (a probe
+// of k sits on k's sphere, which therefore cannot hold the circle).  The
+// pair then has neither probes nor torus points.  Computed torus points and
+// probes lie within 1e-4 A of the circle (the caller excludes pairs with a
+// candidate probe of sin w < 1e-6) and the containment test's own rounding
+// is below 1e-7 A, both far inside the margin.
+constexpr double COVER_MARGIN = 0.01;
+
+__device__ __forceinline__ bool circle_in_one_sphere(const int* lj, const double4* lx, int nn, int j, V3 t, V3 u,
+                                                     double rho, double rp) {
+  bool hit = false;
+  for (int sl = lane(); sl < nn; sl += WAVE) {
+    if (lj[sl] == j) continue;
+    const V3 v = ld3(lx[sl]) - t;
+    const double a = dot_seq(v, u);
+    const double d = sqrt(fmax(dot_seq(v, v) - a * a, 0.0));
+    const double R = (lx[sl].w + rp) - COVER_MARGIN;
+    const double far2 = (d + rho) * (d + rho) + a * a;
+    hit = hit || (R > 0.0 && far2 < R * R);
+  }
+  return wave_any(hit);
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
@@ -167,13 +197,14 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
     // (1.4e-3, pi - 1.4e-3), so none of these holds and arccos / sin are
     // evaluated only near |dt| = 1.
     uint64_t vmask[NCH];
-    bool killed = false;
+    bool killed = false, near_deg = false;
+    int ncand = 0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
       vmask[c] = 0;
       if (c < nch) {
         const int sl = c * WAVE + lane();
-        bool cand = false, kill = false;
+        bool cand = false, kill = false, nd = false;
         if (sl < nn) {
           const V3 xk = ld3(lx[sl]);
           const double erk = lx[sl].w + s.rp;
@@ -192,10 +223,27 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
               kill = dot_seq(dk, dk) < erk * erk - rij * rij;
             }
             cand = !deg;
+            nd = cand && !(fabs(dt) < 1.0 - 1e-12);   // sin w below ~1.4e-6
           }
         }
         vmask[c] = ballot(cand);
+        ncand += __popcll(vmask[c]);
         killed = killed || wave_any(kill);
+        near_deg = near_deg || wave_any(nd);
+      }
+    }
+    // pairs whose torus circle lies inside one neighbour's sphere have no
+    // probes and no torus points (flag bit 4, read by the toroidal stage)
+    if (!EMIT) {
+      const bool tor = (w.pflag[p] & 2) != 0;
+      if (!near_deg && (tor || (ncand > 0 && !killed)) && circle_in_one_sphere(lj, lx, nn, j, tij, uij, rij, s.rp)) {
+        if (lane() == 0) {
+          w.pflag[p] = (int8_t)(w.pflag[p] | 4);
+          w.prcnt[vp] = 0;
+          w.prcnt[vm] = 0;
+        }
+        __syncthreads();
+        continue;
       }
     }
 
@@ -512,7 +560,7 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
     if (EMIT && w.t1cnt[p] == 0 && w.t2cnt[p] == 0) continue;
     int tot1 = 0, tot2 = 0;
     bool any_live = false;
-    if (w.pflag[p] & 2) {
+    if ((w.pflag[p] & 6) == 2) {   // toroid queue, circle not covered
       const int i = w.pi[p], j = w.pj[p];
       const int ri_ = w.rpos[i], rj_ = w.rpos[j];
       const int nn = w.nn[ri_];
