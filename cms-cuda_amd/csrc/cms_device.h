@@ -174,4 +174,25 @@ __device__ __forceinline__ double dist_to_line(V3 cen, V3 axis, V3 p) {
                u = u##_x * u##_per + blockIdx.x / u##_nx;                                       \
        u < u##_hi; u += u##_step)
 
+// Width of the unit blocks of a sparse loop: 64 when there are enough units
+// to give every workgroup several blocks, down to 1 for small launches.
+__device__ __forceinline__ int64_t sparse_width(int64_t n, int64_t grid) {
+  const int64_t w = n / (4 * grid);
+  return w < 1 ? 1 : (w > WAVE ? WAVE : w);
+}
+
+// Sparse unit loop: units in blocks of sparse_width(n) (blocks visited like
+// CMS_UNIT_LOOP); the lanes evaluate pred on the block's units at once and
+// the wave runs the body, uniformly, for each unit whose pred holds, in
+// increasing order.  `continue` in the body moves to the next such unit.
+#define CMS_SPARSE_LOOP(u, n, pred)                                                               \
+  for (int64_t u##_nn = (int64_t)(n), u##_w = sparse_width(u##_nn, gridDim.x), u##_one = 1; u##_one; \
+       u##_one = 0)                                                                               \
+    CMS_UNIT_LOOP(u##_blk, (u##_nn + u##_w - 1) / u##_w)                                          \
+      for (uint64_t u##_m = ballot(lane() < u##_w && u##_blk * u##_w + lane() < u##_nn &&         \
+                                   (pred)(u##_blk * u##_w + lane()));                             \
+           u##_m;)                                                                                \
+        for (int64_t u = u##_blk * u##_w + __builtin_ctzll(u##_m), u##_o = (u##_m &= u##_m - 1, 1); \
+             u##_o; u##_o = 0)
+
 }  // namespace cms

@@ -729,7 +729,12 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CKL();
   // ---- dot counts
   if (est[M_R] > 0) TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
-  if (est[M_P] > 0) TIMED("toroid_count", k_toroid<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_P] > 0) {
+    // the count pass visits only live toroid-queue pairs
+    fill_int(ctx->t1cnt.as<int>(), ctx->cap[M_P], 0, st);
+    fill_int(ctx->t2cnt.as<int>(), ctx->cap[M_P], 0, st);
+    TIMED("toroid_count", k_toroid<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  }
   if (est[M_Q] > 0) {
     if (s.rp > 0.0)
       TIMED("concave_count", k_concave<false><<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));

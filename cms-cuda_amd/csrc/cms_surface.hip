@@ -29,8 +29,9 @@ __global__ void k_compact_index(int n, const int* __restrict__ flag, const int* 
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
   const int nR = *w.nR;
-  CMS_UNIT_LOOP(r, nR) {
-    if (EMIT && w.pcnt[r] == 0) continue;   // the count pass found nothing here
+  // the emit pass visits only rows whose count pass found pairs
+  auto live = [&](int64_t q) { return !EMIT || w.pcnt[q] != 0; };
+  CMS_SPARSE_LOOP(r, nR, live) {
     const int g = w.ratom[r];
     const int nn = w.nn[r];
     const double4 ai = b.atom[g];
@@ -164,14 +165,14 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
   __shared__ double4 lx[KCAP];
   constexpr int NCH = KCAP / WAVE;
   const int nP = *w.nP;
-  CMS_UNIT_LOOP(p, nP) {
+  // the emit pass visits only pairs with surviving probes (flag bit 8)
+  auto live = [&](int64_t q) { return !EMIT || (w.pflag[q] & 8) != 0; };
+  CMS_SPARSE_LOOP(p, nP, live) {
     const int i = w.pi[p], j = w.pj[p];
     const int grp = group_of_atom(b, i);
     const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
     const int64_t vp = 2ll * g0 + (p - g0);
     const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
-    // pairs without surviving probes (most of them) are done
-    if (EMIT && w.prcnt[vp] == 0 && w.prcnt[vm] == 0) continue;
     const int r = w.rpos[i];
     const int nn = w.nn[r];
     const int* row = w.nbr + (int64_t)r * KCAP;
@@ -338,6 +339,7 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
     if (!EMIT && lane() == 0) {
       w.prcnt[vp] = np_plus;
       w.prcnt[vm] = np_minus;
+      if (np_plus + np_minus > 0) w.pflag[p] = (int8_t)(w.pflag[p] | 8);
     }
     __syncthreads();
   }
@@ -427,8 +429,8 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
   __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
   __shared__ int lat_start[MAX_SUBDIV + 1];
   const int nR = *w.nR;
-  CMS_UNIT_LOOP(r, nR) {
-    if (EMIT && w.ccnt[r] == 0) continue;
+  auto live = [&](int64_t q) { return !EMIT || w.ccnt[q] != 0; };
+  CMS_SPARSE_LOOP(r, nR, live) {
     const int g = w.ratom[r];
     const int nn = w.nn[r];
     int total = 0;
@@ -556,8 +558,13 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
   __shared__ double4 lx[KCAP];
   __shared__ int rl[KCAP];
   const int nP = *w.nP;
-  CMS_UNIT_LOOP(p, nP) {
-    if (EMIT && w.t1cnt[p] == 0 && w.t2cnt[p] == 0) continue;
+  // count pass: pairs of the toroid queue whose circle is not inside a
+  // neighbour's sphere (the counts of the others are zero-filled before);
+  // emit pass: pairs with toroidal dots
+  auto live = [&](int64_t q) {
+    return EMIT ? (w.t1cnt[q] != 0 || w.t2cnt[q] != 0) : (w.pflag[q] & 6) == 2;
+  };
+  CMS_SPARSE_LOOP(p, nP, live) {
     int tot1 = 0, tot2 = 0;
     bool any_live = false;
     if ((w.pflag[p] & 6) == 2) {   // toroid queue, circle not covered
@@ -718,8 +725,8 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
   __shared__ double4 lnear[NEARCAP];
   const int nQ = *w.nQ;
   const double rp = s.rp, rp2 = s.rp * s.rp;
-  CMS_UNIT_LOOP(q, nQ) {
-    if (EMIT && w.kcnt[q] == 0) continue;
+  auto live = [&](int64_t u) { return !EMIT || w.kcnt[u] != 0; };
+  CMS_SPARSE_LOOP(q, nQ, live) {
     const int4 at = w.patom[q];
     const double4 pq = w.ppos[q];
     const V3 p = ld3(pq);
