@@ -138,10 +138,13 @@ __device__ __forceinline__ bool probe_triple_valid(int k, int j, V3 xk, double e
 // r_s + rp - COVER_MARGIN -- then every torus point collides with s, and so
 // does every probe: s is neither j nor the probe's own third atom k (a probe
 // of k sits on k's sphere, which therefore cannot hold the circle).  The
-// pair then has neither probes nor torus points.  Computed torus points and
-// probes lie within 1e-4 A of the circle (the caller excludes pairs with a
-// candidate probe of sin w < 1e-6) and the containment test's own rounding
-// is below 1e-7 A, both far inside the margin.
+// pair then has neither probes nor torus points.  Computed torus points lie
+// within 1e-12 A of the circle; a computed probe lies within 1e-4 A of it
+// when sin w >= 1.4e-5 (arccos amplifies the rounding of the cosine by
+// 1 / sin w, and the base point moves by at most twice the probe radius
+// times that relative error), so the caller excludes pairs with a candidate
+// closer to degenerate; the containment test's own rounding is below
+// 1e-7 A.  All far inside the margin.
 constexpr double COVER_MARGIN = 0.01;
 
 __device__ __forceinline__ bool circle_in_one_sphere(const int* lj, const double4* lx, int nn, int j, V3 t, V3 u,
@@ -167,6 +170,7 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
   const int nP = *w.nP;
   // the emit pass visits only pairs with surviving probes (flag bit 8)
   auto live = [&](int64_t q) { return !EMIT || (w.pflag[q] & 8) != 0; };
+  int staged = -1;   // neighbour row held in lj / lx (consecutive pairs share it)
   CMS_SPARSE_LOOP(p, nP, live) {
     const int i = w.pi[p], j = w.pj[p];
     const int grp = group_of_atom(b, i);
@@ -175,13 +179,16 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
     const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
     const int r = w.rpos[i];
     const int nn = w.nn[r];
-    const int* row = w.nbr + (int64_t)r * KCAP;
-    for (int sl = lane(); sl < nn; sl += WAVE) {
-      const int k = row[sl];
-      lj[sl] = k;
-      lx[sl] = b.atom[k];
+    if (r != staged) {
+      const int* row = w.nbr + (int64_t)r * KCAP;
+      for (int sl = lane(); sl < nn; sl += WAVE) {
+        const int k = row[sl];
+        lj[sl] = k;
+        lx[sl] = b.atom[k];
+      }
+      __syncthreads();
+      staged = r;
     }
-    __syncthreads();
     const double4 ai = b.atom[i], aj = b.atom[j];
     const V3 xi = ld3(ai), xj = ld3(aj);
     const double eri = ai.w + s.rp, erj = aj.w + s.rp;
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
               kill = dot_seq(dk, dk) < erk * erk - rij * rij;
             }
             cand = !deg;
-            nd = cand && !(fabs(dt) < 1.0 - 1e-12);   // sin w below ~1.4e-6
+            nd = cand && !(fabs(dt) < 1.0 - 1e-10);   // sin w below ~1.4e-5
           }
         }
         vmask[c] = ballot(cand);
@@ -564,6 +571,7 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
   auto live = [&](int64_t q) {
     return EMIT ? (w.t1cnt[q] != 0 || w.t2cnt[q] != 0) : (w.pflag[q] & 6) == 2;
   };
+  int staged = -1;   // neighbour row held in lj / lx
   CMS_SPARSE_LOOP(p, nP, live) {
     int tot1 = 0, tot2 = 0;
     bool any_live = false;
@@ -571,13 +579,16 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
       const int i = w.pi[p], j = w.pj[p];
       const int ri_ = w.rpos[i], rj_ = w.rpos[j];
       const int nn = w.nn[ri_];
-      const int* row = w.nbr + (int64_t)ri_ * KCAP;
-      for (int sl = lane(); sl < nn; sl += WAVE) {
-        const int k = row[sl];
-        lj[sl] = k;
-        lx[sl] = b.atom[k];
+      if (ri_ != staged) {
+        const int* row = w.nbr + (int64_t)ri_ * KCAP;
+        for (int sl = lane(); sl < nn; sl += WAVE) {
+          const int k = row[sl];
+          lj[sl] = k;
+          lx[sl] = b.atom[k];
+        }
+        __syncthreads();
+        staged = ri_;
       }
-      __syncthreads();
       const double4 ai = b.atom[i], aj = b.atom[j];
       const V3 xi = ld3(ai), xj = ld3(aj);
       const double ri = ai.w, rj = aj.w;
