@@ -262,6 +262,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--kernel", default="auto", help="kernel timed for the roofline (auto: the slowest modelled)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="also run a 512 MiB read + 512 MiB write elementwise kernel three times (the reference "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the CMS/SC path shards by complex across GPUs "
@@ -301,6 +304,15 @@ def main():
     if kname == "auto":
         cand = [k for k in models if k in breakdown]
         kname = max(cand, key=lambda k: breakdown[k]) if cand else "attention"
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    # this workload (scaled by a copy kernel of known size), when recorded
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("precision") == prec_name:
+            traffic = tr.get("bytes_per_launch", {})
     roofline = None
     by_kernel = {}
     for k in models:
@@ -311,11 +323,20 @@ def main():
         scale = 1e12 if unit == "TFLOP/s" else 1e9
         ach = work / avg_s / scale
         rec = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
-               "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": launches,
+               "traffic": traffic.get(k), "avg_launch_us": avg_s * 1e6, "launches": launches,
                "work_per_launch": work, "work": definition}
         by_kernel[k] = rec
         if k == kname:
             roofline = dict(rec, kernel=k)
+
+    if args.traffic_probe:
+        # x * 2 over 2^26 float64: 512 MiB read, 512 MiB written per launch
+        xs = torch.ones(1 << 26, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            ys = xs * 2.0
+        torch.cuda.synchronize()
+        del xs, ys
+        log("traffic probe: 3 launches of a 512 MiB -> 512 MiB elementwise kernel")
 
     # FP32 search against FP64 on the same batch (surface identical by
     # construction: only the searches change precision)

@@ -55,6 +55,7 @@ _PROTOS = {
     "cms_dot_count": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.POINTER(_i64)]),
     "cms_copy_dots": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cms_copy_atom_state": (_c.c_int, [_vp, _vp, _vp]),
+    "cms_copy_dot_desc": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp]),
     "cms_probe_count": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_i64)]),
     "cms_copy_probes": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp]),
     "cms_nearest": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),

@@ -40,29 +40,29 @@ struct V3 {
   double x, y, z;
 };
 
-__device__ __forceinline__ V3 v3(double x, double y, double z) { return V3{x, y, z}; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V3 operator*(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ V3 operator/(V3 a, double s) { return V3{a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ __forceinline__ V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+__host__ __device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ __forceinline__ V3 operator*(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ __forceinline__ V3 operator/(V3 a, double s) { return V3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ V3 neg(V3 a) { return V3{-a.x, -a.y, -a.z}; }
 
-__device__ __forceinline__ double dot_seq(V3 a, V3 b) {
+__host__ __device__ __forceinline__ double dot_seq(V3 a, V3 b) {
   double p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
   return (p0 + p1) + p2;
 }
-__device__ __forceinline__ double dot_alt(V3 a, V3 b) {
+__host__ __device__ __forceinline__ double dot_alt(V3 a, V3 b) {
   double p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
   return (p0 + p2) + p1;
 }
-__device__ __forceinline__ double norm_seq(V3 a) { return sqrt(dot_seq(a, a)); }
+__host__ __device__ __forceinline__ double norm_seq(V3 a) { return sqrt(dot_seq(a, a)); }
 // norm_seq(a) < lim, decided without the square root unless q = |a|^2 lies
 // within a relative 2^-40 of lim^2.  Outside that band the rounded square
 // root provably sits on the same side of lim: q < fl(fl(lim^2)(1 - 2^-40))
 // implies sqrt(q) < lim (1 - 2^-42), whose rounding stays below lim, and
 // q > fl(fl(lim^2)(1 + 2^-40)) implies sqrt(q) > lim (1 + 2^-43).  NaN, 0
 // and infinite cases fall into the exact branch or agree with it.
-__device__ __forceinline__ bool norm_lt(V3 a, double lim) {
+__host__ __device__ __forceinline__ bool norm_lt(V3 a, double lim) {
   const double q = dot_seq(a, a);
   const double l2 = lim * lim;
   if (q < l2 * (1.0 - 0x1p-40)) return true;

@@ -1278,6 +1278,27 @@ int cms_copy_dots(cms_ctx* ctx, int c, int mol, double* coor, double* nml, doubl
   return CMS_OK;
 }
 
+__global__ void k_unpack_desc(int n, int lo, const int4* d1, const int4* d2, int32_t* o1, int32_t* o2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int4 a = d1[lo + i], c = d2[lo + i];
+    o1[4 * i] = a.x; o1[4 * i + 1] = a.y; o1[4 * i + 2] = a.z; o1[4 * i + 3] = a.w;
+    o2[4 * i] = c.x; o2[4 * i + 1] = c.y; o2[4 * i + 2] = c.z; o2[4 * i + 3] = c.w;
+  }
+}
+
+int cms_copy_dot_desc(cms_ctx* ctx, int c, int mol, int32_t* d1, int32_t* d2) {
+  int64_t n = 0;
+  int r = cms_dot_count(ctx, c, mol, &n);
+  if (r) return r;
+  if (!d1 || !d2) return CMS_ERR_INVALID;
+  if (n == 0) return CMS_OK;
+  const int g = 2 * c + mol;
+  k_unpack_desc<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>((int)n, ctx->h_dgrp[g], ctx->ddesc.as<int4>(),
+                                                                   ctx->ddesc2.as<int4>(), d1, d2);
+  CKL();
+  return CMS_OK;
+}
+
 int cms_copy_atom_state(cms_ctx* ctx, int8_t* atten, int8_t* access) {
   if (!ctx) return CMS_ERR_INVALID;
   if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_copy_atom_state: no surface");

@@ -168,6 +168,16 @@ int cms_dot_count(cms_ctx* ctx, int complex_index, int mol, int64_t* n);
 int cms_copy_dots(cms_ctx* ctx, int complex_index, int mol, double* coor, double* nml,
                   double* area, int8_t* buried, int8_t* type, int32_t* atom);
 
+/* Frozen-topology descriptors of the dots of (complex, molecule), in the
+   order of cms_copy_dots: d1, d2 device int32 4n each.  d1 = (kind | mm << 4,
+   atom a, atom b, atom c) with kind 1 convex (a = owner, b = its nearest
+   neighbour), 2 / 3 toroidal arc of atom i / j (a = i, b = j), 4 concave
+   (a, b, c = the probe's atoms, mm = the vertex the latitudes start from);
+   d2 = (latitude or torus point, point on it, latitude / torus point count,
+   point count).  Atom indices are batch-global.  No reference counterpart
+   (what the backward pass re-evaluates a dot from). */
+int cms_copy_dot_desc(cms_ctx* ctx, int complex_index, int mol, int32_t* d1, int32_t* d2);
+
 /* Per-atom attention numbers and access flags (device, ntot int8 each) as
    left by surface generation before the toroidal stage, plus the per-pair
    toroid-queue flags needed to finish access the reference's way.

@@ -119,3 +119,60 @@ def test_autograd_function(complexes):
     assert b.grad is not None and t.grad is not None
     assert torch.isfinite(b.grad).all() and torch.isfinite(t.grad).all()
     assert b.grad.abs().sum() > 0 and t.grad.abs().sum() > 0
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_gradient_matches_torch_fp64_reference(complexes, which):
+    """The HIP backward (hand-written adjoint + dual-number VJP) against
+    torch autograd through an independent FP64 torch re-implementation of
+    the frozen-topology CMS (tests/torch_frozen.py): dot positions and areas
+    from the exported descriptors, partners from the forward dots."""
+    import torch
+    from scipy.spatial import cKDTree
+    from cms_cuda import _lib
+    from cms_cuda.autograd import _ContactMS
+    import torch_frozen as tf
+    say("torch FP64 reference gradient, complex %d" % which)
+    c = complexes[which]
+    xyz, rad, nb, nt = batch_inputs([c])
+    st = _ContactMS(torch.from_numpy(xyz).cuda(), torch.from_numpy(rad).cuda(), nb, nt, None, _lib.PREC_FP64)
+    per_mol = []
+    for mol in (0, 1):
+        n = ctypes.c_int64()
+        st.ctx("cms_dot_count", 0, mol, ctypes.byref(n))
+        n = n.value
+        coor = torch.empty((max(n, 1), 3), dtype=torch.float64, device="cuda")
+        area = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+        bur = torch.empty(max(n, 1), dtype=torch.int8, device="cuda")
+        d1 = torch.empty((max(n, 1), 4), dtype=torch.int32, device="cuda")
+        d2 = torch.empty((max(n, 1), 4), dtype=torch.int32, device="cuda")
+        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        st.ctx("cms_copy_dots", 0, mol, P(coor), None, P(area), P(bur), None, None)
+        st.ctx("cms_copy_dot_desc", 0, mol, P(d1), P(d2))
+        b = bur[:n].cpu().numpy().astype(bool)
+        per_mol.append(dict(coor=coor[:n].cpu().numpy()[b], area=area[:n].cpu().numpy()[b],
+                            d1=d1[:n].cpu().long()[torch.from_numpy(b)], d2=d2[:n].cpu().long()[torch.from_numpy(b)]))
+    assert len(per_mol[0]["area"]) > 100 and len(per_mol[1]["area"]) > 100
+    X = torch.from_numpy(xyz.copy()).requires_grad_(True)
+    R = torch.from_numpy(rad.copy())
+    D = torch.full((len(rad),), 15.0, dtype=torch.float64)
+    # the torch evaluator reproduces the forward dots
+    for m in (0, 1):
+        Pq, Aq = tf.dots(X.detach(), R, D, 1.7, per_mol[m]["d1"], per_mol[m]["d2"])
+        assert np.max(np.abs(Pq.numpy() - per_mol[m]["coor"])) < 1e-11
+        assert np.max(np.abs(Aq.numpy() - per_mol[m]["area"])) < 1e-11
+    # target side: query = buried dots of molecule 1, partners among molecule 0's
+    q, r = per_mol[1], per_mol[0]
+    partner = torch.from_numpy(cKDTree(r["coor"]).query(q["coor"])[1])
+    cms = tf.frozen_cms(X, R, D, 1.7, 0.5, (q["d1"], q["d2"]), (r["d1"], r["d2"]), partner)
+    tot = st.totals.cpu().numpy()
+    assert abs(cms.item() - tot[1]) <= 1e-10 * max(1.0, abs(tot[1]))
+    (g_ref,) = torch.autograd.grad(cms, X)
+    g_tot = torch.tensor([0.0, 1.0], dtype=torch.float64, device="cuda")
+    g_hip = st.backward(g_tot, None).cpu().numpy()
+    g_ref = g_ref.numpy()
+    scale = np.max(np.abs(g_ref))
+    err = np.max(np.abs(g_hip - g_ref)) / scale
+    say("  max |grad| %.3e, max abs diff / max |grad| %.2e" % (scale, err))
+    assert scale > 0 and err < 1e-9
+    st.close()
