@@ -12,6 +12,7 @@ for p in (ROOT, PKG):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU and the built HIP library")
+    config.addinivalue_line("markers", "slow: CPU test of several seconds")
 
 
 @pytest.fixture(scope="session")

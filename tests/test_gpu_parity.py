@@ -311,3 +311,59 @@ def test_fp32_mode_batch_identical(cms_cuda, golden_complex):
     assert np.array_equal(a64["per_atom"], a32["per_atom"])
     assert np.array_equal(s64, s32)
     bc.close()
+
+
+# ---------------------------------------------------------------- vectors
+# written by the reference's own calculator (tests/golden/make_golden.py)
+
+def _fixture(name):
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name), allow_pickle=False)
+
+
+def _gpu_run(cms_cuda, bx, br, tx, tr):
+    calc = cms_cuda.MolecularSurfaceCalculator()
+    calc.add_binder_and_target(bx, br, tx, tr)
+    cms_t, per_t = calc.CalcLoaded()
+    cms_b, per_b = calc.calc_contact_molecular_surface(target_side=False)
+    sc = calc.CalcLoadedSC()
+    return calc, cms_t, per_t, cms_b, per_b, sc
+
+
+def _check_scalars(f, cms_t, per_t, cms_b, per_b, sc):
+    assert rel_err(cms_t, f["cms_target"]) <= TOL_OUT
+    assert rel_err(cms_b, f["cms_binder"]) <= TOL_OUT
+    assert len(per_t) == len(f["per_target"]) and rel_err(per_t, f["per_target"]) <= TOL_OUT
+    assert len(per_b) == len(f["per_binder"]) and rel_err(per_b, f["per_binder"]) <= TOL_OUT
+    assert rel_err(list(sc), f["sc"]) <= TOL_OUT
+
+
+def test_reference_vectors_small_complex(cms_cuda):
+    """Every dot of a synthetic complex in the reference's order."""
+    say("reference vectors: small complex")
+    f = _fixture("ref_small.npz")
+    calc, *out = _gpu_run(cms_cuda, f["binder_xyz"], f["binder_radii"], f["target_xyz"], f["target_radii"])
+    _check_scalars(f, *out)
+    for m in (0, 1):
+        g = calc.run.dots[m]
+        assert len(g) == len(f["dots%d_area" % m])
+        assert np.array_equal(g.atom_idx, f["dots%d_atom" % m])
+        assert np.array_equal(g.buried, f["dots%d_buried" % m])
+        assert np.array_equal(g.type_, f["dots%d_type" % m])
+        assert rel_err(g.coor_xyz, f["dots%d_coor" % m]) <= TOL_DOT
+        assert rel_err(g.outnml_xyz, f["dots%d_nml" % m]) <= TOL_DOT
+        assert rel_err(g.area, f["dots%d_area" % m]) <= TOL_DOT
+
+
+def test_reference_vectors_radius0_maxcms_c1(cms_cuda, golden_complex):
+    say("reference vectors: radius 0, max-CMS, c1")
+    g = golden_complex
+    f = _fixture("ref_radius0.npz")
+    _, *out = _gpu_run(cms_cuda, g["binder_xyz"], f["binder_radii"], g["target_xyz"], g["target_radii"])
+    _check_scalars(f, *out)
+    f = _fixture("ref_maxcms.npz")
+    tot, per, calc = cms_cuda.calculate_maximum_possible_contact_ms(g["binder_xyz"], g["binder_radii"])
+    assert rel_err(tot, f["total"]) <= TOL_OUT and rel_err(per, f["per_atom"]) <= TOL_OUT
+    assert len(calc.run.dots[0]) == int(f["ndots"])
+    f = _fixture("ref_c1.npz")
+    _, *out = _gpu_run(cms_cuda, f["binder_xyz"], f["binder_radii"], f["target_xyz"], f["target_radii"])
+    _check_scalars(f, *out)
