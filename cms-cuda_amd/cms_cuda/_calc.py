@@ -449,10 +449,25 @@ class MolecularSurfaceCalculator:
         mol = np.concatenate([np.zeros(len(self._rad[0]), np.int8), np.ones(len(self._rad[1]), np.int8)])
         arr = AtomArray(xyz, rad, np.concatenate(self._dens), mol)
         if self._mode is not None and len(rad):
+            # the cached surface covers the interface region only; the
+            # reference's per-atom state covers every atom, so it comes from a
+            # full-state generation of the same inputs (identical dots)
             torch = _torch()
             att = torch.empty(len(rad), dtype=torch.int8, device=self._dev)
             acc = torch.empty(len(rad), dtype=torch.int8, device=self._dev)
-            self._ctx("cms_copy_atom_state", _ptr(att), _ptr(acc))
-            arr.atten = att.cpu().numpy()
-            arr.access = acc.cpu().numpy()
+            full = _lib.Context(self._dev.index, torch.cuda.current_stream(self._dev).cuda_stream)
+            try:
+                d_xyz = torch.from_numpy(np.ascontiguousarray(xyz, dtype=np.float64)).to(self._dev)
+                d_rad = torch.from_numpy(np.ascontiguousarray(rad, dtype=np.float64)).to(self._dev)
+                d_den = torch.from_numpy(np.ascontiguousarray(np.concatenate(self._dens), dtype=np.float64)).to(self._dev)
+                nb = (ctypes.c_int32 * 1)(len(self._rad[0]))
+                nt = (ctypes.c_int32 * 1)(len(self._rad[1]))
+                st = self.settings.to_c()
+                full("cms_load", 1, nb, nt, _ptr(d_xyz), _ptr(d_rad), _ptr(d_den), ctypes.byref(st))
+                full("cms_generate", 1 if self._mode else 0, 1)
+                full("cms_copy_atom_state", _ptr(att), _ptr(acc))
+                arr.atten = att.cpu().numpy()
+                arr.access = acc.cpu().numpy()
+            finally:
+                full.close()
         return arr

@@ -88,7 +88,7 @@ struct cms_ctx {
   DBuf bacc, trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
   DBuf frame, bxyz32, axyz32, txyz32, qpack32, rpack32, nresc;
-  DBuf bidx, bgof, adj, eatom, evpos, chk;
+  DBuf bidx, bgof, adj, eatom, evpos, chk, qlast;
   bool cms_ready = false;
   bool float_lists = false;
   // per-run count block and capacities
@@ -263,7 +263,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
                   &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
                   &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
-                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->meta};
+                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->meta};
 }
 
 // Identity of the workspace: a captured graph holds these addresses.
@@ -733,6 +733,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     // the count pass visits only live toroid-queue pairs
     fill_int(ctx->t1cnt.as<int>(), ctx->cap[M_P], 0, st);
     fill_int(ctx->t2cnt.as<int>(), ctx->cap[M_P], 0, st);
+    fill_int8(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0, st);
     TIMED("toroid_count", k_toroid<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
   }
   if (est[M_Q] > 0) {
@@ -1299,11 +1300,51 @@ int cms_copy_dot_desc(cms_ctx* ctx, int c, int mol, int32_t* d1, int32_t* d2) {
   return CMS_OK;
 }
 
+// The reference finishes access with two fancy-index ORs over the toroid
+// queue (cms-cuda/_core.py:1247-1248): access[atom1s] |= has1 and then
+// access[atom2s] |= has2, where has = the pair has a torus point clear of
+// its neighbours.  With repeated indices NumPy keeps the last write, so each
+// atom takes the flag of the last queue pair naming it as atom 1, then of
+// the last naming it as atom 2.  Queue pairs are the toroid-flagged pairs in
+// pair order.
+__global__ void k_queue_last(Count np_, const int* pi, const int* pj, const int8_t* pflag, int* last1,
+                             int* last2) {
+  const int n = np_.get();
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+    if (pflag[p] & 2) {
+      atomicMax(&last1[pi[p]], p);
+      atomicMax(&last2[pj[p]], p);
+    }
+}
+
+__global__ void k_access_final(int n, const int8_t* access, const int* last1, const int* last2,
+                               const int8_t* live, int8_t* out) {
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
+    int8_t v = access[a];
+    if (last1[a] >= 0 && live[last1[a]]) v = 1;
+    if (last2[a] >= 0 && live[last2[a]]) v = 1;
+    out[a] = v;
+  }
+}
+
 int cms_copy_atom_state(cms_ctx* ctx, int8_t* atten, int8_t* access) {
   if (!ctx) return CMS_ERR_INVALID;
   if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_copy_atom_state: no surface");
-  if (atten) CK(hipMemcpyAsync(atten, ctx->atten.p, ctx->ntot, hipMemcpyDeviceToDevice, ctx->stream));
-  if (access) CK(hipMemcpyAsync(access, ctx->access.p, ctx->ntot, hipMemcpyDeviceToDevice, ctx->stream));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->ntot;
+  if (atten) CK(hipMemcpyAsync(atten, ctx->atten.p, n, hipMemcpyDeviceToDevice, st));
+  if (access && n > 0) {
+    CK(ctx->qlast.ensure(sizeof(int) * 2 * ((int64_t)n + 1)));
+    int* last1 = ctx->qlast.as<int>();
+    int* last2 = last1 + n + 1;
+    fill_int(last1, 2 * ((int64_t)n + 1), -1, st);
+    k_queue_last<<<grid_for(std::max<int64_t>(ctx->est[M_P], 1), 256, 4096), 256, 0, st>>>(
+        dev_count(ctx->meta.as<int>() + M_P), ctx->pi.as<int>(), ctx->pj.as<int>(), ctx->pflag.as<int8_t>(), last1,
+        last2);
+    k_access_final<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->access.as<int8_t>(), last1, last2,
+                                                            ctx->tflag.as<int8_t>(), access);
+    CKL();
+  }
   return CMS_OK;
 }
 

@@ -163,6 +163,10 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
+__global__ void k_fill_int8(int8_t* p, int64_t n, int8_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
 __global__ void k_fill_double(double* p, int64_t n, double v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -179,6 +183,12 @@ void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_
   int g = (int)((n + 255) / 256);
   if (g > 4096) g = 4096;
   k_fill_u64<<<g, 256, 0, st>>>(p, n, v);
+}
+void fill_int8(int8_t* p, int64_t n, int8_t v, hipStream_t st) {
+  if (n <= 0) return;
+  int g = (int)((n + 255) / 256);
+  if (g > 4096) g = 4096;
+  k_fill_int8<<<g, 256, 0, st>>>(p, n, v);
 }
 void fill_double(double* p, int64_t n, double v, hipStream_t st) {
   if (n <= 0) return;

@@ -179,10 +179,12 @@ int cms_copy_dots(cms_ctx* ctx, int complex_index, int mol, double* coor, double
 int cms_copy_dot_desc(cms_ctx* ctx, int complex_index, int mol, int32_t* d1, int32_t* d2);
 
 /* Per-atom attention numbers and access flags (device, ntot int8 each) as
-   left by surface generation before the toroidal stage, plus the per-pair
-   toroid-queue flags needed to finish access the reference's way.
-   Counterpart of calc.run.atoms.atten / .access (_core.py:1185-1188,
-   1315, 1396-1400, 1442). */
+   the reference leaves them after surface generation, including its
+   toroidal-stage merge access[atom1s] |= has1; access[atom2s] |= has2 with
+   NumPy's last-write-wins on repeated atoms.  Atoms outside the interface
+   region are only visited by a full_state generation, so for every atom to
+   match, generate with full_state = 1.  Counterpart of calc.run.atoms.atten
+   / .access (cms-cuda/_core.py:1185-1188, 1247-1248, 1315, 1396-1400). */
 int cms_copy_atom_state(cms_ctx* ctx, int8_t* atten, int8_t* access);
 
 /* Probes of complex c (host count; device copies: atoms 3n int32 local,

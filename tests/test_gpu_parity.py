@@ -367,3 +367,27 @@ def test_reference_vectors_radius0_maxcms_c1(cms_cuda, golden_complex):
     f = _fixture("ref_c1.npz")
     _, *out = _gpu_run(cms_cuda, f["binder_xyz"], f["binder_radii"], f["target_xyz"], f["target_radii"])
     _check_scalars(f, *out)
+
+
+def test_atom_state_matches_oracle(cms_cuda, golden_complex, golden_oracle):
+    """calc.run.atoms.atten / .access for every atom, including the
+    reference's last-write-wins toroidal access merge (golden tally: 258
+    atoms with access)."""
+    from oracle.cms_oracle import OracleCalculator
+    say("atom state")
+    g = golden_complex
+    cases = [(g["binder_xyz"], g["binder_radii"], g["target_xyz"], g["target_radii"], golden_oracle["calc"])]
+    f = _fixture("ref_small.npz")
+    o = OracleCalculator()
+    o.add_binder_and_target(f["binder_xyz"], f["binder_radii"], f["target_xyz"], f["target_radii"])
+    o.generate(False)
+    cases.append((f["binder_xyz"], f["binder_radii"], f["target_xyz"], f["target_radii"], o))
+    for k, (bx, br, tx, tr, oc) in enumerate(cases):
+        calc = cms_cuda.MolecularSurfaceCalculator()
+        calc.add_binder_and_target(bx, br, tx, tr)
+        calc.CalcLoaded()
+        at = calc.run.atoms
+        assert np.array_equal(at.atten, oc.atten), k
+        assert np.array_equal(at.access, oc.access), (k, int(at.access.sum()), int(oc.access.sum()))
+        if k == 0:
+            assert int(at.access.sum()) == 258
