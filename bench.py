@@ -52,7 +52,7 @@ WORKLOAD_TEXT = {
 
 KERNELS = ["attention", "region", "neighbors", "pairs_count", "pairs_emit", "probes_count", "probes_emit", "near",
            "convex_count", "convex_emit", "toroid_count", "toroid_emit", "concave_count", "concave_emit",
-           "nearest", "nearest32", "owner_sums", "sc_sides", "cms_adjoint", "dot_grad"]
+           "nearest", "nearest32", "trim", "owner_sums", "sc_sides", "cms_adjoint", "dot_grad"]
 
 
 def log(msg):

@@ -95,22 +95,26 @@ __device__ __forceinline__ void cell_coords(const GridInfo& gi, double x, double
 
 __global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int* __restrict__ acell,
                              int* __restrict__ ccount) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < b.ntot; g += gridDim.x * blockDim.x) {
-    const GridInfo gi = grid[b.acplx[g]];
-    const double4 a = b.atom[g];
-    int ix, iy, iz;
-    cell_coords(gi, a.x, a.y, a.z, &ix, &iy, &iz);
-    const int cell = 2 * (gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz) + b.amol[g];
-    acell[g] = cell;
-    atomicAdd(&ccount[cell], 1);
+  CMS_WAVE_LOOP(g, b.ntot) {
+    int cell = 0;
+    if (g < b.ntot) {
+      const GridInfo gi = grid[b.acplx[g]];
+      const double4 a = b.atom[g];
+      int ix, iy, iz;
+      cell_coords(gi, a.x, a.y, a.z, &ix, &iy, &iz);
+      cell = 2 * (gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz) + b.amol[g];
+      acell[g] = cell;
+    }
+    wave_counter_add(ccount, cell, g < b.ntot);
   }
 }
 
 __global__ void k_cell_scatter(int n, const int* __restrict__ acell, const int* __restrict__ cstart,
                                int* __restrict__ cursor, int* __restrict__ catom) {
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
-    const int cell = acell[g];
-    catom[cstart[cell] + atomicAdd(&cursor[cell], 1)] = g;
+  CMS_WAVE_LOOP(g, n) {
+    const int cell = g < n ? acell[g] : 0;
+    const int slot = wave_counter_add(cursor, cell, g < n);
+    if (g < n) catom[cstart[cell] + slot] = (int)g;
   }
 }
 

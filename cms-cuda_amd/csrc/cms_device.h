@@ -174,6 +174,38 @@ __device__ __forceinline__ double dist_to_line(V3 cen, V3 axis, V3 p) {
                u = u##_x * u##_per + blockIdx.x / u##_nx;                                       \
        u < u##_hi; u += u##_step)
 
+// Wave-aggregated counter increment: every active lane adds 1 to
+// counters[key]; lanes sharing a key are served by one atomic, in lane order.
+// Returns the counter's value before this lane's increment.  Must be called
+// by all lanes of the wave together (inactive lanes pass active = false).
+__device__ __forceinline__ int wave_counter_add(int* counters, int key, bool active) {
+  int result = 0;
+  bool todo = active;
+  for (;;) {
+    const uint64_t m = ballot(todo);
+    if (!m) break;
+    const int leader = __builtin_ctzll(m);
+    const int lk = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = todo && key == lk;
+    const uint64_t sm = ballot(mine);
+    int base = 0;
+    if (lane() == leader) base = atomicAdd(&counters[lk], __popcll(sm));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (mine) {
+      result = base + lanes_below(sm);
+      todo = false;
+    }
+  }
+  return result;
+}
+
+// Grid-stride loop in which every lane of a wave takes part in every
+// iteration (i may be >= n: callers test it), for wave-level primitives.
+#define CMS_WAVE_LOOP(i, n)                                                                   \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x,                           \
+               i##_end = (((int64_t)(n) + WAVE - 1) / WAVE) * WAVE;                         \
+       i - lane() < i##_end; i += (int64_t)gridDim.x * blockDim.x)
+
 // Width of the unit blocks of a sparse loop: 64 when there are enough units
 // to give every workgroup several blocks, down to 1 for small launches.
 __device__ __forceinline__ int64_t sparse_width(int64_t n, int64_t grid) {

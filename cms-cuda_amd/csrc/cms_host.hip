@@ -85,10 +85,11 @@ struct cms_ctx {
   int nB = 0, nA = 0;
   DBuf bflag, bscan, blist, bxyz, bgrp, aflag, ascan, alist, axyz, agrpd;
   DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm;
-  DBuf bacc, trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
+  DBuf trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
-  DBuf frame, bxyz32, axyz32, txyz32, qpack32, rpack32, nresc;
+  DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;
   DBuf bidx, bgof, adj, eatom, evpos, chk, qlast;
+  DBuf tgrid, tncell, tncscan, tacell, tcount, tstart, tcur, tsorted;   // trim grid
   bool cms_ready = false;
   bool float_lists = false;
   // per-run count block and capacities
@@ -259,11 +260,12 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
                   &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
                   &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
-                  &ctx->bacc, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
+                  &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
                   &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
-                  &ctx->bxyz32, &ctx->axyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
-                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->meta};
+                  &ctx->bxyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
+                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->tgrid, &ctx->tncell, &ctx->tncscan, &ctx->tacell,
+          &ctx->tcount, &ctx->tstart, &ctx->tcur, &ctx->tsorted, &ctx->meta};
 }
 
 // Identity of the workspace: a captured graph holds these addresses.
@@ -493,13 +495,20 @@ static int size_dots(cms_ctx* ctx, int64_t c) {
   for (DBuf* b : d4) CK(b->ensure(sizeof(double4) * (c + 1)));
   DBuf* i4[] = {&ctx->ddesc, &ctx->ddesc2};
   for (DBuf* b : i4) CK(b->ensure(sizeof(int4) * (c + 1)));
-  DBuf* f4[] = {&ctx->bxyz32, &ctx->axyz32, &ctx->txyz32};
+  DBuf* f4[] = {&ctx->bxyz32, &ctx->txyz32};
   for (DBuf* b : f4) CK(b->ensure(sizeof(float4) * (c + 1)));
   DBuf* i1[] = {&ctx->datom, &ctx->bflag, &ctx->bscan, &ctx->aflag, &ctx->ascan, &ctx->blist, &ctx->alist,
                 &ctx->bidx, &ctx->operm, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->tidx, &ctx->tgof,
                 &ctx->bgof};
   for (DBuf* b : i1) CK(b->ensure(sizeof(int) * (c + 1)));
-  DBuf* f8[] = {&ctx->bmin, &ctx->wv, &ctx->bacc, &ctx->tmin, &ctx->tdist, &ctx->trr};
+  DBuf* f8[] = {&ctx->bmin, &ctx->wv, &ctx->tmin, &ctx->tdist, &ctx->trr};
+  CK(ctx->tacell.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->tsorted.ensure(sizeof(double4) * (c + 1)));
+  // trim-grid cells: at most 2 per accessible dot + 64 per group
+  const int64_t cells = 2 * c + 64ll * 2 * ctx->B + 1;
+  CK(ctx->tcount.ensure(sizeof(int) * (cells + 1)));
+  CK(ctx->tstart.ensure(sizeof(int) * (cells + 1)));
+  CK(ctx->tcur.ensure(sizeof(int) * (cells + 1)));
   for (DBuf* b : f8) CK(b->ensure(sizeof(double) * (c + 1)));
   CK(ctx->dbur.ensure(c + 1));
   CK(ctx->dtype.ensure(c + 1));
@@ -934,9 +943,6 @@ static int prepare_float_lists(cms_ctx* ctx) {
     k_to_float<<<grid_for(ctx->est[M_B] + 1, 256, 4096), 256, 0, st>>>(
         dev_count(meta + M_B), ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, 1, ctx->frame.as<double4>(),
         ctx->bxyz32.as<float4>());
-    k_to_float<<<grid_for(ctx->est[M_A] + 1, 256, 4096), 256, 0, st>>>(
-        dev_count(meta + M_A), ctx->axyz.as<double4>(), ctx->agrpd.as<int>(), G, 1, ctx->frame.as<double4>(),
-        ctx->axyz32.as<float4>());
   }
   CKL();
   ctx->float_lists = true;
@@ -1033,13 +1039,47 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   const int64_t capD = ctx->cap[M_D], estB = ctx->est[M_B], estA = ctx->est[M_A], estT = ctx->est[M_T];
   CK(ctx->tgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->side.ensure(sizeof(double) * 8 * G));
-  // trim: buried dots of g against accessible dots of g
-  r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), cB,
-                     capD, estB, estA, ctx->axyz.as<double4>(), ctx->axyz32.as<float4>(), ctx->agrpd.as<int>(), 0,
-                     G, ctx->bacc.as<double>(), nullptr);
-  if (r) return r;
+  // trim: buried dots of g against a grid of the accessible dots of g
+  CK(ctx->tgrid.ensure(sizeof(GridInfo) * (G + 1)));
+  CK(ctx->tncell.ensure(sizeof(int) * (G + 1)));
+  CK(ctx->tncscan.ensure(sizeof(int) * (G + 1)));
+  const Count cA = dev_count(meta + M_A);
+  const int64_t cells_cap = 2 * capD + 64ll * G;
+  const int gA = grid_for(std::max<int64_t>(estA, 1), 256, 4096);
   const int gB = grid_for(std::max<int64_t>(estB, 1), 256, 4096);
-  k_trim_flag<<<gB, 256, 0, st>>>(cB, ctx->bacc.as<double>(), ctx->s.band * ctx->s.band, ctx->trflag.as<int>());
+  GridInfo* tg = ctx->tgrid.as<GridInfo>();
+  k_dot_grid_setup<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->agrpd.as<int>(), ctx->axyz.as<double4>(), ctx->s.band,
+                                                          tg, ctx->tncell.as<int>());
+  scan_excl(ctx->tncell.as<int>(), ctx->tncscan.as<int>(), G, part, st);
+  k_grid_offsets<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, ctx->tncscan.as<int>(), tg);
+  const Count cCells = dev_count(ctx->tncscan.as<int>() + G);
+  fill_int(ctx->tcount.as<int>(), cCells, cells_cap, 0, st);
+  fill_int(ctx->tcur.as<int>(), cCells, cells_cap, 0, st);
+  k_dot_cell_count<<<gA, 256, 0, st>>>(cA, ctx->axyz.as<double4>(), ctx->agrpd.as<int>(), G, tg,
+                                       ctx->tacell.as<int>(), ctx->tcount.as<int>());
+  scan_excl(ctx->tcount.as<int>(), ctx->tstart.as<int>(), cCells, cells_cap, part, st);
+  k_dot_cell_scatter<<<gA, 256, 0, st>>>(cA, ctx->tacell.as<int>(), ctx->tstart.as<int>(), ctx->tcur.as<int>(),
+                                         ctx->axyz.as<double4>(), ctx->tsorted.as<double4>());
+  {
+    // lanes per buried dot: enough threads to fill the GPU for small batches
+    const int64_t nb = std::max<int64_t>(estB, 1);
+    const int L = nb >= (1 << 18) ? 1 : nb >= (1 << 14) ? 8 : 32;
+    const int gL = grid_for(nb * L, 256, 8192);
+    const double band2 = ctx->s.band * ctx->s.band;
+    const int* cs = ctx->tstart.as<int>();
+    const double4* srt = ctx->tsorted.as<double4>();
+    int* fl = ctx->trflag.as<int>();
+    if (L == 1)
+      TIMED("trim", (k_trim_grid<1><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs, srt,
+                                                        band2, fl)));
+    else if (L == 8)
+      TIMED("trim", (k_trim_grid<8><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs, srt,
+                                                        band2, fl)));
+    else
+      TIMED("trim", (k_trim_grid<32><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs,
+                                                         srt, band2, fl)));
+  }
+  CKL();
   scan_excl(ctx->trflag.as<int>(), ctx->trscan.as<int>(), cB, capD, part, st);
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->bgrp.as<int>(), 1, ctx->trscan.as<int>(),
                                                              ctx->tgrp.as<int>());

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cms_cells.h"
 #include "cms_internal.h"
 
 namespace cms {
@@ -39,6 +40,15 @@ __global__ void k_nearest32(const float4* Q32, const double4* Q64, const int* qg
                             const double4* R64, const int* rgrp, int xmask, const int* toff, int G, int nsplit,
                             int64_t pstride, const double4* frame, double* out_d2, int* out_idx,
                             unsigned long long* nrescored);
+__global__ void k_dot_grid_setup(int G, const int* agrp, const double4* axyz, double band, GridInfo* grid,
+                                 int* ncell);
+__global__ void k_dot_cell_count(Count na, const double4* axyz, const int* agrp, int G, const GridInfo* grid,
+                                 int* acell, int* ccount);
+__global__ void k_dot_cell_scatter(Count na, const int* acell, const int* cstart, int* cursor,
+                                   const double4* axyz, double4* sorted);
+template <int L>
+__global__ void k_trim_grid(Count nb, const double4* bxyz, const int* bgrp, int G, const GridInfo* grid,
+                            const int* cstart, const double4* sorted, double band2, int* flag);
 __global__ void k_complex_frame(int B, const int* aoff, const double4* atom, double rp, double4* frame);
 __global__ void k_to_float(Count n, const double4* in, const int* grp, int G, int percomplex,
                            const double4* frame, float4* out);

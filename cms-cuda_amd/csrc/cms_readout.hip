@@ -132,16 +132,19 @@ __global__ void k_cms_weights(Count cnt, const double4* __restrict__ bxyz, const
 __global__ void k_owner_count(Count items, const int* __restrict__ list, const int* __restrict__ datom,
                               int* __restrict__ cnt) {
   const int n = items.get();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    atomicAdd(&cnt[datom[list ? list[i] : i]], 1);
+  CMS_WAVE_LOOP(i, n) {
+    const int a = i < n ? datom[list ? list[i] : i] : 0;
+    wave_counter_add(cnt, a, i < n);
+  }
 }
 __global__ void k_owner_scatter(Count cnt, const int* __restrict__ list, const int* __restrict__ datom,
                                 const int* __restrict__ start, int* __restrict__ cur,
                                 int* __restrict__ perm) {
   const int n = cnt.get();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int a = datom[list ? list[i] : i];
-    perm[start[a] + atomicAdd(&cur[a], 1)] = i;
+  CMS_WAVE_LOOP(i, n) {
+    const int a = i < n ? datom[list ? list[i] : i] : 0;
+    const int slot = wave_counter_add(cur, a, i < n);
+    if (i < n) perm[start[a] + slot] = (int)i;
   }
 }
 // One wavefront per atom: the atom's item indices (scattered in arbitrary
@@ -535,5 +538,193 @@ template __global__ void k_nearest64<false>(const double4*, const int*, const do
                                             const int*, int, int, int64_t, double*, int*);
 template __global__ void k_nearest64<true>(const double4*, const int*, const double4*, const int*, int,
                                            const int*, int, int, int64_t, double*, int*);
+
+}  // namespace cms
+
+namespace cms {
+
+// ------------------------------------------------------------ SC trim
+// _core.py:1037-1077 keeps a buried dot when its squared distance to every
+// accessible dot of the same molecule exceeds band^2 (the minimum of the
+// cdist row against band^2).  Only that boolean is needed, so the search is
+// a fixed-radius test on a per-group grid of the accessible dots with cell
+// edge >= band: every accessible dot within band of a buried dot lies in the
+// 27 cells around it, and d^2 = (dx^2 + dy^2) + dz^2 is the cdist value.
+
+// Per group: bounding box of its accessible dots, cell edge (>= band, grown
+// until the group has at most 2 n + 64 cells) and dims.
+__global__ __launch_bounds__(256) void k_dot_grid_setup(int G, const int* __restrict__ agrp,
+                                                        const double4* __restrict__ axyz, double band,
+                                                        GridInfo* __restrict__ grid, int* __restrict__ ncell) {
+  __shared__ double red[6][256];
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = agrp[g] + threadIdx.x; i < agrp[g + 1]; i += blockDim.x) {
+      const double4 a = axyz[i];
+      mn[0] = fmin(mn[0], a.x); mn[1] = fmin(mn[1], a.y); mn[2] = fmin(mn[2], a.z);
+      mx[0] = fmax(mx[0], a.x); mx[1] = fmax(mx[1], a.y); mx[2] = fmax(mx[2], a.z);
+    }
+    for (int k = 0; k < 3; ++k) { red[k][threadIdx.x] = mn[k]; red[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+        for (int k = 3; k < 6; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + o]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const int n = agrp[g + 1] - agrp[g];
+      double lo[3] = {red[0][0], red[1][0], red[2][0]};
+      double hi[3] = {red[3][0], red[4][0], red[5][0]};
+      if (n == 0 || !(hi[0] >= lo[0]) || !(hi[1] >= lo[1]) || !(hi[2] >= lo[2])) {
+        lo[0] = lo[1] = lo[2] = 0.0;
+        hi[0] = hi[1] = hi[2] = 0.0;
+      }
+      double h = band * (1.0 + 1e-9) + 1e-9;
+      if (!(h > 0.0)) h = 1.0;
+      int d[3];
+      const int64_t cap = 2ll * n + 64;
+      for (int it = 0; it < 64; ++it) {
+        int64_t tot = 1;
+        for (int k = 0; k < 3; ++k) {
+          const double e = (hi[k] - lo[k]) / h;
+          d[k] = e < 1e6 ? (int)floor(e) + 1 : 1000001;
+          tot *= d[k];
+        }
+        if (tot <= cap) break;
+        h *= 1.25;
+      }
+      GridInfo gi;
+      gi.x0 = lo[0]; gi.y0 = lo[1]; gi.z0 = lo[2]; gi.h = h;
+      gi.nx = d[0]; gi.ny = d[1]; gi.nz = d[2];
+      gi.cell0 = 0;
+      grid[g] = gi;
+      ncell[g] = d[0] * d[1] * d[2];
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int group_of_item(const int* __restrict__ grp, int G, int i) {
+  int lo = 0, hi = G;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (grp[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+// The same for the items of a wave: the group of the wave's first item
+// (first, wave-uniform) is tried before a search of the lane's own.
+__device__ __forceinline__ int group_of_item_wave(const int* __restrict__ grp, int G, int i, int first) {
+  const int g0 = group_of_item(grp, G, __builtin_amdgcn_readfirstlane(first));
+  const int lo = __builtin_amdgcn_readfirstlane(grp[g0]), hi = __builtin_amdgcn_readfirstlane(grp[g0 + 1]);
+  return (i >= lo && i < hi) ? g0 : group_of_item(grp, G, i);
+}
+
+__device__ __forceinline__ void dot_cell(const GridInfo& gi, double x, double y, double z, int* ix, int* iy,
+                                         int* iz) {
+  *ix = min(max((int)floor((x - gi.x0) / gi.h), 0), gi.nx - 1);
+  *iy = min(max((int)floor((y - gi.y0) / gi.h), 0), gi.ny - 1);
+  *iz = min(max((int)floor((z - gi.z0) / gi.h), 0), gi.nz - 1);
+}
+
+__global__ void k_dot_cell_count(Count na, const double4* __restrict__ axyz, const int* __restrict__ agrp, int G,
+                                 const GridInfo* __restrict__ grid, int* __restrict__ acell,
+                                 int* __restrict__ ccount) {
+  const int n = na.get();
+  CMS_WAVE_LOOP(i, n) {
+    int cell = 0;
+    if (i < n) {
+      const GridInfo gi = grid[group_of_item_wave(agrp, G, (int)i, (int)(i - lane()))];
+      const double4 a = axyz[i];
+      int ix, iy, iz;
+      dot_cell(gi, a.x, a.y, a.z, &ix, &iy, &iz);
+      cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+      acell[i] = cell;
+    }
+    wave_counter_add(ccount, cell, i < n);
+  }
+}
+
+__global__ void k_dot_cell_scatter(Count na, const int* __restrict__ acell, const int* __restrict__ cstart,
+                                   int* __restrict__ cursor, const double4* __restrict__ axyz,
+                                   double4* __restrict__ sorted) {
+  const int n = na.get();
+  CMS_WAVE_LOOP(i, n) {
+    const int cell = i < n ? acell[i] : 0;
+    const int slot = wave_counter_add(cursor, cell, i < n);
+    if (i < n) sorted[cstart[cell] + slot] = axyz[i];
+  }
+}
+
+// Trim flag of every buried dot: 1 = kept (no accessible dot of its group
+// within band), 0 = trimmed.  A group without accessible dots keeps all.
+// L consecutive lanes test one buried dot: the dot is trimmed when an
+// accessible dot of its group lies within the band.  The cell edge is at
+// least the band, so the 27 cells around the dot's cell hold every such dot.
+template <int L>
+__global__ __launch_bounds__(256) void k_trim_grid(Count nb, const double4* __restrict__ bxyz,
+                                                   const int* __restrict__ bgrp, int G,
+                                                   const GridInfo* __restrict__ grid, const int* __restrict__ cstart,
+                                                   const double4* __restrict__ sorted, double band2,
+                                                   int* __restrict__ flag) {
+  static_assert(WAVE % L == 0, "lanes per dot must divide the wave");
+  const int n = nb.get();
+  const int sub = lane() % L;
+  const uint64_t submask = L == WAVE ? ~0ull : (((1ull << L) - 1) << (lane() - sub));
+  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+  // the trip count is uniform across a wave so that the group lookup may use
+  // wave-level operations; lanes past the end idle
+  const int64_t nq_pad = (((int64_t)n * L + WAVE - 1) / WAVE) * WAVE;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t - lane() < nq_pad; t += nthreads) {
+    const int i = (int)(t / L);
+    const int first = (int)((t - lane()) / L);
+    if (i >= n) continue;
+    const GridInfo gi = grid[group_of_item_wave(bgrp, G, i, first)];
+    const double4 q = bxyz[i];
+    int ix, iy, iz;
+    dot_cell(gi, q.x, q.y, q.z, &ix, &iy, &iz);
+    bool hit = false;
+    for (int dx = -1; dx <= 1 && !hit; ++dx) {
+      const int cx = ix + dx;
+      if (cx < 0 || cx >= gi.nx) continue;
+      for (int dy = -1; dy <= 1 && !hit; ++dy) {
+        const int cy = iy + dy;
+        if (cy < 0 || cy >= gi.ny) continue;
+        // the three cells along z are contiguous in the cell array
+        const int zlo = max(iz - 1, 0), zhi = min(iz + 1, gi.nz - 1);
+        const int c0 = gi.cell0 + (cx * gi.ny + cy) * gi.nz;
+        const int lo = cstart[c0 + zlo], hi = cstart[c0 + zhi + 1];
+        if constexpr (L == 1) {
+          for (int u = lo; u < hi; ++u) {
+            const double4 a = sorted[u];
+            const double ddx = q.x - a.x, ddy = q.y - a.y, ddz = q.z - a.z;
+            if ((ddx * ddx + ddy * ddy) + ddz * ddz <= band2) { hit = true; break; }
+          }
+        } else {
+          // uniform trip count over the L lanes of the dot
+          for (int u0 = lo; u0 < hi; u0 += L) {
+            const int u = u0 + sub;
+            bool h = false;
+            if (u < hi) {
+              const double4 a = sorted[u];
+              const double ddx = q.x - a.x, ddy = q.y - a.y, ddz = q.z - a.z;
+              h = (ddx * ddx + ddy * ddy) + ddz * ddz <= band2;
+            }
+            if (ballot(h) & submask) { hit = true; break; }
+          }
+        }
+      }
+    }
+    if (sub == 0) flag[i] = hit ? 0 : 1;
+  }
+}
+template __global__ void k_trim_grid<1>(Count, const double4*, const int*, int, const GridInfo*, const int*,
+                                        const double4*, double, int*);
+template __global__ void k_trim_grid<8>(Count, const double4*, const int*, int, const GridInfo*, const int*,
+                                        const double4*, double, int*);
+template __global__ void k_trim_grid<32>(Count, const double4*, const int*, int, const GridInfo*, const int*,
+                                         const double4*, double, int*);
 
 }  // namespace cms

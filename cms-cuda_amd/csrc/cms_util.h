@@ -17,6 +17,8 @@ __global__ void k_copy_word(const int* src, Count at, int* dst);
 void fill_int(int* p, int64_t n, int v, hipStream_t st);
 void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t st);
 void fill_int8(int8_t* p, int64_t n, int8_t v, hipStream_t st);
+// fill of a device-counted range; cap bounds the count and sizes the grid
+void fill_int(int* p, Count n, int64_t cap, int v, hipStream_t st);
 void fill_double(double* p, int64_t n, double v, hipStream_t st);
 
 __global__ void k_pack_atoms(int ntot, const double* xyz, const double* rad, const double* dens_in,

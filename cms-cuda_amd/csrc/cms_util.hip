@@ -184,6 +184,17 @@ void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_
   if (g > 4096) g = 4096;
   k_fill_u64<<<g, 256, 0, st>>>(p, n, v);
 }
+__global__ void k_fill_int_count(int* p, Count cnt, int v) {
+  const int64_t n = cnt.get();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+void fill_int(int* p, Count n, int64_t cap, int v, hipStream_t st) {
+  if (cap <= 0) return;
+  int g = (int)((cap + 255) / 256);
+  if (g > 4096) g = 4096;
+  k_fill_int_count<<<g, 256, 0, st>>>(p, n, v);
+}
 void fill_int8(int8_t* p, int64_t n, int8_t v, hipStream_t st) {
   if (n <= 0) return;
   int g = (int)((n + 255) / 256);
