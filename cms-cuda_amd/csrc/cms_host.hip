@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -104,6 +105,10 @@ struct cms_ctx {
   uint64_t layout_epoch = 0;        // bumped when the batch layout changes
   // one captured cms_run step
   bool use_graph = false;
+  // probes count pass: one wavefront per pair suits small batches (more
+  // units than rows), one per neighbour row large ones (full lanes); the
+  // environment variable CMS_PROBE_COUNT = "pairs" / "rows" fixes the choice
+  int probe_count_mode = 0;   // 0 by size, 1 pairs, 2 rows
   bool capturing = false;
   hipStream_t gstream = nullptr;    // stream the graph is captured on and launched to
   hipEvent_t gev0 = nullptr, gev1 = nullptr;
@@ -313,6 +318,10 @@ int cms_create(int device, void* stream, cms_ctx** out) {
   }
   cms_ctx* ctx = new cms_ctx();
   ctx->device = device;
+  if (const char* pm = std::getenv("CMS_PROBE_COUNT")) {
+    if (!std::strcmp(pm, "pairs")) ctx->probe_count_mode = 1;
+    if (!std::strcmp(pm, "rows")) ctx->probe_count_mode = 2;
+  }
   ctx->stream = static_cast<hipStream_t>(stream);
   if (ctx->nresc.ensure(sizeof(unsigned long long) * 2) != hipSuccess ||
       ctx->meta.ensure(sizeof(int) * M_WORDS) != hipSuccess) {
@@ -445,6 +454,9 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
 // estimates, so a learned run can be captured in a HIP graph.
 
 static int64_t headroom(int64_t v) { return v + v / 4 + 64; }
+
+// neighbour rows from which the probes count pass runs one wavefront per row
+constexpr int64_t PROBE_ROWS_MIN = 4096;
 
 static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->nn.ensure(sizeof(int) * (c + 1)));
@@ -717,7 +729,13 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     w = work(ctx);
   }
   if (est[M_R] > 0) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
-  if (est[M_P] > 0) TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_P] > 0) {
+    const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
+    if (by_row)
+      TIMED("probes_count", k_probe_rows<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+    else
+      TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  }
   CKL();
   scan_excl(ctx->prcnt.as<int>(), ctx->prscan.as<int>(), dev_count(meta + M_P, 2), 2 * ctx->cap[M_P], part, st);
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->pgrp.as<int>(), 2,

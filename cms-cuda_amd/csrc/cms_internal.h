@@ -115,6 +115,7 @@ struct Work {
 
 template <bool EMIT> __global__ void k_pairs(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
+__global__ void k_probe_rows(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_toroid(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);

@@ -162,6 +162,46 @@ __device__ __forceinline__ bool circle_in_one_sphere(const int* lj, const double
   return wave_any(hit);
 }
 
+// Probe positions of the non-degenerate triple (i, j, k) (_core.py:1545-1575):
+// base point on the axis of the pair's torus circle, height h above the
+// plane of the three centres, probes bijk +- h uijk.  False when h^2 <= 0.
+// Both probe passes evaluate the triple through this function, so their
+// collision outcomes agree bit for bit.
+__device__ __forceinline__ bool probe_points(V3 xi, double eri, V3 xk, double erk, V3 uij, V3 tij, V3* pp, V3* pm,
+                                             V3* uijk_out, double* h_out) {
+  const double dik = norm_seq(xk - xi);
+  const V3 uik = (xk - xi) / dik;
+  const double dt = dot_alt(uij, uik);
+  const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+  const double sw = sin(wk);
+  const V3 uijk = cross(uij, uik) / sw;
+  const V3 utb = cross(uijk, uij);
+  const double asymm = (eri * eri - erk * erk) / dik;
+  const V3 tik = (xi + xk) * 0.5 + uik * (asymm * 0.5);
+  const double dtb = dot_alt(uik, tik - tij);
+  const V3 bijk = tij + utb * (dtb / sw);
+  const V3 bi_ = bijk - xi;
+  const double hsq = eri * eri - dot_alt(bi_, bi_);
+  *uijk_out = uijk;
+  if (!(hsq > 0.0)) return false;
+  const double h = sqrt(hsq);
+  *pp = bijk + uijk * h;
+  *pm = bijk + uijk * (-h);
+  *h_out = h;
+  return true;
+}
+
+// Degenerate test of a valid triple (_core.py:1505-1543): dt >= 1, dt <= -1,
+// w <= 0 or sin w <= 0 with w = arccos(clip(dt)).  For |dt| < 1 - 1e-6, w
+// lies in (1.4e-3, pi - 1.4e-3), so none of these holds and arccos / sin are
+// evaluated only near |dt| = 1.
+__device__ __forceinline__ bool triple_degenerate(double dt) {
+  if (fabs(dt) < 1.0 - 1e-6) return false;
+  const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+  const double sw = sin(wk);
+  return (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
@@ -200,10 +240,6 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
 
     // pass 1: valid, non-degenerate triples of every 64-slot chunk, and
     // whether a degenerate triple kills the pair (_core.py:1505-1543).
-    // Degenerate means dt >= 1, dt <= -1, w <= 0 or sin w <= 0 with
-    // w = arccos(clip(dt)); for |dt| < 1 - 1e-6, w lies in
-    // (1.4e-3, pi - 1.4e-3), so none of these holds and arccos / sin are
-    // evaluated only near |dt| = 1.
     uint64_t vmask[NCH];
     bool killed = false, near_deg = false;
     int ncand = 0;
@@ -220,12 +256,7 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
           if (probe_triple_valid(lj[sl], j, xk, erk, xi, xj, eri, erj, bij, w.atten, &dik)) {
             const V3 uik = (xk - xi) / dik;
             const double dt = dot_alt(uij, uik);
-            bool deg = false;
-            if (!(fabs(dt) < 1.0 - 1e-6)) {
-              const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
-              const double sw = sin(wk);
-              deg = (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
-            }
+            const bool deg = triple_degenerate(dt);
             if (deg) {
               const V3 dk = tij - xk;
               kill = dot_seq(dk, dk) < erk * erk - rij * rij;
@@ -276,27 +307,7 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
         double h = 0.0;
         if (mine) {
           k = lj[sl];
-          const V3 xk = ld3(lx[sl]);
-          const double erk = lx[sl].w + s.rp;
-          const double dik = norm_seq(xk - xi);
-          const V3 uik = (xk - xi) / dik;
-          const double dt = dot_alt(uij, uik);
-          const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
-          const double sw = sin(wk);
-          uijk = cross(uij, uik) / sw;
-          const V3 utb = cross(uijk, uij);
-          const double asymm = (eri * eri - erk * erk) / dik;
-          const V3 tik = (xi + xk) * 0.5 + uik * (asymm * 0.5);
-          const double dtb = dot_alt(uik, tik - tij);
-          const V3 bijk = tij + utb * (dtb / sw);
-          const V3 bi_ = bijk - xi;
-          const double hsq = eri * eri - dot_alt(bi_, bi_);
-          if (hsq > 0.0) {
-            h = sqrt(hsq);
-            pp = bijk + uijk * h;
-            pm = bijk + uijk * (-h);
-            okp = okm = true;
-          }
+          okp = okm = probe_points(xi, eri, ld3(lx[sl]), lx[sl].w + s.rp, uij, tij, &pp, &pm, &uijk, &h);
         }
         uint64_t cm = ballot(okp);
         while (cm) {
@@ -347,6 +358,233 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
       w.prcnt[vp] = np_plus;
       w.prcnt[vm] = np_minus;
       if (np_plus + np_minus > 0) w.pflag[p] = (int8_t)(w.pflag[p] | 8);
+    }
+    __syncthreads();
+  }
+}
+
+// Count pass of the probes stage, one wavefront per neighbour row: the
+// row's forward pairs (i, j) are taken in batches, and each step spreads its
+// work over the lanes across pair boundaries, so that lanes are not left
+// idle by the few valid triples of a single pair:
+//   1. triangle tests of every (pair, slot) item; valid triples are queued;
+//   2. degenerate / kill tests of the queued triples, compacting the
+//      non-degenerate ones (candidates) in place;
+//   3. the single-sphere containment test of the pairs that have candidates
+//      or belong to the toroid queue;
+//   4. probe positions, one candidate per lane, and their collision with the
+//      row's atoms other than j and k (slots in row order, until every lane's
+//      probes have collided).
+// Per pair it writes what the pair-per-wavefront emit pass relies on: the
+// probe counts of both signs, pflag bits 4 (circle inside one neighbour
+// sphere) and 8 (probes), and the access marks of the atoms of every
+// surviving probe (_core.py:1445-1643).
+constexpr int PR_PAIRS = 32;    // pairs per batch (at most one per lane in step 3 / 5)
+constexpr int PR_QCAP = 768;    // (pair, slot) items per batch, bounds the triple queue
+
+// pair-batch flag bits
+constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32;
+
+__global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work w) {
+  __shared__ int lj[KCAP];
+  __shared__ double4 lx[KCAP];
+  __shared__ int8_t latt[KCAP];
+  __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
+  __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
+  __shared__ int queue[PR_QCAP];
+  static_assert(KCAP <= 256 && PR_PAIRS <= WAVE, "queue entries pack (pair << 8) | slot");
+  const int nR = *w.nR;
+  const int nP = *w.nP;
+  auto live = [&](int64_t q) { return w.pcnt[q] != 0; };
+  CMS_SPARSE_LOOP(r, nR, live) {
+    const int i = w.ratom[r];
+    const int nn = w.nn[r];
+    const int p_lo = w.pscan[r];
+    const int p_hi = min(p_lo + w.pcnt[r], nP);
+    if (p_lo >= p_hi || nn <= 0) continue;
+    {
+      const int* row = w.nbr + (int64_t)r * KCAP;
+      for (int sl = lane(); sl < nn; sl += WAVE) {
+        const int k = row[sl];
+        lj[sl] = k;
+        lx[sl] = b.atom[k];
+        latt[sl] = w.atten[k];
+      }
+    }
+    const double4 ai = b.atom[i];
+    const V3 xi = ld3(ai);
+    const double eri = ai.w + s.rp;
+    const int grp = group_of_atom(b, i);
+    const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
+    const bool iblock = w.atten[i] <= ATTEN_BLOCKER;
+    const int PB = max(1, min(PR_PAIRS, PR_QCAP / nn));
+    for (int pb = p_lo; pb < p_hi; pb += PB) {
+      const int nb = min(PB, p_hi - pb);
+      __syncthreads();   // row staged / previous batch consumed
+      if (lane() < nb) {
+        const int p = pb + lane();
+        const int j = w.pj[p];
+        bj[lane()] = j;
+        bu[lane()] = w.pu[p];
+        bt[lane()] = w.pt[p];
+        bx[lane()] = b.atom[j];
+        const bool bij = iblock && w.atten[j] <= ATTEN_BLOCKER;
+        bflag[lane()] = ((w.pflag[p] & 2) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0);
+        bnp[lane()] = 0;
+        bnm[lane()] = 0;
+      }
+      __syncthreads();
+
+      // 1. triangle tests (_core.py:1478-1500)
+      const int items = nb * nn;
+      int nq = 0;
+      {
+        int a = lane() / nn, sl = lane() - (lane() / nn) * nn;
+        for (int it0 = 0; it0 < items; it0 += WAVE) {
+          bool valid = false;
+          if (it0 + lane() < items) {
+            const int k = lj[sl];
+            if (k > bj[a]) {
+              const double4 ak = lx[sl], aj = bx[a];
+              const V3 xk = ld3(ak);
+              const double erk = ak.w + s.rp, erj = aj.w + s.rp;
+              valid = norm_lt(xk - ld3(aj), erj + erk) && norm_lt(xk - xi, eri + erk) &&
+                      !((bflag[a] & PB_BLOCKERS) && latt[sl] <= ATTEN_BLOCKER);
+            }
+          }
+          const uint64_t m = ballot(valid);
+          if (valid) queue[nq + lanes_below(m)] = (a << 8) | sl;
+          nq += __popcll(m);
+          sl += WAVE;
+          while (sl >= nn) { sl -= nn; ++a; }
+        }
+      }
+      __syncthreads();
+
+      // 2. degenerate triples kill the pair when the torus centre lies inside
+      // k's sphere shrunk to the circle (_core.py:1505-1543); candidates are
+      // compacted in place (each lane writes at or below the entry it read)
+      int nc = 0;
+      for (int q0 = 0; q0 < nq; q0 += WAVE) {
+        const int q = q0 + lane();
+        bool cand = false;
+        int e = 0;
+        if (q < nq) {
+          e = queue[q];
+          const int a = e >> 8, sl = e & 255;
+          const double4 ak = lx[sl], pu = bu[a];
+          const V3 xk = ld3(ak);
+          const double erk = ak.w + s.rp;
+          const double dik = norm_seq(xk - xi);
+          const V3 uik = (xk - xi) / dik;
+          const double dt = dot_alt(ld3(pu), uik);
+          if (triple_degenerate(dt)) {
+            const V3 dk = ld3(bt[a]) - xk;
+            if (dot_seq(dk, dk) < erk * erk - pu.w * pu.w) atomicOr(&bflag[a], PB_KILLED);
+          } else {
+            cand = true;
+            atomicOr(&bflag[a], (fabs(dt) < 1.0 - 1e-10) ? PB_CAND : (PB_CAND | PB_NEARDEG));
+          }
+        }
+        const uint64_t m = ballot(cand);
+        if (cand) queue[nc + lanes_below(m)] = e;
+        nc += __popcll(m);
+      }
+      __syncthreads();
+
+      // 3. pairs whose torus circle lies inside one neighbour's sphere have
+      // neither probes nor torus points (see circle_in_one_sphere)
+      {
+        bool need = false;
+        if (lane() < nb) {
+          const int f = bflag[lane()];
+          need = !(f & PB_NEARDEG) && ((f & PB_TOR) || ((f & PB_CAND) && !(f & PB_KILLED)));
+        }
+        const uint64_t m = ballot(need);
+        if (need) cpair[lanes_below(m)] = lane();
+        const int citems = __popcll(m) * nn;
+        __syncthreads();
+        int c = lane() / nn, sl = lane() - (lane() / nn) * nn;
+        for (int it0 = 0; it0 < citems; it0 += WAVE) {
+          if (it0 + lane() < citems) {
+            const int a = cpair[c];
+            if (lj[sl] != bj[a]) {
+              const double4 ax = lx[sl], pu = bu[a];
+              const V3 v = ld3(ax) - ld3(bt[a]);
+              const double aa = dot_seq(v, ld3(pu));
+              const double d = sqrt(fmax(dot_seq(v, v) - aa * aa, 0.0));
+              const double R = (ax.w + s.rp) - COVER_MARGIN;
+              const double far2 = (d + pu.w) * (d + pu.w) + aa * aa;
+              if (R > 0.0 && far2 < R * R) atomicOr(&bflag[a], PB_COVERED);
+            }
+          }
+          sl += WAVE;
+          while (sl >= nn) { sl -= nn; ++c; }
+        }
+      }
+      __syncthreads();
+
+      // 4. probes of the candidates of live pairs and their collisions
+      // (_core.py:1545-1623)
+      for (int q0 = 0; q0 < nc; q0 += WAVE) {
+        const int q = q0 + lane();
+        bool okp = false, okm = false;
+        int a = 0, j = -1, k = -1;
+        V3 pp{0, 0, 0}, pm{0, 0, 0};
+        if (q < nc) {
+          const int e = queue[q];
+          a = e >> 8;
+          const int sl = e & 255;
+          if (!(bflag[a] & (PB_KILLED | PB_COVERED))) {
+            j = bj[a];
+            k = lj[sl];
+            V3 uijk;
+            double h;
+            const double4 ak = lx[sl], pu = bu[a], pt = bt[a];
+            okp = okm = probe_points(xi, eri, ld3(ak), ak.w + s.rp, ld3(pu), ld3(pt), &pp, &pm, &uijk, &h);
+          }
+        }
+        for (int t = 0; t < nn && wave_any(okp || okm); ++t) {
+          const int n = lj[t];
+          const double4 ax = lx[t];
+          if (n != j && n != k) {
+            const double rr = ax.w + s.rp;
+            const double lim = rr * rr;
+            const V3 xs = ld3(ax);
+            const V3 dp = pp - xs, dm = pm - xs;
+            okp = okp && !(dot_alt(dp, dp) <= lim);
+            okm = okm && !(dot_alt(dm, dm) <= lim);
+          }
+        }
+        if (okp || okm) {
+          w.access[i] = 1;
+          w.access[j] = 1;
+          w.access[k] = 1;
+        }
+        if (okp) atomicAdd(&bnp[a], 1);
+        if (okm) atomicAdd(&bnm[a], 1);
+      }
+      __syncthreads();
+
+      // 5. per-pair results
+      if (lane() < nb) {
+        const int p = pb + lane();
+        const int64_t vp = 2ll * g0 + (p - g0);
+        const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
+        const int f = bflag[lane()];
+        int8_t pf = w.pflag[p];
+        int np = 0, nm = 0;
+        if (f & PB_COVERED) {
+          pf = (int8_t)(pf | 4);
+        } else if (!(f & PB_KILLED)) {
+          np = bnp[lane()];
+          nm = bnm[lane()];
+          if (np + nm > 0) pf = (int8_t)(pf | 8);
+        }
+        w.prcnt[vp] = np;
+        w.prcnt[vm] = nm;
+        w.pflag[p] = pf;
+      }
     }
     __syncthreads();
   }

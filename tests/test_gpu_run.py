@@ -155,3 +155,24 @@ def test_graph_replays_match_eager(golden_complex):
     say("  captures %d replays %d exact %d overflow %d" % (cap, rep, exact, ovf))
     assert cap >= 1 and rep >= 3
     bc.close()
+
+
+def test_probe_count_kernels_agree(golden_complex, monkeypatch):
+    """The probes count pass has a per-pair and a per-row form (chosen by
+    batch size); both must give identical counts and results."""
+    from cms_cuda.batch import BatchCalculator
+    cs = _batch(golden_complex, [(400, 40), (700, 200), (300, 300), (900, 500)], 700)
+    outs = []
+    for mode in ("pairs", "rows"):
+        monkeypatch.setenv("CMS_PROBE_COUNT", mode)
+        say("probes count pass per %s, %d complexes" % (mode[:-1], len(cs)))
+        bc = BatchCalculator()
+        bc.load(cs)
+        o = bc.contact_ms(0)
+        o["sc"] = bc.shape_complementarity(0)
+        o["counts"] = bc.counts()
+        bc.close()
+        outs.append(o)
+    a, b = outs
+    assert np.array_equal(a["counts"], b["counts"])
+    assert _same(a, b)
