@@ -126,6 +126,13 @@ __device__ __forceinline__ V3 subdiv_point(V3 cen, V3 x, V3 y, double rad, doubl
   return (cen + x * c) + y * s;
 }
 
+// The same with cos(a) and sin(a) given
+__device__ __forceinline__ V3 subdiv_point_cs(V3 cen, V3 x, V3 y, double rad, double cosa, double sina) {
+  double c = rad * cosa;
+  double s = rad * sina;
+  return (cen + x * c) + y * s;
+}
+
 // Angle from x to v about axis in [0, 2pi), NaN if v is (numerically)
 // perpendicular to the plane.  Also returns y = axis x x.
 __device__ __forceinline__ double arc_angle(V3 x, V3 v, V3 axis, V3* y_out) {

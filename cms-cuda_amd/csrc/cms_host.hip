@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, t1cnt, t1scan, t2cnt, t2scan, tflag, kcnt, kscan, dbase, dgrp;
+  DBuf ccnt, cscan, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, kcnt, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -234,6 +234,7 @@ static Work work(cms_ctx* ctx) {
   w.t2cnt = ctx->t2cnt.as<int>();
   w.t2scan = ctx->t2scan.as<int>();
   w.tflag = ctx->tflag.as<int8_t>();
+  w.tmask = ctx->tmask.as<uint64_t>();
   w.kcnt = ctx->kcnt.as<int>();
   w.kscan = ctx->kscan.as<int>();
   w.dbase = ctx->dbase.as<int64_t>();
@@ -260,7 +261,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
                   &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -485,6 +486,7 @@ static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->t2cnt.ensure(sizeof(int) * (c + 1)));
   CK(ctx->t2scan.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tflag.ensure(c + 1));
+  CK(ctx->tmask.ensure(sizeof(uint64_t) * (c + 1)));
   ctx->cap[M_P] = std::max(ctx->cap[M_P], c);
   return CMS_OK;
 }

@@ -95,6 +95,7 @@ struct Work {
   int* t2cnt;
   int* t2scan;
   int8_t* tflag;
+  uint64_t* tmask;        // count pass: live torus points 0..63 of each pair
   int* kcnt;
   int* kscan;
   int64_t* dbase;         // G*4 stage bases

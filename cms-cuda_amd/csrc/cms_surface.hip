@@ -797,6 +797,45 @@ struct TorusSub {
   int n1, n2;
 };
 
+// Dots of the saddle arcs of one chunk of torus points, spread over the
+// lanes: lane m holds torus point m's arc (start x, direction y, n dots of
+// spacing ps; inc = inclusive scan of n over the lanes), and dot d of the
+// chunk, in emission order, is dot d - (inc[src] - n[src]) of the arc of the
+// first lane src with inc[src] > d.  ctab / stab: cos / sin of
+// adelta * (lane + 1/2), shared by every arc of the pair.
+__device__ __forceinline__ void emit_arc_dots(const BatchView& b, const Settings& s, const Work& w, int64_t obase,
+                                              int n, int inc, V3 p, V3 x, V3 y, double ps, double ts, V3 tij,
+                                              V3 uij, double rij, double adelta, double ctab, double stab,
+                                              int owner, int rowner, int4 desc, int m0, int M) {
+  const int T = __shfl(inc, WAVE - 1, WAVE);
+  for (int d0 = 0; d0 < T; d0 += WAVE) {
+    const int d = d0 + lane();
+    int src = 0;
+#pragma unroll
+    for (int step = WAVE / 2; step > 0; step >>= 1)
+      if (__shfl(inc, src + step - 1, WAVE) <= d) src += step;
+    const int sn = __shfl(n, src, WAVE);
+    const int k = d - (__shfl(inc, src, WAVE) - sn);
+    const V3 sp = v3(__shfl(p.x, src, WAVE), __shfl(p.y, src, WAVE), __shfl(p.z, src, WAVE));
+    const V3 sx = v3(__shfl(x.x, src, WAVE), __shfl(x.y, src, WAVE), __shfl(x.z, src, WAVE));
+    const V3 sy = v3(__shfl(y.x, src, WAVE), __shfl(y.y, src, WAVE), __shfl(y.z, src, WAVE));
+    const double sps = __shfl(ps, src, WAVE);
+    const int kk = k < WAVE ? k : 0;
+    double ck = __shfl(ctab, kk, WAVE), sk = __shfl(stab, kk, WAVE);
+    if (d < T) {
+      if (k >= WAVE) {
+        const double a = adelta * ((double)k + 0.5);
+        ck = cos(a);
+        sk = sin(a);
+      }
+      const V3 pt = subdiv_point_cs(sp, sx, sy, s.rp, ck, sk);
+      const double dist = dist_to_line(tij, uij, pt);
+      const double area = sps * ts * dist / rij;
+      store_dot(b, s, w, obase + d, pt, sp, area, owner, rowner, 2, desc, make_int4(m0 + src, k, M, sn));
+    }
+  }
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
@@ -817,16 +856,6 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
       const int i = w.pi[p], j = w.pj[p];
       const int ri_ = w.rpos[i], rj_ = w.rpos[j];
       const int nn = w.nn[ri_];
-      if (ri_ != staged) {
-        const int* row = w.nbr + (int64_t)ri_ * KCAP;
-        for (int sl = lane(); sl < nn; sl += WAVE) {
-          const int k = row[sl];
-          lj[sl] = k;
-          lx[sl] = b.atom[k];
-        }
-        __syncthreads();
-        staged = ri_;
-      }
       const double4 ai = b.atom[i], aj = b.atom[j];
       const V3 xi = ld3(ai), xj = ld3(aj);
       const double ri = ai.w, rj = aj.w;
@@ -850,33 +879,53 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
       const double adelta = 1.0 / (sqrt(dens) * s.rp);
       const double rp2 = s.rp * s.rp;
       const bool arc1_on = w.atten[i] >= ATTEN_2, arc2_on = w.atten[j] >= ATTEN_2;
-      // Neighbours (other than j) that come within r + rp of the torus
-      // circle (centre tij, axis uij, radius rij): only they can collide
-      // with a torus point.  The 1e-4 A margin covers the rounding of the
-      // circle distance (cancellation in the radial term near the axis)
-      // and of the torus points, so the pruned set gives the same
-      // collision results.
+      // the emit pass takes the collision outcome of torus points 0..63 from
+      // the count pass
+      const bool masked = EMIT && M <= WAVE;
       int nrel = 0;
-      for (int s0 = 0; s0 < nn; s0 += WAVE) {
-        const int sl = s0 + lane();
-        bool rel = false;
-        if (sl < nn && lj[sl] != j) {
-          const V3 v = ld3(lx[sl]) - tij;
-          const double ax = dot_seq(v, uij);
-          const double rad = sqrt(fmax(dot_seq(v, v) - ax * ax, 0.0)) - rij;
-          const double reach = (lx[sl].w + s.rp) + 1e-4;
-          rel = rad * rad + ax * ax < reach * reach;
+      if (!masked) {
+        if (ri_ != staged) {
+          const int* row = w.nbr + (int64_t)ri_ * KCAP;
+          for (int sl = lane(); sl < nn; sl += WAVE) {
+            const int k = row[sl];
+            lj[sl] = k;
+            lx[sl] = b.atom[k];
+          }
+          __syncthreads();
+          staged = ri_;
         }
-        const uint64_t mk = ballot(rel);
-        if (rel) rl[nrel + lanes_below(mk)] = sl;
-        nrel += __popcll(mk);
+        // Neighbours (other than j) that come within r + rp of the torus
+        // circle (centre tij, axis uij, radius rij): only they can collide
+        // with a torus point.  The 1e-4 A margin covers the rounding of the
+        // circle distance (cancellation in the radial term near the axis)
+        // and of the torus points, so the pruned set gives the same
+        // collision results.
+        for (int s0 = 0; s0 < nn; s0 += WAVE) {
+          const int sl = s0 + lane();
+          bool rel = false;
+          if (sl < nn && lj[sl] != j) {
+            const V3 v = ld3(lx[sl]) - tij;
+            const double ax = dot_seq(v, uij);
+            const double rad = sqrt(fmax(dot_seq(v, v) - ax * ax, 0.0)) - rij;
+            const double reach = (lx[sl].w + s.rp) + 1e-4;
+            rel = rad * rad + ax * ax < reach * reach;
+          }
+          const uint64_t mk = ballot(rel);
+          if (rel) rl[nrel + lanes_below(mk)] = sl;
+          nrel += __popcll(mk);
+        }
+        __syncthreads();
       }
-      __syncthreads();
+      const uint64_t lmask = masked ? w.tmask[p] : 0ull;
       int64_t ob1 = 0, ob2 = 0;
+      double ctab = 0.0, stab = 0.0;
       if (EMIT) {
         const int grp = group_of_atom(b, i);
         ob1 = w.dbase[grp * 4 + 1] + w.t1scan[p];
         ob2 = w.dbase[grp * 4 + 2] + w.t2scan[p];
+        const double a = adelta * ((double)lane() + 0.5);
+        ctab = cos(a);
+        stab = sin(a);
       }
       for (int m0 = 0; m0 < M; m0 += WAVE) {
         const int m = m0 + lane();
@@ -885,12 +934,16 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
         bool live = false;
         if (m < M) {
           const V3 pm = subdiv_point(tij, cx, cy, rij, tdelta, m);
-          live = true;
-          for (int t = 0; t < nrel; ++t) {
-            const int sl = rl[t];
-            const V3 d = pm - ld3(lx[sl]);
-            const double lim = lx[sl].w + s.rp;
-            if (dot_alt(d, d) < lim * lim) { live = false; break; }
+          if (masked) {
+            live = (lmask >> lane()) & 1ull;
+          } else {
+            live = true;
+            for (int t = 0; t < nrel; ++t) {
+              const int sl = rl[t];
+              const V3 d = pm - ld3(lx[sl]);
+              const double lim = lx[sl].w + s.rp;
+              if (dot_alt(d, d) < lim * lim) { live = false; break; }
+            }
           }
           if (live) {
             ts_.p = pm;
@@ -922,32 +975,18 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
             }
           }
         }
-        any_live = any_live || wave_any(live);
+        const uint64_t lm = ballot(live);
+        if (!EMIT && m0 == 0 && lane() == 0) w.tmask[p] = lm;
+        any_live = any_live || lm != 0ull;
         const int inc1 = wave_incl_scan(ts_.n1);
         const int inc2 = wave_incl_scan(ts_.n2);
         if (EMIT) {
-          if (ts_.n1 > 0) {
-            const double ps = (s.rp * ts_.ang1) / (double)ts_.n1;
-            const int64_t o0 = ob1 + tot1 + inc1 - ts_.n1;
-            for (int k = 0; k < ts_.n1; ++k) {
-              const V3 pt = subdiv_point(ts_.p, ts_.pi, ts_.y1, s.rp, adelta, k);
-              const double dist = dist_to_line(tij, uij, pt);
-              const double area = ps * ts * dist / rij;
-              store_dot(b, s, w, o0 + k, pt, ts_.p, area, i, ri_, 2, make_int4(DK_ARC1, i, j, -1),
-                        make_int4(m, k, M, ts_.n1));
-            }
-          }
-          if (ts_.n2 > 0) {
-            const double ps = (s.rp * ts_.ang2) / (double)ts_.n2;
-            const int64_t o0 = ob2 + tot2 + inc2 - ts_.n2;
-            for (int k = 0; k < ts_.n2; ++k) {
-              const V3 pt = subdiv_point(ts_.p, ts_.pqj, ts_.y2, s.rp, adelta, k);
-              const double dist = dist_to_line(tij, uij, pt);
-              const double area = ps * ts * dist / rij;
-              store_dot(b, s, w, o0 + k, pt, ts_.p, area, j, rj_, 2, make_int4(DK_ARC2, i, j, -1),
-                        make_int4(m, k, M, ts_.n2));
-            }
-          }
+          const double ps1 = ts_.n1 > 0 ? (s.rp * ts_.ang1) / (double)ts_.n1 : 0.0;
+          const double ps2 = ts_.n2 > 0 ? (s.rp * ts_.ang2) / (double)ts_.n2 : 0.0;
+          emit_arc_dots(b, s, w, ob1 + tot1, ts_.n1, inc1, ts_.p, ts_.pi, ts_.y1, ps1, ts, tij, uij, rij, adelta,
+                        ctab, stab, i, ri_, make_int4(DK_ARC1, i, j, -1), m0, M);
+          emit_arc_dots(b, s, w, ob2 + tot2, ts_.n2, inc2, ts_.p, ts_.pqj, ts_.y2, ps2, ts, tij, uij, rij, adelta,
+                        ctab, stab, j, rj_, make_int4(DK_ARC2, i, j, -1), m0, M);
         }
         tot1 += __shfl(inc1, WAVE - 1, WAVE);
         tot2 += __shfl(inc2, WAVE - 1, WAVE);
