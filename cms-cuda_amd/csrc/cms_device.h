@@ -106,15 +106,16 @@ __device__ __forceinline__ double wave_min_double(double v) {
 // ------------------------------------------------------- arc subdivision
 // Number of points a_i = delta*(i+1/2) <= angle with i < MAX_SUBDIV, where
 // delta = 1/(sqrt(density)*rad).  NaN angle or infinite delta give 0.
-__device__ __forceinline__ int subdiv_count(double delta, double angle) {
+__host__ __device__ __forceinline__ int subdiv_count(double delta, double angle) {
   double raw = angle / delta;
   if (!(raw > 0.0) || !(raw < 1e300)) return 0;
-  int n = 0;
-  int lim = (int)fmin((double)MAX_SUBDIV, ceil(raw) + 1.0);
-  for (int i = 0; i < lim; ++i) {
-    double a = delta * ((double)i + 0.5);
-    if (a <= angle) n = i + 1; else break;
-  }
+  const int lim = (int)fmin((double)MAX_SUBDIV, ceil(raw) + 1.0);
+  // a_i is non-decreasing in i (a rounded product of delta > 0 and an exact
+  // i + 1/2), so the count is the first i < lim with a_i > angle: start
+  // from the closed-form estimate and step to it with the exact test
+  int n = (int)fmin(fmax(floor(raw - 0.5) + 1.0, 0.0), (double)lim);
+  while (n > 0 && delta * ((double)(n - 1) + 0.5) > angle) --n;
+  while (n < lim && delta * ((double)n + 0.5) <= angle) ++n;
   return n;
 }
 

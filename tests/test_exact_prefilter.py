@@ -1,6 +1,8 @@
-"""The exact squared-distance prefilter of the triangle tests (norm_lt in
-cms-cuda_amd/csrc/cms_device.h) against the square-root comparison it
-replaces, compiled for the host from the same header (hipcc, no GPU)."""
+"""Exact shortcuts of cms-cuda_amd/csrc/cms_device.h against the definitions
+they replace, compiled for the host from the same header (hipcc, no GPU):
+the squared-distance prefilter of the triangle tests (norm_lt) against the
+square-root comparison, and the closed-form subdivision count against its
+defining loop."""
 
 import os
 import shutil
@@ -11,12 +13,12 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_norm_lt_matches_sqrt(tmp_path):
+def _run_check(tmp_path, name):
     hipcc = os.environ.get("HIPCC") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(hipcc) and shutil.which("hipcc") is None:
         pytest.skip("hipcc not available")
-    exe = str(tmp_path / "norm_lt_check")
-    src = os.path.join(ROOT, "tests", "cpp", "norm_lt_check.cpp")
+    exe = str(tmp_path / name)
+    src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
     inc = os.path.join(ROOT, "cms-cuda_amd", "csrc")
     cmd = [hipcc if os.path.exists(hipcc) else "hipcc", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
            "-x", "hip", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -28,4 +30,14 @@ def test_norm_lt_matches_sqrt(tmp_path):
     assert out.returncode == 0, out.stdout
     fields = out.stdout.split()
     assert int(fields[fields.index("mismatches") + 1]) == 0
+    return fields
+
+
+def test_norm_lt_matches_sqrt(tmp_path):
+    fields = _run_check(tmp_path, "norm_lt_check")
     assert int(fields[fields.index("near-band") + 1]) > 1000   # the exact branch was exercised
+
+
+def test_subdiv_count_matches_loop(tmp_path):
+    fields = _run_check(tmp_path, "subdiv_count_check")
+    assert int(fields[fields.index("on-point") + 1]) > 100000   # angles exactly on a subdivision point
