@@ -19,6 +19,7 @@ constexpr int WAVE = 64;
 constexpr int MAX_SUBDIV = 100;
 constexpr int KCAP = 256;      // max neighbours per atom row
 constexpr int BCAP = 256;      // max buried (other-molecule) partners per atom
+constexpr int CMASK_WORDS = 16; // convex points per row whose collision outcome is kept (x 64)
 constexpr int NEARCAP = 128;   // max near low probes per low probe
 constexpr int8_t ATTEN_BLOCKER = 1;
 constexpr int8_t ATTEN_2 = 2;

@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, kcnt, kscan, dbase, dgrp;
+  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, kcnt, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -229,6 +229,7 @@ static Work work(cms_ctx* ctx) {
   w.nnear = ctx->nnear.as<int>();
   w.ccnt = ctx->ccnt.as<int>();
   w.cscan = ctx->cscan.as<int>();
+  w.cmask = ctx->cmask.as<uint64_t>();
   w.t1cnt = ctx->t1cnt.as<int>();
   w.t1scan = ctx->t1scan.as<int>();
   w.t2cnt = ctx->t2cnt.as<int>();
@@ -260,7 +261,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
-                  &ctx->cscan, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
+                  &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
                   &ctx->tflag, &ctx->tmask, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
@@ -469,6 +470,7 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->pscan.ensure(sizeof(int) * (c + 1)));
   CK(ctx->ccnt.ensure(sizeof(int) * (c + 1)));
   CK(ctx->cscan.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->cmask.ensure(sizeof(uint64_t) * (c * CMASK_WORDS + 1)));
   ctx->cap[M_R] = std::max(ctx->cap[M_R], c);
   return CMS_OK;
 }

@@ -90,6 +90,7 @@ struct Work {
   // dot counts
   int* ccnt;
   int* cscan;
+  uint64_t* cmask;        // count pass: kept convex points, CMASK_WORDS words per row
   int* t1cnt;
   int* t1scan;
   int* t2cnt;

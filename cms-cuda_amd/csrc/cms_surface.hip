@@ -681,7 +681,6 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
     int total = 0;
     if (nn > 0 && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER) {
       const int* row = w.nbr + (int64_t)r * KCAP;
-      for (int sl = lane(); sl < nn; sl += WAVE) lx[sl] = b.atom[row[sl]];
       const double4 ai = b.atom[g];
       const V3 xi = ld3(ai);
       const double ri = ai.w, eri = ri + s.rp;
@@ -749,8 +748,18 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
         obase = w.dbase[grp * 4 + 0] + w.cscan[r];
       }
       const double scale = eri / ri;
+      // the neighbours, for the collision tests the emit pass does not take
+      // from the count pass
+      if (!EMIT || C > CMASK_WORDS * WAVE) {
+        for (int sl = lane(); sl < nn; sl += WAVE) lx[sl] = b.atom[row[sl]];
+        __syncthreads();
+      }
       for (int q0 = 0; q0 < C; q0 += WAVE) {
         const int q = q0 + lane();
+        // the emit pass takes the collision outcome of the first
+        // CMASK_WORDS * 64 points from the count pass
+        const bool masked = EMIT && q0 < CMASK_WORDS * WAVE;
+        const uint64_t kmask = masked ? w.cmask[(int64_t)r * CMASK_WORDS + q0 / WAVE] : 0ull;
         bool keep = false;
         V3 pt{0, 0, 0}, pc{0, 0, 0};
         int t = 0;
@@ -764,14 +773,19 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
           const int kk = q - lat_start[t];
           pt = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], lat_delta[t], kk);
           pc = xi + (pt - xi) * scale;
-          keep = true;
-          for (int sl = 1; sl < nn; ++sl) {
-            const V3 d = ld3(lx[sl]) - pc;
-            const double lim = lx[sl].w + s.rp;
-            if (dot_seq(d, d) <= lim * lim) { keep = false; break; }
+          if (masked) {
+            keep = (kmask >> lane()) & 1ull;
+          } else {
+            keep = true;
+            for (int sl = 1; sl < nn; ++sl) {
+              const V3 d = ld3(lx[sl]) - pc;
+              const double lim = lx[sl].w + s.rp;
+              if (dot_seq(d, d) <= lim * lim) { keep = false; break; }
+            }
           }
         }
         const uint64_t mk = ballot(keep);
+        if (!EMIT && q0 < CMASK_WORDS * WAVE && lane() == 0) w.cmask[(int64_t)r * CMASK_WORDS + q0 / WAVE] = mk;
         if (EMIT && keep) {
           const int64_t o = obase + total + lanes_below(mk);
           store_dot(b, s, w, o, pt, pc, lat_ps[t] * cs, g, r, 1, make_int4(DK_CONVEX, g, row[0], -1),
