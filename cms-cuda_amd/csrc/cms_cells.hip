@@ -110,11 +110,15 @@ __global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int
 }
 
 __global__ void k_cell_scatter(int n, const int* __restrict__ acell, const int* __restrict__ cstart,
-                               int* __restrict__ cursor, int* __restrict__ catom) {
+                               int* __restrict__ cursor, int* __restrict__ catom,
+                               const double4* __restrict__ atom, double4* __restrict__ cxyz) {
   CMS_WAVE_LOOP(g, n) {
     const int cell = g < n ? acell[g] : 0;
     const int slot = wave_counter_add(cursor, cell, g < n);
-    if (g < n) catom[cstart[cell] + slot] = (int)g;
+    if (g < n) {
+      catom[cstart[cell] + slot] = (int)g;
+      cxyz[cstart[cell] + slot] = atom[g];
+    }
   }
 }
 
@@ -159,7 +163,7 @@ __device__ __forceinline__ uint32_t coincident_cells(const GridInfo& gi, double 
       for (int cellvar = cell_of((gi), (ix), (iy), (iz), k_), once_ = 1; once_ && cellvar >= 0; once_ = 0)
 
 // Flattened candidate list of the cells around (ix, iy, iz) selected by
-// mask, molecule range sel: cbeg[k] / cpre[k] are the start in catom and the
+// mask, molecule range sel: cbeg[k] / cpre[k] are the start in catom / cxyz and the
 // prefix count of cell k; cpre[27] is the total.  Wave-cooperative (lanes
 // 0..26 each own a cell).
 __device__ __forceinline__ void cells_around(const GridInfo& gi, int ix, int iy, int iz,
@@ -180,14 +184,13 @@ __device__ __forceinline__ void cells_around(const GridInfo& gi, int ix, int iy,
   __syncthreads();
 }
 
-__device__ __forceinline__ int cell_candidate(const int* cbeg, const int* cpre, const int* __restrict__ catom,
-                                              int idx) {
+__device__ __forceinline__ int cell_slot(const int* cbeg, const int* cpre, int idx) {
   int lo = 0, hi = 27;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (cpre[mid] <= idx) lo = mid; else hi = mid;
   }
-  return catom[cbeg[lo] + idx - cpre[lo]];
+  return cbeg[lo] + idx - cpre[lo];
 }
 
 // Attention (_core.py:1150-1195) and coincident atoms (_core.py:1285-1293):
@@ -215,6 +218,7 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
                                                         const GridInfo* __restrict__ grid,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
+                                                        const double4* __restrict__ cxyz,
                                                         int8_t* __restrict__ atten, int8_t* __restrict__ access,
                                                         int* __restrict__ need,
                                                         unsigned long long* __restrict__ coinc,
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
     const int a0 = b.aoff[c];
     const int m = b.amol[g];
     const GridInfo gi = grid[c];
-    const double4 ai = b.atom[g];
+    const double4 ai = cxyz[sidx];
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
@@ -235,8 +239,7 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
       cells_around(gi, ix, iy, iz, cstart, 1 - m, 0x7ffffffu, cbeg, cpre);
       const int T = cpre[27];
       for (int idx = lane(); idx < T; idx += WAVE) {
-        const int j = cell_candidate(cbeg, cpre, catom, idx);
-        const V3 d = xi - ld3(b.atom[j]);
+        const V3 d = xi - ld3(cxyz[cell_slot(cbeg, cpre, idx)]);
         mn = fmin(mn, dot_seq(d, d));
       }
       mn = wave_min_double(mn);
@@ -246,8 +249,9 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
     cells_around(gi, ix, iy, iz, cstart, m, coincident_cells(gi, ai.x, ai.y, ai.z, ix, iy, iz), cbeg, cpre);
     const int T = cpre[27];
     for (int idx = lane(); idx < T; idx += WAVE) {
-      const int j = cell_candidate(cbeg, cpre, catom, idx);
-      const V3 d = xi - ld3(b.atom[j]);
+      const int t = cell_slot(cbeg, cpre, idx);
+      const int j = catom[t];
+      const V3 d = xi - ld3(cxyz[t]);
       if (j != g && dot_seq(d, d) <= 0.0001) {
         const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
         ckey = ckey < key ? ckey : key;
@@ -268,6 +272,7 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
 __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, const GridInfo* __restrict__ grid,
                                                      const int* __restrict__ cstart,
                                                      const int* __restrict__ catom,
+                                                     const double4* __restrict__ cxyz,
                                                      const int8_t* __restrict__ atten, int* __restrict__ need) {
   __shared__ int cbeg[27], cpre[28];
   CMS_UNIT_LOOP(sidx, b.ntot) {
@@ -276,15 +281,16 @@ __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, co
     const int c = b.acplx[g];
     const int m = b.amol[g];
     const GridInfo gi = grid[c];
-    const double4 ai = b.atom[g];
+    const double4 ai = cxyz[sidx];
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     cells_around(gi, ix, iy, iz, cstart, m, 0x7ffffffu, cbeg, cpre);
     const int T = cpre[27];
     for (int idx = lane(); idx < T; idx += WAVE) {
-      const int j = cell_candidate(cbeg, cpre, catom, idx);
-      const double4 aj = b.atom[j];
+      const int t = cell_slot(cbeg, cpre, idx);
+      const int j = catom[t];
+      const double4 aj = cxyz[t];
       const V3 d = xi - ld3(aj);
       const double br = (ai.w + aj.w) + 2.0 * s.rp;
       if (dot_seq(d, d) < br * br) need[j] = 1;
@@ -300,6 +306,7 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
                                                            const GridInfo* __restrict__ grid,
                                                            const int* __restrict__ cstart,
                                                            const int* __restrict__ catom,
+                                                           const double4* __restrict__ cxyz,
                                                            int8_t* __restrict__ atten, int8_t* __restrict__ access,
                                                            int* __restrict__ need,
                                                            unsigned long long* __restrict__ coinc,
@@ -310,7 +317,7 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
     const int a0 = b.aoff[c];
     const int m = b.amol[g];
     const GridInfo gi = grid[c];
-    const double4 ai = b.atom[g];
+    const double4 ai = cxyz[sidx];
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
@@ -318,8 +325,9 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
     if (!all_atoms) {
       FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, 0x7ffffffu, cell) {
         const int lo = cstart[2 * cell + 1 - m], hi = cstart[2 * cell + 2 - m];
+#pragma unroll 4
         for (int t = lo; t < hi; ++t) {
-          const V3 d = xi - ld3(b.atom[catom[t]]);
+          const V3 d = xi - ld3(cxyz[t]);
           mn = fmin(mn, dot_seq(d, d));
         }
       }
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
     FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cmask, cell) {
       for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
         const int j = catom[t];
-        const V3 d = xi - ld3(b.atom[j]);
+        const V3 d = xi - ld3(cxyz[t]);
         if (j != g && dot_seq(d, d) <= 0.0001) {
           const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
           ckey = ckey < key ? ckey : key;
@@ -343,20 +351,22 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
 __global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s, const GridInfo* __restrict__ grid,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
+                                                        const double4* __restrict__ cxyz,
                                                         const int8_t* __restrict__ atten, int* __restrict__ need) {
   for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
     const int g = catom[sidx];
     if (atten[g] <= ATTEN_BLOCKER) continue;
     const int m = b.amol[g];
     const GridInfo gi = grid[b.acplx[g]];
-    const double4 ai = b.atom[g];
+    const double4 ai = cxyz[sidx];
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, 0x7ffffffu, cell) {
+#pragma unroll 4
       for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
         const int j = catom[t];
-        const double4 aj = b.atom[j];
+        const double4 aj = cxyz[t];
         const V3 d = xi - ld3(aj);
         const double br = (ai.w + aj.w) + 2.0 * s.rp;
         if (dot_seq(d, d) < br * br) need[j] = 1;
@@ -371,7 +381,8 @@ __global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s,
 __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s, Work w,
                                                         const GridInfo* __restrict__ grid,
                                                         const int* __restrict__ cstart,
-                                                        const int* __restrict__ catom) {
+                                                        const int* __restrict__ catom,
+                                                        const double4* __restrict__ cxyz) {
   __shared__ int cj[KCAP];
   __shared__ double cd[KCAP];
   __shared__ int bj[BCAP];
@@ -395,8 +406,9 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
       double d2 = 0.0;
       int j = -1;
       if (idx < T) {
-        j = cell_candidate(cbeg, cpre, catom, idx);
-        const double4 aj = b.atom[j];
+        const int t = cell_slot(cbeg, cpre, idx);
+        j = catom[t];
+        const double4 aj = cxyz[t];
         const V3 d = xi - ld3(aj);
         d2 = dot_seq(d, d);
         const double br = (ai.w + aj.w) + 2.0 * s.rp;

@@ -75,7 +75,7 @@ struct cms_ctx {
   int nR = 0, nP = 0, nQ = 0, nD = 0;
   std::vector<int> h_dgrp, h_prgrp;
   DBuf atten, access, status, coinc, flag, nbscan, nbl, need, rpos, ratom, part;
-  DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom;
+  DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom, cxyz;
   DBuf nn, nbr, nb, bur, nbmax;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
@@ -256,7 +256,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
   return {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
                   &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
                   &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
-                  &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->need, &ctx->rpos, &ctx->ratom,
+                  &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
@@ -639,6 +639,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CK(ctx->ncscan.ensure(sizeof(int) * (B + 1)));
   CK(ctx->acell.ensure(sizeof(int) * (n + 1)));
   CK(ctx->catom.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->cxyz.ensure(sizeof(double4) * (n + 1)));
   CK(ctx->ccount.ensure(sizeof(int) * (cell_cap + 1)));
   CK(ctx->cstart.ensure(sizeof(int) * (cell_cap + 1)));
   CK(ctx->ccur.ensure(sizeof(int) * (cell_cap + 1)));
@@ -667,28 +668,30 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), dev_count(ctx->ncscan.as<int>() + B, 2), cell_cap, part,
               st);
     k_cell_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
-                                                            ctx->ccur.as<int>(), ctx->catom.as<int>());
+                                                            ctx->ccur.as<int>(), ctx->catom.as<int>(),
+                                                            ctx->atom.as<double4>(), ctx->cxyz.as<double4>());
     // small batches: one wavefront per atom (parallelism); large batches:
     // one thread per cell-sorted atom (cell data shared within a wavefront)
     const bool per_thread = n >= 200000;
     const int* cst = ctx->cstart.as<int>();
     const int* cat = ctx->catom.as<int>();
+    const double4* cxyz = ctx->cxyz.as<double4>();
     // every atom is in the region in all-atoms / full-state runs
     const int full = (full_state || all_atoms) ? 1 : 0;
     if (per_thread) {
       TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
-          b, s, all_atoms, full, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
+          b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
           ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
       if (!full)
         TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
-            b, s, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+            b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
     } else {
       TIMED("attention", k_attention_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
-          b, s, all_atoms, full, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
+          b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
           ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
       if (!full)
         TIMED("region", k_region_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
-            b, s, grid, cst, cat, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+            b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
     }
     scan_excl(ctx->need.as<int>(), ctx->rpos.as<int>(), n, part, st);
     k_compact_index<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->need.as<int>(),
@@ -712,7 +715,8 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   if (est[M_R] > 0) {
     TIMED("neighbors", k_neighbors_cells<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w, ctx->grid.as<GridInfo>(),
                                                                                 ctx->cstart.as<int>(),
-                                                                                ctx->catom.as<int>()));
+                                                                                ctx->catom.as<int>(),
+                                                                                ctx->cxyz.as<double4>()));
     TIMED("pairs_count", k_pairs<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
     CKL();
   }

@@ -367,20 +367,28 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
 // row's forward pairs (i, j) are taken in batches, and each step spreads its
 // work over the lanes across pair boundaries, so that lanes are not left
 // idle by the few valid triples of a single pair:
-//   1. triangle tests of every (pair, slot) item; valid triples are queued;
+//   1. triangle tests of the triples (i, j, k), k > j: the row's forward
+//      atoms are ranked by atom index once, so each pair visits exactly the
+//      atoms after j; valid triples are queued;
 //   2. degenerate / kill tests of the queued triples, compacting the
 //      non-degenerate ones (candidates) in place;
-//   3. the single-sphere containment test of the pairs that have candidates
-//      or belong to the toroid queue;
-//   4. probe positions, one candidate per lane, and their collision with the
-//      row's atoms other than j and k (slots in row order, until every lane's
-//      probes have collided).
+//   3. per (pair, row atom): the single-sphere containment test of the pairs
+//      that have candidates or belong to the toroid queue, and the row atoms
+//      that can reach the pair's torus circle (the only ones a probe of the
+//      pair can collide with);
+//   4. probe positions, one candidate per lane, and their collisions with the
+//      reaching row atoms other than j and k.
 // Per pair it writes what the pair-per-wavefront emit pass relies on: the
 // probe counts of both signs, pflag bits 4 (circle inside one neighbour
 // sphere) and 8 (probes), and the access marks of the atoms of every
 // surviving probe (_core.py:1445-1643).
-constexpr int PR_PAIRS = 32;    // pairs per batch (at most one per lane in step 3 / 5)
-constexpr int PR_QCAP = 768;    // (pair, slot) items per batch, bounds the triple queue
+constexpr int PR_PAIRS = 32;            // pairs per batch (one per lane in the per-pair steps)
+constexpr int PR_QCAP = 1024;           // triangle-test items per batch, bounds the triple queue
+constexpr int PR_RELW = KCAP / WAVE;    // reach mask words per pair
+// Reach margin: a computed probe of a non-near-degenerate triple lies within
+// 1e-4 A of its pair's torus circle (see circle_in_one_sphere), and the
+// squared-distance forms below round far below 1e-10 A.
+constexpr double REACH_MARGIN = 0.01;
 
 // pair-batch flag bits
 constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32;
@@ -389,10 +397,13 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ int lj[KCAP];
   __shared__ double4 lx[KCAP];
   __shared__ int8_t latt[KCAP];
+  __shared__ uint8_t fws[KCAP];   // forward slots (atom index > i) by ascending atom index
   __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
-  __shared__ int queue[PR_QCAP];
-  static_assert(KCAP <= 256 && PR_PAIRS <= WAVE, "queue entries pack (pair << 8) | slot");
+  __shared__ int bfirst[PR_PAIRS], bpre[PR_PAIRS + 1];
+  __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
+  __shared__ uint16_t queue[PR_QCAP];
+  static_assert(KCAP <= 256 && PR_PAIRS <= 32, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
   auto live = [&](int64_t q) { return w.pcnt[q] != 0; };
@@ -411,19 +422,52 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         latt[sl] = w.atten[k];
       }
     }
+    __syncthreads();
+    // rank the forward atoms by index
+    int F = 0;
+    for (int s0 = 0; s0 < nn; s0 += WAVE) {
+      const int sl = s0 + lane();
+      int k = -1;
+      if (sl < nn) k = lj[sl];
+      const bool fwd = k > i;
+      if (fwd) {
+        int rank = 0;
+        for (int t = 0; t < nn; ++t) {
+          const int kt = lj[t];
+          rank += (kt > i) & (kt < k);
+        }
+        fws[rank] = (uint8_t)sl;
+      }
+      F += __popcll(ballot(fwd));
+    }
     const double4 ai = b.atom[i];
     const V3 xi = ld3(ai);
     const double eri = ai.w + s.rp;
     const int grp = group_of_atom(b, i);
     const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
     const bool iblock = w.atten[i] <= ATTEN_BLOCKER;
-    const int PB = max(1, min(PR_PAIRS, PR_QCAP / nn));
-    for (int pb = p_lo; pb < p_hi; pb += PB) {
-      const int nb = min(PB, p_hi - pb);
-      __syncthreads();   // row staged / previous batch consumed
+    const int nw = (nn + WAVE - 1) / WAVE;
+    __syncthreads();
+    for (int pb = p_lo; pb < p_hi;) {
+      // the batch: consecutive pairs whose triangle items fit the queue
+      const int avail = min(PR_PAIRS, p_hi - pb);
+      int j = -1, first = F, cnt = 0;
+      if (lane() < avail) {
+        j = w.pj[pb + lane()];
+        // forward atoms after j: positions first .. F-1 of fws
+        int lo = 0, hi = F;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (lj[fws[mid]] <= j) lo = mid + 1; else hi = mid;
+        }
+        first = lo;
+        cnt = F - lo;
+      }
+      const int inc = wave_incl_scan(cnt);
+      const int nb = __popcll(ballot(lane() < avail && inc <= PR_QCAP));   // >= 1: cnt < KCAP <= PR_QCAP
+      const int items = __shfl(inc, nb - 1, WAVE);
       if (lane() < nb) {
         const int p = pb + lane();
-        const int j = w.pj[p];
         bj[lane()] = j;
         bu[lane()] = w.pu[p];
         bt[lane()] = w.pt[p];
@@ -432,32 +476,32 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         bflag[lane()] = ((w.pflag[p] & 2) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0);
         bnp[lane()] = 0;
         bnm[lane()] = 0;
+        bfirst[lane()] = first;
+        bpre[lane()] = inc - cnt;
+        for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = 0ull;
       }
       __syncthreads();
 
       // 1. triangle tests (_core.py:1478-1500)
-      const int items = nb * nn;
       int nq = 0;
-      {
-        int a = lane() / nn, sl = lane() - (lane() / nn) * nn;
-        for (int it0 = 0; it0 < items; it0 += WAVE) {
-          bool valid = false;
-          if (it0 + lane() < items) {
-            const int k = lj[sl];
-            if (k > bj[a]) {
-              const double4 ak = lx[sl], aj = bx[a];
-              const V3 xk = ld3(ak);
-              const double erk = ak.w + s.rp, erj = aj.w + s.rp;
-              valid = norm_lt(xk - ld3(aj), erj + erk) && norm_lt(xk - xi, eri + erk) &&
-                      !((bflag[a] & PB_BLOCKERS) && latt[sl] <= ATTEN_BLOCKER);
-            }
-          }
-          const uint64_t m = ballot(valid);
-          if (valid) queue[nq + lanes_below(m)] = (a << 8) | sl;
-          nq += __popcll(m);
-          sl += WAVE;
-          while (sl >= nn) { sl -= nn; ++a; }
+      for (int it0 = 0; it0 < items; it0 += WAVE) {
+        const int it = it0 + lane();
+        bool valid = false;
+        int a = 0, sl = 0;
+        if (it < items) {
+#pragma unroll
+          for (int step = PR_PAIRS / 2; step > 0; step >>= 1)
+            if (a + step < nb && bpre[a + step] <= it) a += step;
+          sl = fws[bfirst[a] + (it - bpre[a])];
+          const double4 ak = lx[sl], aj = bx[a];
+          const V3 xk = ld3(ak);
+          const double erk = ak.w + s.rp, erj = aj.w + s.rp;
+          valid = norm_lt(xk - ld3(aj), erj + erk) && norm_lt(xk - xi, eri + erk) &&
+                  !((bflag[a] & PB_BLOCKERS) && latt[sl] <= ATTEN_BLOCKER);
         }
+        const uint64_t m = ballot(valid);
+        if (valid) queue[nq + lanes_below(m)] = (uint16_t)((a << 8) | sl);
+        nq += __popcll(m);
       }
       __syncthreads();
 
@@ -487,18 +531,20 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           }
         }
         const uint64_t m = ballot(cand);
-        if (cand) queue[nc + lanes_below(m)] = e;
+        if (cand) queue[nc + lanes_below(m)] = (uint16_t)e;
         nc += __popcll(m);
       }
       __syncthreads();
 
-      // 3. pairs whose torus circle lies inside one neighbour's sphere have
-      // neither probes nor torus points (see circle_in_one_sphere)
+      // 3. containment (circle_in_one_sphere, in squared form) and the row
+      // atoms within reach of the circle; near-degenerate pairs take every
+      // row atom
       {
         bool need = false;
         if (lane() < nb) {
           const int f = bflag[lane()];
-          need = !(f & PB_NEARDEG) && ((f & PB_TOR) || ((f & PB_CAND) && !(f & PB_KILLED)));
+          const bool probes = (f & PB_CAND) && !(f & PB_KILLED);
+          need = probes || (!(f & PB_NEARDEG) && (f & PB_TOR));
         }
         const uint64_t m = ballot(need);
         if (need) cpair[lanes_below(m)] = lane();
@@ -509,13 +555,26 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           if (it0 + lane() < citems) {
             const int a = cpair[c];
             if (lj[sl] != bj[a]) {
+              const int f = bflag[a];
               const double4 ax = lx[sl], pu = bu[a];
               const V3 v = ld3(ax) - ld3(bt[a]);
               const double aa = dot_seq(v, ld3(pu));
-              const double d = sqrt(fmax(dot_seq(v, v) - aa * aa, 0.0));
-              const double R = (ax.w + s.rp) - COVER_MARGIN;
-              const double far2 = (d + pu.w) * (d + pu.w) + aa * aa;
-              if (R > 0.0 && far2 < R * R) atomicOr(&bflag[a], PB_COVERED);
+              const double d2 = fmax(dot_seq(v, v) - aa * aa, 0.0);
+              const double rho2 = pu.w * pu.w;
+              const double rr = ax.w + s.rp;
+              if (!(f & PB_NEARDEG)) {
+                // farthest circle point within R: (d + rho)^2 + a^2 < R^2
+                const double R = rr - COVER_MARGIN;
+                const double D = ((R * R - aa * aa) - d2) - rho2;
+                if (R > 0.0 && D > 0.0 && 4.0 * d2 * rho2 < D * D) atomicOr(&bflag[a], PB_COVERED);
+              }
+              if ((f & PB_CAND) && !(f & PB_KILLED)) {
+                // nearest circle point within reach: (d - rho)^2 + a^2 < reach^2
+                const double reach = rr + REACH_MARGIN;
+                const double E = ((d2 + rho2) + aa * aa) - reach * reach;
+                if ((f & PB_NEARDEG) || E < 0.0 || 4.0 * d2 * rho2 > E * E)
+                  atomicOr((unsigned long long*)&brel[a][sl / WAVE], 1ull << (sl % WAVE));
+              }
             }
           }
           sl += WAVE;
@@ -544,10 +603,13 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
             okp = okm = probe_points(xi, eri, ld3(ak), ak.w + s.rp, ld3(pu), ld3(pt), &pp, &pm, &uijk, &h);
           }
         }
-        for (int t = 0; t < nn && wave_any(okp || okm); ++t) {
-          const int n = lj[t];
-          const double4 ax = lx[t];
-          if (n != j && n != k) {
+        for (int wd = 0; wd < nw && (okp || okm); ++wd) {
+          uint64_t rm = brel[a][wd];
+          while (rm && (okp || okm)) {
+            const int t = wd * WAVE + __builtin_ctzll(rm);
+            rm &= rm - 1;
+            if (lj[t] == k) continue;
+            const double4 ax = lx[t];
             const double rr = ax.w + s.rp;
             const double lim = rr * rr;
             const V3 xs = ld3(ax);
@@ -585,8 +647,9 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         w.prcnt[vm] = nm;
         w.pflag[p] = pf;
       }
+      pb += nb;
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
