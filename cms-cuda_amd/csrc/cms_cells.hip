@@ -439,15 +439,20 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
       cntb = min(cntb, BCAP);
     }
     int* row = w.nbr + (int64_t)r * KCAP;
+    uint8_t* fwd = w.fwd + (int64_t)r * KCAP;
     for (int p = lane(); p < cnt; p += WAVE) {
       const double dp = cd[p];
       const int jp = cj[p];
-      int rank = 0;
+      int rank = 0, frank = 0;
       for (int t = 0; t < cnt; ++t) {
         const double dt = cd[t];
-        rank += (dt < dp) || (dt == dp && cj[t] < jp);
+        const int jt = cj[t];
+        rank += (dt < dp) || (dt == dp && jt < jp);
+        frank += (jt > g) & (jt < jp);
       }
       row[rank] = jp;
+      // forward neighbours (index > g) by ascending index: their row slots
+      if (jp > g) fwd[frank] = (uint8_t)rank;
     }
     int* brow = w.bur + (int64_t)r * BCAP;
     for (int p = lane(); p < cntb; p += WAVE) {

@@ -76,8 +76,8 @@ struct cms_ctx {
   std::vector<int> h_dgrp, h_prgrp;
   DBuf atten, access, status, coinc, flag, nbscan, nbl, need, rpos, ratom, part;
   DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom, cxyz;
-  DBuf nn, nbr, nb, bur, nbmax;
-  DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
+  DBuf nn, nbr, nb, bur, nbmax, fwd;
+  DBuf pcnt, pscan, pi, pj, pslot, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
   DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, kcnt, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
@@ -209,11 +209,13 @@ static Work work(cms_ctx* ctx) {
   w.nb = ctx->nb.as<int>();
   w.bur = ctx->bur.as<int>();
   w.nbmax = ctx->nbmax.as<int>();
+  w.fwd = ctx->fwd.as<uint8_t>();
   w.pcnt = ctx->pcnt.as<int>();
   w.pscan = ctx->pscan.as<int>();
   w.nP = ctx->meta.as<int>() + M_P;
   w.pi = ctx->pi.as<int>();
   w.pj = ctx->pj.as<int>();
+  w.pslot = ctx->pslot.as<uint8_t>();
   w.pu = ctx->pu.as<double4>();
   w.pt = ctx->pt.as<double4>();
   w.pflag = ctx->pflag.as<int8_t>();
@@ -257,8 +259,8 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
                   &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
                   &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
-                  &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
-                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
+                  &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
+                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pslot, &ctx->pu, &ctx->pt, &ctx->pflag,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
@@ -465,6 +467,7 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->nb.ensure(sizeof(int) * (c + 1)));
   CK(ctx->nbmax.ensure(sizeof(int) * (c + 1)));
   CK(ctx->nbr.ensure(sizeof(int) * (c * KCAP + 1)));
+  CK(ctx->fwd.ensure(c * KCAP + 1));
   CK(ctx->bur.ensure(sizeof(int) * (c * BCAP + 1)));
   CK(ctx->pcnt.ensure(sizeof(int) * (c + 1)));
   CK(ctx->pscan.ensure(sizeof(int) * (c + 1)));
@@ -478,6 +481,7 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
 static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->pi.ensure(sizeof(int) * (c + 1)));
   CK(ctx->pj.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->pslot.ensure(c + 1));
   CK(ctx->pu.ensure(sizeof(double4) * (c + 1)));
   CK(ctx->pt.ensure(sizeof(double4) * (c + 1)));
   CK(ctx->pflag.ensure(c + 1));

@@ -67,12 +67,14 @@ struct Work {
   int* nb;
   int* bur;
   int* nbmax;             // largest non-blocker neighbour index per row (-1: none)
+  uint8_t* fwd;            // per row: slots of the neighbours with index > the row atom, by index
   // pairs
   int* pcnt;
   int* pscan;
   const int* nP;
   int* pi;
   int* pj;
+  uint8_t* pslot;          // row slot of j
   double4* pu;
   double4* pt;
   int8_t* pflag;          // bit0 between, bit1 toroid queue

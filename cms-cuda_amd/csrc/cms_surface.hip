@@ -87,6 +87,7 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
         const int o = base + total + lanes_below(mk);
         w.pi[o] = g;
         w.pj[o] = j;
+        w.pslot[o] = (uint8_t)sl;
         w.pu[o] = make_double4(u.x, u.y, u.z, rij);
         w.pt[o] = make_double4(t.x, t.y, t.z, 0.0);
         const int8_t ai8 = w.atten[g], aj8 = w.atten[j];
@@ -398,6 +399,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ double4 lx[KCAP];
   __shared__ int8_t latt[KCAP];
   __shared__ uint8_t fws[KCAP];   // forward slots (atom index > i) by ascending atom index
+  __shared__ uint8_t frk[KCAP];   // forward rank of a forward slot
   __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
   __shared__ int bfirst[PR_PAIRS], bpre[PR_PAIRS + 1];
@@ -423,22 +425,19 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       }
     }
     __syncthreads();
-    // rank the forward atoms by index
+    // forward atoms by ascending index (ordered by the neighbour pass)
     int F = 0;
     for (int s0 = 0; s0 < nn; s0 += WAVE) {
       const int sl = s0 + lane();
-      int k = -1;
-      if (sl < nn) k = lj[sl];
-      const bool fwd = k > i;
-      if (fwd) {
-        int rank = 0;
-        for (int t = 0; t < nn; ++t) {
-          const int kt = lj[t];
-          rank += (kt > i) & (kt < k);
-        }
-        fws[rank] = (uint8_t)sl;
+      F += __popcll(ballot(sl < nn && lj[sl] > i));
+    }
+    {
+      const uint8_t* fw = w.fwd + (int64_t)r * KCAP;
+      for (int f = lane(); f < F; f += WAVE) {
+        const uint8_t sl = fw[f];
+        fws[f] = sl;
+        frk[sl] = (uint8_t)f;
       }
-      F += __popcll(ballot(fwd));
     }
     const double4 ai = b.atom[i];
     const V3 xi = ld3(ai);
@@ -453,15 +452,10 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       const int avail = min(PR_PAIRS, p_hi - pb);
       int j = -1, first = F, cnt = 0;
       if (lane() < avail) {
-        j = w.pj[pb + lane()];
         // forward atoms after j: positions first .. F-1 of fws
-        int lo = 0, hi = F;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (lj[fws[mid]] <= j) lo = mid + 1; else hi = mid;
-        }
-        first = lo;
-        cnt = F - lo;
+        j = w.pj[pb + lane()];
+        first = frk[w.pslot[pb + lane()]] + 1;
+        cnt = F - first;
       }
       const int inc = wave_incl_scan(cnt);
       const int nb = __popcll(ballot(lane() < avail && inc <= PR_QCAP));   // >= 1: cnt < KCAP <= PR_QCAP
