@@ -402,7 +402,6 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ uint8_t frk[KCAP];   // forward rank of a forward slot
   __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
-  __shared__ int bfirst[PR_PAIRS], bpre[PR_PAIRS + 1];
   __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
   __shared__ uint16_t queue[PR_QCAP];
   static_assert(KCAP <= 256 && PR_PAIRS <= 32, "queue entries pack (pair << 8) | slot in 16 bits");
@@ -470,23 +469,22 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         bflag[lane()] = ((w.pflag[p] & 2) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0);
         bnp[lane()] = 0;
         bnm[lane()] = 0;
-        bfirst[lane()] = first;
-        bpre[lane()] = inc - cnt;
         for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = 0ull;
+        // the pair's triangle items (pair, forward slot after j)
+        for (int t = 0; t < cnt; ++t) queue[inc - cnt + t] = (uint16_t)((lane() << 8) | fws[first + t]);
       }
       __syncthreads();
 
-      // 1. triangle tests (_core.py:1478-1500)
+      // 1. triangle tests (_core.py:1478-1500); valid triples are compacted
+      // in place
       int nq = 0;
       for (int it0 = 0; it0 < items; it0 += WAVE) {
         const int it = it0 + lane();
         bool valid = false;
-        int a = 0, sl = 0;
+        int e = 0;
         if (it < items) {
-#pragma unroll
-          for (int step = PR_PAIRS / 2; step > 0; step >>= 1)
-            if (a + step < nb && bpre[a + step] <= it) a += step;
-          sl = fws[bfirst[a] + (it - bpre[a])];
+          e = queue[it];
+          const int a = e >> 8, sl = e & 255;
           const double4 ak = lx[sl], aj = bx[a];
           const V3 xk = ld3(ak);
           const double erk = ak.w + s.rp, erj = aj.w + s.rp;
@@ -494,7 +492,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
                   !((bflag[a] & PB_BLOCKERS) && latt[sl] <= ATTEN_BLOCKER);
         }
         const uint64_t m = ballot(valid);
-        if (valid) queue[nq + lanes_below(m)] = (uint16_t)((a << 8) | sl);
+        if (valid) queue[nq + lanes_below(m)] = (uint16_t)e;
         nq += __popcll(m);
       }
       __syncthreads();
