@@ -105,7 +105,7 @@ __global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int
       cell = 2 * (gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz) + b.amol[g];
       acell[g] = cell;
     }
-    wave_counter_add(ccount, cell, g < b.ntot);
+    wave_counter_inc(ccount, cell, g < b.ntot);
   }
 }
 

@@ -183,29 +183,46 @@ __device__ __forceinline__ double dist_to_line(V3 cen, V3 axis, V3 p) {
                u = u##_x * u##_per + blockIdx.x / u##_nx;                                       \
        u < u##_hi; u += u##_step)
 
-// Wave-aggregated counter increment: every active lane adds 1 to
-// counters[key]; lanes sharing a key are served by one atomic, in lane order.
-// Returns the counter's value before this lane's increment.  Must be called
-// by all lanes of the wave together (inactive lanes pass active = false).
-__device__ __forceinline__ int wave_counter_add(int* counters, int key, bool active) {
-  int result = 0;
+// Groups the active lanes of a wave by key: each lane gets the lowest lane
+// with its key (leader), its rank among those lanes and their number.  One
+// ballot round per distinct key.  Call with all lanes of the wave.
+__device__ __forceinline__ void wave_group_by_key(int key, bool active, int* leader, int* rank, int* cnt) {
+  *leader = lane();
+  *rank = 0;
+  *cnt = 0;
   bool todo = active;
   for (;;) {
     const uint64_t m = ballot(todo);
     if (!m) break;
-    const int leader = __builtin_ctzll(m);
-    const int lk = __builtin_amdgcn_readlane(key, leader);
-    const bool mine = todo && key == lk;
+    const int l = __builtin_ctzll(m);
+    const bool mine = todo && key == __builtin_amdgcn_readlane(key, l);
     const uint64_t sm = ballot(mine);
-    int base = 0;
-    if (lane() == leader) base = atomicAdd(&counters[lk], __popcll(sm));
-    base = __builtin_amdgcn_readlane(base, leader);
     if (mine) {
-      result = base + lanes_below(sm);
+      *leader = l;
+      *rank = lanes_below(sm);
+      *cnt = __popcll(sm);
       todo = false;
     }
   }
-  return result;
+}
+
+// Every active lane adds 1 to counters[key]; the lanes of one key are served
+// by one atomic (issued together for all keys), in lane order.  Returns the
+// counter's value before this lane's increment.  Call with all lanes.
+__device__ __forceinline__ int wave_counter_add(int* counters, int key, bool active) {
+  int leader, rank, cnt;
+  wave_group_by_key(key, active, &leader, &rank, &cnt);
+  int base = 0;
+  if (active && lane() == leader) base = atomicAdd(&counters[key], cnt);
+  base = __shfl(base, leader, WAVE);
+  return base + rank;
+}
+
+// The same without the result.
+__device__ __forceinline__ void wave_counter_inc(int* counters, int key, bool active) {
+  int leader, rank, cnt;
+  wave_group_by_key(key, active, &leader, &rank, &cnt);
+  if (active && lane() == leader) atomicAdd(&counters[key], cnt);
 }
 
 // Grid-stride loop in which every lane of a wave takes part in every

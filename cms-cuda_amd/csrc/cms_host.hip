@@ -666,11 +666,11 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
                                                                       ctx->ncellc.as<int>()));
     scan_excl(ctx->ncellc.as<int>(), ctx->ncscan.as<int>(), B, part, st);
     k_grid_offsets<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->ncscan.as<int>(), grid);
-    fill_int(ctx->ccount.as<int>(), cell_cap, 0, st);
-    fill_int(ctx->ccur.as<int>(), cell_cap, 0, st);
+    const Count cCells = dev_count(ctx->ncscan.as<int>() + B, 2);   // two slots per cell
+    fill_int(ctx->ccount.as<int>(), cCells, cell_cap, 0, st);
+    fill_int(ctx->ccur.as<int>(), cCells, cell_cap, 0, st);
     k_cell_count<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
-    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), dev_count(ctx->ncscan.as<int>() + B, 2), cell_cap, part,
-              st);
+    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cCells, cell_cap, part, st);
     k_cell_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
                                                             ctx->ccur.as<int>(), ctx->catom.as<int>(),
                                                             ctx->atom.as<double4>(), ctx->cxyz.as<double4>());

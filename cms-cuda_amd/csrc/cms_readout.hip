@@ -134,7 +134,7 @@ __global__ void k_owner_count(Count items, const int* __restrict__ list, const i
   const int n = items.get();
   CMS_WAVE_LOOP(i, n) {
     const int a = i < n ? datom[list ? list[i] : i] : 0;
-    wave_counter_add(cnt, a, i < n);
+    wave_counter_inc(cnt, a, i < n);
   }
 }
 __global__ void k_owner_scatter(Count cnt, const int* __restrict__ list, const int* __restrict__ datom,
@@ -643,7 +643,7 @@ __global__ void k_dot_cell_count(Count na, const double4* __restrict__ axyz, con
       cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
       acell[i] = cell;
     }
-    wave_counter_add(ccount, cell, i < n);
+    wave_counter_inc(ccount, cell, i < n);
   }
 }
 
