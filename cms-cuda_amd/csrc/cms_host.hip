@@ -741,8 +741,9 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     w = work(ctx);
   }
   if (est[M_R] > 0) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  // large batches count probes and torus points per neighbour row
+  const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
   if (est[M_P] > 0) {
-    const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
     if (by_row)
       TIMED("probes_count", k_probe_rows<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
     else
@@ -768,7 +769,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CKL();
   // ---- dot counts
   if (est[M_R] > 0) TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
-  if (est[M_P] > 0) {
+  if (est[M_P] > 0 && !by_row) {
     // the count pass visits only live toroid-queue pairs
     fill_int(ctx->t1cnt.as<int>(), ctx->cap[M_P], 0, st);
     fill_int(ctx->t2cnt.as<int>(), ctx->cap[M_P], 0, st);

@@ -364,6 +364,78 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
   }
 }
 
+// ------------------------------------------------------------ torus geometry
+// Shared by the toroidal passes and the per-row count pass.
+struct TorusSub {
+  V3 p, pi, pj, axis, pqi, pqj;
+  double ang1, ang2;
+  V3 y1, y2;
+  int n1, n2;
+};
+
+// Torus points of the pair (i, j) (_core.py:1863-1885): the circle frame
+// (cx, cy) about uij, the spacing tdelta of the M points, their arc length
+// ts, and the angular spacing adelta of the saddle arcs.
+struct TorusFrame {
+  V3 cx, cy;
+  double tdelta, ts, adelta;
+  int M;
+};
+__device__ __forceinline__ TorusFrame torus_frame(double ri, double rj, double rp, double rij, V3 uij,
+                                                  double dens) {
+  const double eri = ri + rp, erj = rj + rp;
+  const double rci = rij * ri / eri;
+  const double rcj = rij * rj / erj;
+  const double rb = fmax(rij - rp, 0.0);
+  const double rs = (rci + 2 * rb + rcj) * 0.25;
+  const double e = rs / rij;
+  const double edens = e * e * dens;
+  TorusFrame f;
+  circle_frame(uij, &f.cx, &f.cy);
+  f.tdelta = 1.0 / (sqrt(edens) * rij);
+  f.M = subdiv_count(f.tdelta, TWO_PI);
+  f.ts = f.M > 0 ? (rij * TWO_PI) / (double)f.M : 0.0;
+  f.adelta = 1.0 / (sqrt(dens) * rp);
+  return f;
+}
+
+// Saddle arcs of the free torus point pm (_core.py:1947-2044): the probe
+// directions towards i and j, the arc axis, the arc ends (the cusp point
+// below the circle when the probe cuts the axis between the atoms), and the
+// arc dot counts of the arcs that are on.
+__device__ __forceinline__ void torus_point_arcs(V3 pm, V3 xi, V3 xj, double eri, double erj, V3 tij, V3 uij,
+                                                 double rij, double rp, bool between, bool arc1_on, bool arc2_on,
+                                                 double adelta, TorusSub* t) {
+  const double rp2 = rp * rp;
+  t->p = pm;
+  t->pi = (xi - pm) / eri;
+  t->pj = (xj - pm) / erj;
+  t->axis = unit(cross(t->pi, t->pj));
+  const double dtq = rp2 - rij * rij;
+  if (dtq > 0.0 && between) {
+    const double dq = sqrt(dtq);
+    const V3 qij = tij - uij * dq;
+    t->pqi = (qij - pm) / rp;
+    t->pqj = v3(0.0, 0.0, 0.0);
+  } else {
+    const V3 tmp = unit(t->pi + t->pj);
+    t->pqi = tmp;
+    t->pqj = tmp;
+  }
+  const double dt1 = dot_seq(t->pqi, t->pi);
+  const double dt2 = dot_seq(t->pqj, t->pj);
+  if (fabs(dt1) < 1.0 && fabs(dt2) < 1.0) {
+    if (arc1_on) {
+      t->ang1 = arc_angle(t->pi, t->pqi, t->axis, &t->y1);
+      t->n1 = subdiv_count(adelta, t->ang1);
+    }
+    if (arc2_on) {
+      t->ang2 = arc_angle(t->pqj, t->pj, t->axis, &t->y2);
+      t->n2 = subdiv_count(adelta, t->ang2);
+    }
+  }
+}
+
 // Count pass of the probes stage, one wavefront per neighbour row: the
 // row's forward pairs (i, j) are taken in batches, and each step spreads its
 // work over the lanes across pair boundaries, so that lanes are not left
@@ -378,11 +450,16 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
 //      that can reach the pair's torus circle (the only ones a probe of the
 //      pair can collide with);
 //   4. probe positions, one candidate per lane, and their collisions with the
-//      reaching row atoms other than j and k.
-// Per pair it writes what the pair-per-wavefront emit pass relies on: the
-// probe counts of both signs, pflag bits 4 (circle inside one neighbour
-// sphere) and 8 (probes), and the access marks of the atoms of every
-// surviving probe (_core.py:1445-1643).
+//      reaching row atoms other than j and k;
+//   5. the torus points of the toroid-queue pairs whose circle is not inside
+//      a neighbour's sphere, one per lane, their collisions with the reaching
+//      row atoms, and the saddle-arc dot counts of the free ones.
+// Per pair it writes what the pair-per-wavefront probe and toroidal emit
+// passes rely on: the probe counts of both signs, pflag bits 4 (circle
+// inside one neighbour sphere) and 8 (probes), the access marks of the atoms
+// of every surviving probe (_core.py:1445-1643), the arc dot counts of both
+// atoms, whether any torus point is free and the free points among the
+// first 64 (_core.py:1847-2044).
 constexpr int PR_PAIRS = 32;            // pairs per batch (one per lane in the per-pair steps)
 constexpr int PR_QCAP = 1024;           // triangle-test items per batch, bounds the triple queue
 constexpr int PR_RELW = KCAP / WAVE;    // reach mask words per pair
@@ -392,7 +469,8 @@ constexpr int PR_RELW = KCAP / WAVE;    // reach mask words per pair
 constexpr double REACH_MARGIN = 0.01;
 
 // pair-batch flag bits
-constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32;
+constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32,
+              PB_BETWEEN = 64, PB_ARC2 = 128;
 
 __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
@@ -404,6 +482,11 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
   __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
   __shared__ uint16_t queue[PR_QCAP];
+  // toroid-queue pairs: torus circle frame, spacings, point prefix, results
+  __shared__ double4 tcx[PR_PAIRS], tcy[PR_PAIRS];
+  __shared__ double bdens[PR_PAIRS], ttd[PR_PAIRS], tad[PR_PAIRS];
+  __shared__ int tpre[PR_PAIRS], tn1[PR_PAIRS], tn2[PR_PAIRS], tlive[PR_PAIRS];
+  __shared__ unsigned long long tmk[PR_PAIRS];
   static_assert(KCAP <= 256 && PR_PAIRS <= 32, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
@@ -443,7 +526,9 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
     const double eri = ai.w + s.rp;
     const int grp = group_of_atom(b, i);
     const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
-    const bool iblock = w.atten[i] <= ATTEN_BLOCKER;
+    const int8_t iatt = w.atten[i];
+    const bool iblock = iatt <= ATTEN_BLOCKER, iarc = iatt >= ATTEN_2;
+    const double idens = b.dens[i];
     const int nw = (nn + WAVE - 1) / WAVE;
     __syncthreads();
     for (int pb = p_lo; pb < p_hi;) {
@@ -465,8 +550,12 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         bu[lane()] = w.pu[p];
         bt[lane()] = w.pt[p];
         bx[lane()] = b.atom[j];
-        const bool bij = iblock && w.atten[j] <= ATTEN_BLOCKER;
-        bflag[lane()] = ((w.pflag[p] & 2) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0);
+        bdens[lane()] = b.dens[j];
+        const int8_t jatt = w.atten[j];
+        const int8_t pf = w.pflag[p];
+        const bool bij = iblock && jatt <= ATTEN_BLOCKER;
+        bflag[lane()] = ((pf & 2) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0) | ((pf & 1) ? PB_BETWEEN : 0) |
+                        (jatt >= ATTEN_2 ? PB_ARC2 : 0);
         bnp[lane()] = 0;
         bnm[lane()] = 0;
         for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = 0ull;
@@ -536,7 +625,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         if (lane() < nb) {
           const int f = bflag[lane()];
           const bool probes = (f & PB_CAND) && !(f & PB_KILLED);
-          need = probes || (!(f & PB_NEARDEG) && (f & PB_TOR));
+          need = probes || (f & PB_TOR);
         }
         const uint64_t m = ballot(need);
         if (need) cpair[lanes_below(m)] = lane();
@@ -560,7 +649,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
                 const double D = ((R * R - aa * aa) - d2) - rho2;
                 if (R > 0.0 && D > 0.0 && 4.0 * d2 * rho2 < D * D) atomicOr(&bflag[a], PB_COVERED);
               }
-              if ((f & PB_CAND) && !(f & PB_KILLED)) {
+              if (((f & PB_CAND) && !(f & PB_KILLED)) || (f & PB_TOR)) {
                 // nearest circle point within reach: (d - rho)^2 + a^2 < reach^2
                 const double reach = rr + REACH_MARGIN;
                 const double E = ((d2 + rho2) + aa * aa) - reach * reach;
@@ -620,7 +709,73 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       }
       __syncthreads();
 
-      // 5. per-pair results
+      // 5. torus points (_core.py:1863-1940) and saddle arcs of the free ones
+      {
+        int M = 0;
+        if (lane() < nb) {
+          const int f = bflag[lane()];
+          if ((f & PB_TOR) && !(f & PB_COVERED)) {
+            const double4 pu = bu[lane()];
+            const TorusFrame tf = torus_frame(ai.w, bx[lane()].w, s.rp, pu.w, ld3(pu),
+                                              (idens + bdens[lane()]) * 0.5);
+            tcx[lane()] = make_double4(tf.cx.x, tf.cx.y, tf.cx.z, 0.0);
+            tcy[lane()] = make_double4(tf.cy.x, tf.cy.y, tf.cy.z, 0.0);
+            ttd[lane()] = tf.tdelta;
+            tad[lane()] = tf.adelta;
+            M = tf.M;
+          }
+          tn1[lane()] = 0;
+          tn2[lane()] = 0;
+          tlive[lane()] = 0;
+          tmk[lane()] = 0ull;
+        }
+        const int incM = wave_incl_scan(M);
+        if (lane() < nb) tpre[lane()] = incM - M;
+        const int T = __shfl(incM, WAVE - 1, WAVE);
+        __syncthreads();
+        for (int it0 = 0; it0 < T; it0 += WAVE) {
+          const int it = it0 + lane();
+          if (it < T) {
+            int a = 0;
+#pragma unroll
+            for (int step = PR_PAIRS / 2; step > 0; step >>= 1)
+              if (a + step < nb && tpre[a + step] <= it) a += step;
+            const int m = it - tpre[a];
+            const double4 pu = bu[a], pt = bt[a], aj = bx[a];
+            const V3 uij = ld3(pu), tij = ld3(pt);
+            const double rij = pu.w;
+            const V3 pm = subdiv_point(tij, ld3(tcx[a]), ld3(tcy[a]), rij, ttd[a], m);
+            // collisions with the row atoms (other than j) within reach of
+            // the circle (strict <, _core.py:1905-1917)
+            bool live = true;
+            for (int wd = 0; wd < nw && live; ++wd) {
+              uint64_t rm = brel[a][wd];
+              while (rm) {
+                const int t = wd * WAVE + __builtin_ctzll(rm);
+                rm &= rm - 1;
+                const double4 ax = lx[t];
+                const V3 d = pm - ld3(ax);
+                const double lim = ax.w + s.rp;
+                if (dot_alt(d, d) < lim * lim) { live = false; break; }
+              }
+            }
+            if (live) {
+              const int f = bflag[a];
+              TorusSub ts_{};
+              ts_.n1 = ts_.n2 = 0;
+              torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, tij, uij, rij, s.rp, (f & PB_BETWEEN) != 0, iarc,
+                               (f & PB_ARC2) != 0, tad[a], &ts_);
+              if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
+              if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
+              atomicOr(&tlive[a], 1);
+              if (m < WAVE) atomicOr(&tmk[a], 1ull << m);
+            }
+          }
+        }
+      }
+      __syncthreads();
+
+      // 6. per-pair results
       if (lane() < nb) {
         const int p = pb + lane();
         const int64_t vp = 2ll * g0 + (p - g0);
@@ -638,6 +793,10 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         w.prcnt[vp] = np;
         w.prcnt[vm] = nm;
         w.pflag[p] = pf;
+        w.t1cnt[p] = tn1[lane()];
+        w.t2cnt[p] = tn2[lane()];
+        w.tflag[p] = tlive[lane()] ? 1 : 0;
+        w.tmask[p] = tmk[lane()];
       }
       pb += nb;
       __syncthreads();
@@ -859,13 +1018,6 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
 // neighbours other than atom 2 (strict <), then the saddle arcs towards atom
 // 1 (owner i) and atom 2 (owner j).  Cusp pairs keep pqj = 0, which leaves
 // the atom-2 arc without an angle and hence without dots.
-struct TorusSub {
-  V3 p, pi, pj, axis, pqi, pqj;
-  double ang1, ang2;
-  V3 y1, y2;
-  int n1, n2;
-};
-
 // Dots of the saddle arcs of one chunk of torus points, spread over the
 // lanes: lane m holds torus point m's arc (start x, direction y, n dots of
 // spacing ps; inc = inclusive scan of n over the lanes), and dot d of the
@@ -934,19 +1086,10 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
       const V3 uij = ld3(pu), tij = ld3(ptt);
       const double rij = pu.w;
       const bool between = (w.pflag[p] & 1) != 0;
-      const double rci = rij * ri / eri;
-      const double rcj = rij * rj / erj;
-      const double rb = fmax(rij - s.rp, 0.0);
-      const double rs = (rci + 2 * rb + rcj) * 0.25;
-      const double e = rs / rij;
-      const double edens = e * e * dens;
-      V3 cx, cy;
-      circle_frame(uij, &cx, &cy);
-      const double tdelta = 1.0 / (sqrt(edens) * rij);
-      const int M = subdiv_count(tdelta, TWO_PI);
-      const double ts = M > 0 ? (rij * TWO_PI) / (double)M : 0.0;
-      const double adelta = 1.0 / (sqrt(dens) * s.rp);
-      const double rp2 = s.rp * s.rp;
+      const TorusFrame tf = torus_frame(ri, rj, s.rp, rij, uij, dens);
+      const V3 cx = tf.cx, cy = tf.cy;
+      const double tdelta = tf.tdelta, ts = tf.ts, adelta = tf.adelta;
+      const int M = tf.M;
       const bool arc1_on = w.atten[i] >= ATTEN_2, arc2_on = w.atten[j] >= ATTEN_2;
       // the emit pass takes the collision outcome of torus points 0..63 from
       // the count pass
@@ -1014,35 +1157,8 @@ __global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) 
               if (dot_alt(d, d) < lim * lim) { live = false; break; }
             }
           }
-          if (live) {
-            ts_.p = pm;
-            ts_.pi = (xi - pm) / eri;
-            ts_.pj = (xj - pm) / erj;
-            ts_.axis = unit(cross(ts_.pi, ts_.pj));
-            const double dtq = rp2 - rij * rij;
-            if (dtq > 0.0 && between) {
-              const double dq = sqrt(dtq);
-              const V3 qij = tij - uij * dq;
-              ts_.pqi = (qij - pm) / s.rp;
-              ts_.pqj = v3(0.0, 0.0, 0.0);
-            } else {
-              const V3 tmp = unit(ts_.pi + ts_.pj);
-              ts_.pqi = tmp;
-              ts_.pqj = tmp;
-            }
-            const double dt1 = dot_seq(ts_.pqi, ts_.pi);
-            const double dt2 = dot_seq(ts_.pqj, ts_.pj);
-            if (fabs(dt1) < 1.0 && fabs(dt2) < 1.0) {
-              if (arc1_on) {
-                ts_.ang1 = arc_angle(ts_.pi, ts_.pqi, ts_.axis, &ts_.y1);
-                ts_.n1 = subdiv_count(adelta, ts_.ang1);
-              }
-              if (arc2_on) {
-                ts_.ang2 = arc_angle(ts_.pqj, ts_.pj, ts_.axis, &ts_.y2);
-                ts_.n2 = subdiv_count(adelta, ts_.ang2);
-              }
-            }
-          }
+          if (live)
+            torus_point_arcs(pm, xi, xj, eri, erj, tij, uij, rij, s.rp, between, arc1_on, arc2_on, adelta, &ts_);
         }
         const uint64_t lm = ballot(live);
         if (!EMIT && m0 == 0 && lane() == 0) w.tmask[p] = lm;
