@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pslot, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, kcnt, kscan, dbase, dgrp;
+  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -264,7 +264,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
                   &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -492,7 +492,10 @@ static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->t2cnt.ensure(sizeof(int) * (c + 1)));
   CK(ctx->t2scan.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tflag.ensure(c + 1));
-  CK(ctx->tmask.ensure(sizeof(uint64_t) * (c + 1)));
+  CK(ctx->tmask.ensure(sizeof(uint64_t) * 2 * (c + 1)));
+  CK(ctx->tef.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->tes.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->tel.ensure(sizeof(int) * (c + 1)));
   ctx->cap[M_P] = std::max(ctx->cap[M_P], c);
   return CMS_OK;
 }
@@ -774,7 +777,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     fill_int(ctx->t1cnt.as<int>(), ctx->cap[M_P], 0, st);
     fill_int(ctx->t2cnt.as<int>(), ctx->cap[M_P], 0, st);
     fill_int8(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0, st);
-    TIMED("toroid_count", k_toroid<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+    TIMED("toroid_count", k_toroid_count<<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
   }
   if (est[M_Q] > 0) {
     if (s.rp > 0.0)
@@ -807,7 +810,16 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     }
   }
   if (est[M_R] > 0) TIMED("convex_emit", k_convex<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
-  if (est[M_P] > 0) TIMED("toroid_emit", k_toroid<true><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_P] > 0) {
+    // the pairs with toroidal dots, then their dots in batches of pairs
+    const int gP = grid_for(est[M_P], 256, 4096);
+    k_toroid_dot_pairs<<<gP, 256, 0, st>>>(cP, ctx->t1cnt.as<int>(), ctx->t2cnt.as<int>(), ctx->tef.as<int>());
+    scan_excl(ctx->tef.as<int>(), ctx->tes.as<int>(), cP, ctx->cap[M_P], part, st);
+    k_compact_count<<<gP, 256, 0, st>>>(cP, ctx->tef.as<int>(), ctx->tes.as<int>(), ctx->tel.as<int>());
+    k_copy_word<<<1, 64, 0, st>>>(ctx->tes.as<int>(), cP, meta + M_TE);
+    TIMED("toroid_emit", k_toroid_emit<<<grid_for(est[M_P] / 16 + 1, 1), 64, 0, st>>>(b, s, w, ctx->tel.as<int>(),
+                                                                                      meta + M_TE));
+  }
   if (est[M_Q] > 0 && s.rp > 0.0)
     TIMED("concave_emit", k_concave<true><<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
   CKL();

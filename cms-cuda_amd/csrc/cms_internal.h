@@ -36,7 +36,9 @@ inline Count dev_count(const int* p, int mul = 1) { return Count{0, p, mul}; }
 // Device words of the per-run count block ("meta"): clamped stage counts
 // that kernels loop over, the raw totals, and the overflow bits (one per
 // stage whose total exceeded the buffer capacity it was enqueued with).
-constexpr int M_R = 0, M_P = 1, M_Q = 2, M_D = 3, M_B = 4, M_A = 5, M_T = 6;
+// M_TE: the number of pairs with toroidal dots (a subset of the pairs, so it
+// needs no capacity of its own).
+constexpr int M_R = 0, M_P = 1, M_Q = 2, M_D = 3, M_B = 4, M_A = 5, M_T = 6, M_TE = 7;
 constexpr int M_RAW = 8, M_OVF = 15, M_WORDS = 16;
 
 struct Settings {
@@ -98,7 +100,7 @@ struct Work {
   int* t2cnt;
   int* t2scan;
   int8_t* tflag;
-  uint64_t* tmask;        // count pass: live torus points 0..63 of each pair
+  uint64_t* tmask;        // count pass: free torus points of each pair (two words)
   int* kcnt;
   int* kscan;
   int64_t* dbase;         // G*4 stage bases
@@ -121,7 +123,10 @@ template <bool EMIT> __global__ void k_pairs(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
 __global__ void k_probe_rows(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
-template <bool EMIT> __global__ void k_toroid(BatchView, Settings, Work);
+__global__ void k_toroid_count(BatchView, Settings, Work);
+__global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);
+__global__ void k_compact_count(Count, const int*, const int*, int*);
+__global__ void k_toroid_emit(BatchView, Settings, Work, const int*, const int*);
 template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);
 __global__ void k_compact_index(int, const int*, const int*, int*);
 __global__ void k_near(BatchView, Settings, Work);

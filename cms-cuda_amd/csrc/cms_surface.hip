@@ -486,7 +486,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ double4 tcx[PR_PAIRS], tcy[PR_PAIRS];
   __shared__ double bdens[PR_PAIRS], ttd[PR_PAIRS], tad[PR_PAIRS];
   __shared__ int tpre[PR_PAIRS], tn1[PR_PAIRS], tn2[PR_PAIRS], tlive[PR_PAIRS];
-  __shared__ unsigned long long tmk[PR_PAIRS];
+  __shared__ unsigned long long tmk[PR_PAIRS][2];
   static_assert(KCAP <= 256 && PR_PAIRS <= 32, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
@@ -727,7 +727,8 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           tn1[lane()] = 0;
           tn2[lane()] = 0;
           tlive[lane()] = 0;
-          tmk[lane()] = 0ull;
+          tmk[lane()][0] = 0ull;
+          tmk[lane()][1] = 0ull;
         }
         const int incM = wave_incl_scan(M);
         if (lane() < nb) tpre[lane()] = incM - M;
@@ -768,7 +769,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
               if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
               if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
               atomicOr(&tlive[a], 1);
-              if (m < WAVE) atomicOr(&tmk[a], 1ull << m);
+              atomicOr(&tmk[a][m / WAVE], 1ull << (m % WAVE));
             }
           }
         }
@@ -796,7 +797,8 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         w.t1cnt[p] = tn1[lane()];
         w.t2cnt[p] = tn2[lane()];
         w.tflag[p] = tlive[lane()] ? 1 : 0;
-        w.tmask[p] = tmk[lane()];
+        w.tmask[2 * (int64_t)p] = tmk[lane()][0];
+        w.tmask[2 * (int64_t)p + 1] = tmk[lane()][1];
       }
       pb += nb;
       __syncthreads();
@@ -1018,16 +1020,137 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
 // neighbours other than atom 2 (strict <), then the saddle arcs towards atom
 // 1 (owner i) and atom 2 (owner j).  Cusp pairs keep pqj = 0, which leaves
 // the atom-2 arc without an angle and hence without dots.
-// Dots of the saddle arcs of one chunk of torus points, spread over the
-// lanes: lane m holds torus point m's arc (start x, direction y, n dots of
-// spacing ps; inc = inclusive scan of n over the lanes), and dot d of the
-// chunk, in emission order, is dot d - (inc[src] - n[src]) of the arc of the
-// first lane src with inc[src] > d.  ctab / stab: cos / sin of
-// adelta * (lane + 1/2), shared by every arc of the pair.
-__device__ __forceinline__ void emit_arc_dots(const BatchView& b, const Settings& s, const Work& w, int64_t obase,
-                                              int n, int inc, V3 p, V3 x, V3 y, double ps, double ts, V3 tij,
-                                              V3 uij, double rij, double adelta, double ctab, double stab,
-                                              int owner, int rowner, int4 desc, int m0, int M) {
+// Count pass for small batches, one wavefront per toroid-queue pair whose
+// circle is not inside a neighbour's sphere (the counts of the other pairs
+// are zero-filled before): the torus points on the lanes, their collisions
+// with the neighbours of i that can reach the circle, and the saddle-arc
+// dot counts of the free ones; tmask keeps the free points (two words per
+// pair: MAX_SUBDIV <= 128).
+__global__ __launch_bounds__(64) void k_toroid_count(BatchView b, Settings s, Work w) {
+  __shared__ int lj[KCAP];
+  __shared__ double4 lx[KCAP];
+  __shared__ int rl[KCAP];
+  static_assert(MAX_SUBDIV <= 2 * WAVE, "two mask words per pair");
+  const int nP = *w.nP;
+  auto live = [&](int64_t q) { return (w.pflag[q] & 6) == 2; };
+  int staged = -1;   // neighbour row held in lj / lx
+  CMS_SPARSE_LOOP(p, nP, live) {
+    int tot1 = 0, tot2 = 0;
+    uint64_t lmw[2] = {0ull, 0ull};
+    const int i = w.pi[p], j = w.pj[p];
+    const int ri_ = w.rpos[i];
+    const int nn = w.nn[ri_];
+    const double4 ai = b.atom[i], aj = b.atom[j];
+    const V3 xi = ld3(ai), xj = ld3(aj);
+    const double ri = ai.w, rj = aj.w;
+    const double eri = ri + s.rp, erj = rj + s.rp;
+    const double dens = (b.dens[i] + b.dens[j]) * 0.5;
+    const double4 pu = w.pu[p], ptt = w.pt[p];
+    const V3 uij = ld3(pu), tij = ld3(ptt);
+    const double rij = pu.w;
+    const bool between = (w.pflag[p] & 1) != 0;
+    const TorusFrame tf = torus_frame(ri, rj, s.rp, rij, uij, dens);
+    const bool arc1_on = w.atten[i] >= ATTEN_2, arc2_on = w.atten[j] >= ATTEN_2;
+    if (ri_ != staged) {
+      const int* row = w.nbr + (int64_t)ri_ * KCAP;
+      for (int sl = lane(); sl < nn; sl += WAVE) {
+        const int k = row[sl];
+        lj[sl] = k;
+        lx[sl] = b.atom[k];
+      }
+      __syncthreads();
+      staged = ri_;
+    }
+    // Neighbours (other than j) that come within r + rp of the torus circle
+    // (centre tij, axis uij, radius rij): only they can collide with a torus
+    // point.  The 1e-4 A margin covers the rounding of the circle distance
+    // (cancellation in the radial term near the axis) and of the torus
+    // points, so the pruned set gives the same collision results.
+    int nrel = 0;
+    for (int s0 = 0; s0 < nn; s0 += WAVE) {
+      const int sl = s0 + lane();
+      bool rel = false;
+      if (sl < nn && lj[sl] != j) {
+        const V3 v = ld3(lx[sl]) - tij;
+        const double ax = dot_seq(v, uij);
+        const double rad = sqrt(fmax(dot_seq(v, v) - ax * ax, 0.0)) - rij;
+        const double reach = (lx[sl].w + s.rp) + 1e-4;
+        rel = rad * rad + ax * ax < reach * reach;
+      }
+      const uint64_t mk = ballot(rel);
+      if (rel) rl[nrel + lanes_below(mk)] = sl;
+      nrel += __popcll(mk);
+    }
+    __syncthreads();
+    for (int m0 = 0; m0 < tf.M; m0 += WAVE) {
+      const int m = m0 + lane();
+      TorusSub ts_{};
+      ts_.n1 = ts_.n2 = 0;
+      bool free_pt = false;
+      if (m < tf.M) {
+        const V3 pm = subdiv_point(tij, tf.cx, tf.cy, rij, tf.tdelta, m);
+        free_pt = true;
+        for (int t = 0; t < nrel; ++t) {
+          const int sl = rl[t];
+          const V3 d = pm - ld3(lx[sl]);
+          const double lim = lx[sl].w + s.rp;
+          if (dot_alt(d, d) < lim * lim) { free_pt = false; break; }
+        }
+        if (free_pt)
+          torus_point_arcs(pm, xi, xj, eri, erj, tij, uij, rij, s.rp, between, arc1_on, arc2_on, tf.adelta, &ts_);
+      }
+      const uint64_t lm = ballot(free_pt);
+      if (m0 == 0) lmw[0] = lm; else lmw[1] = lm;
+      tot1 += wave_sum_int(ts_.n1);
+      tot2 += wave_sum_int(ts_.n2);
+    }
+    if (lane() == 0) {
+      w.t1cnt[p] = tot1;
+      w.t2cnt[p] = tot2;
+      w.tflag[p] = (lmw[0] | lmw[1]) ? 1 : 0;
+      w.tmask[2 * p] = lmw[0];
+      w.tmask[2 * p + 1] = lmw[1];
+    }
+    __syncthreads();
+  }
+}
+
+// Pairs with toroidal dots: flag, then a scan and a compaction give plist.
+__global__ void k_toroid_dot_pairs(Count nP, const int* __restrict__ t1cnt, const int* __restrict__ t2cnt,
+                                   int* __restrict__ flag) {
+  const int n = nP.get();
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+    flag[p] = (t1cnt[p] | t2cnt[p]) != 0 ? 1 : 0;
+}
+__global__ void k_compact_count(Count n, const int* __restrict__ flag, const int* __restrict__ scan,
+                                int* __restrict__ out) {
+  const int c = n.get();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+    if (flag[i]) out[scan[i]] = i;
+}
+
+// Emit pass of the toroidal stage (_core.py:1940-2061) over the pairs with
+// toroidal dots (plist, count at *nlist), TE_PAIRS consecutive pairs per
+// wavefront: the free torus points of the batch (the count pass's masks) are
+// spread over the lanes, and the dots of their saddle arcs likewise.  Dot k
+// of a point's arc goes to the pair's offset plus the arc dots of the pair's
+// earlier points plus k -- the order of the reference (pair, torus point,
+// arc point).
+constexpr int TE_PAIRS = 16;
+struct ToroidBatch {
+  int i[TE_PAIRS], j[TE_PAIRS], ri[TE_PAIRS], rj[TE_PAIRS], flag[TE_PAIRS], M[TE_PAIRS];
+  int tot1[TE_PAIRS], tot2[TE_PAIRS];
+  double4 xi[TE_PAIRS], xj[TE_PAIRS], u[TE_PAIRS], t[TE_PAIRS], cx[TE_PAIRS], cy[TE_PAIRS];
+  double tdelta[TE_PAIRS], ts[TE_PAIRS], adelta[TE_PAIRS];
+  int64_t ob1[TE_PAIRS], ob2[TE_PAIRS];
+};
+
+// Dots of one arc per lane (n dots, inclusive scan inc over the lanes; the
+// point lies at offset off of its pair a's arc list), spread over the lanes:
+// dot d belongs to the first lane src with inc[src] > d.
+__device__ __forceinline__ void emit_arc_dots(const BatchView& b, const Settings& s, const Work& w,
+                                              const ToroidBatch& tb, int arc, int n, int inc, V3 p, V3 x, V3 y,
+                                              double ps, int off, int a, int m) {
   const int T = __shfl(inc, WAVE - 1, WAVE);
   for (int d0 = 0; d0 < T; d0 += WAVE) {
     const int d = d0 + lane();
@@ -1041,148 +1164,110 @@ __device__ __forceinline__ void emit_arc_dots(const BatchView& b, const Settings
     const V3 sx = v3(__shfl(x.x, src, WAVE), __shfl(x.y, src, WAVE), __shfl(x.z, src, WAVE));
     const V3 sy = v3(__shfl(y.x, src, WAVE), __shfl(y.y, src, WAVE), __shfl(y.z, src, WAVE));
     const double sps = __shfl(ps, src, WAVE);
-    const int kk = k < WAVE ? k : 0;
-    double ck = __shfl(ctab, kk, WAVE), sk = __shfl(stab, kk, WAVE);
+    const int soff = __shfl(off, src, WAVE), sa = __shfl(a, src, WAVE), sm = __shfl(m, src, WAVE);
     if (d < T) {
-      if (k >= WAVE) {
-        const double a = adelta * ((double)k + 0.5);
-        ck = cos(a);
-        sk = sin(a);
-      }
-      const V3 pt = subdiv_point_cs(sp, sx, sy, s.rp, ck, sk);
-      const double dist = dist_to_line(tij, uij, pt);
-      const double area = sps * ts * dist / rij;
-      store_dot(b, s, w, obase + d, pt, sp, area, owner, rowner, 2, desc, make_int4(m0 + src, k, M, sn));
+      const double4 pu = tb.u[sa], pt = tb.t[sa];
+      const V3 pt_ = subdiv_point(sp, sx, sy, s.rp, tb.adelta[sa], k);
+      const double dist = dist_to_line(ld3(pt), ld3(pu), pt_);
+      const double area = sps * tb.ts[sa] * dist / pu.w;
+      const int owner = arc == 1 ? tb.i[sa] : tb.j[sa];
+      const int rowner = arc == 1 ? tb.ri[sa] : tb.rj[sa];
+      const int64_t ob = arc == 1 ? tb.ob1[sa] : tb.ob2[sa];
+      store_dot(b, s, w, ob + soff + k, pt_, sp, area, owner, rowner, 2,
+                make_int4(arc == 1 ? DK_ARC1 : DK_ARC2, tb.i[sa], tb.j[sa], -1), make_int4(sm, k, tb.M[sa], sn));
     }
   }
 }
 
-template <bool EMIT>
-__global__ __launch_bounds__(64) void k_toroid(BatchView b, Settings s, Work w) {
-  __shared__ int lj[KCAP];
-  __shared__ double4 lx[KCAP];
-  __shared__ int rl[KCAP];
-  const int nP = *w.nP;
-  // count pass: pairs of the toroid queue whose circle is not inside a
-  // neighbour's sphere (the counts of the others are zero-filled before);
-  // emit pass: pairs with toroidal dots
-  auto live = [&](int64_t q) {
-    return EMIT ? (w.t1cnt[q] != 0 || w.t2cnt[q] != 0) : (w.pflag[q] & 6) == 2;
-  };
-  int staged = -1;   // neighbour row held in lj / lx
-  CMS_SPARSE_LOOP(p, nP, live) {
-    int tot1 = 0, tot2 = 0;
-    bool any_live = false;
-    if ((w.pflag[p] & 6) == 2) {   // toroid queue, circle not covered
+__global__ __launch_bounds__(64) void k_toroid_emit(BatchView b, Settings s, Work w, const int* __restrict__ plist,
+                                                    const int* __restrict__ nlist) {
+  __shared__ ToroidBatch tb;
+  __shared__ uint16_t items[TE_PAIRS * 2 * WAVE];
+  const int nE = *nlist;
+  const int nbatch = (nE + TE_PAIRS - 1) / TE_PAIRS;
+  CMS_UNIT_LOOP(bt, nbatch) {
+    const int e0 = (int)bt * TE_PAIRS;
+    const int np = min(TE_PAIRS, nE - e0);
+    int lc = 0;
+    uint64_t mw0 = 0ull, mw1 = 0ull;
+    if (lane() < np) {
+      const int a = lane();
+      const int p = plist[e0 + a];
       const int i = w.pi[p], j = w.pj[p];
-      const int ri_ = w.rpos[i], rj_ = w.rpos[j];
-      const int nn = w.nn[ri_];
-      const double4 ai = b.atom[i], aj = b.atom[j];
-      const V3 xi = ld3(ai), xj = ld3(aj);
-      const double ri = ai.w, rj = aj.w;
-      const double eri = ri + s.rp, erj = rj + s.rp;
-      const double dens = (b.dens[i] + b.dens[j]) * 0.5;
-      const double4 pu = w.pu[p], ptt = w.pt[p];
-      const V3 uij = ld3(pu), tij = ld3(ptt);
-      const double rij = pu.w;
-      const bool between = (w.pflag[p] & 1) != 0;
-      const TorusFrame tf = torus_frame(ri, rj, s.rp, rij, uij, dens);
-      const V3 cx = tf.cx, cy = tf.cy;
-      const double tdelta = tf.tdelta, ts = tf.ts, adelta = tf.adelta;
-      const int M = tf.M;
-      const bool arc1_on = w.atten[i] >= ATTEN_2, arc2_on = w.atten[j] >= ATTEN_2;
-      // the emit pass takes the collision outcome of torus points 0..63 from
-      // the count pass
-      const bool masked = EMIT && M <= WAVE;
-      int nrel = 0;
-      if (!masked) {
-        if (ri_ != staged) {
-          const int* row = w.nbr + (int64_t)ri_ * KCAP;
-          for (int sl = lane(); sl < nn; sl += WAVE) {
-            const int k = row[sl];
-            lj[sl] = k;
-            lx[sl] = b.atom[k];
-          }
-          __syncthreads();
-          staged = ri_;
-        }
-        // Neighbours (other than j) that come within r + rp of the torus
-        // circle (centre tij, axis uij, radius rij): only they can collide
-        // with a torus point.  The 1e-4 A margin covers the rounding of the
-        // circle distance (cancellation in the radial term near the axis)
-        // and of the torus points, so the pruned set gives the same
-        // collision results.
-        for (int s0 = 0; s0 < nn; s0 += WAVE) {
-          const int sl = s0 + lane();
-          bool rel = false;
-          if (sl < nn && lj[sl] != j) {
-            const V3 v = ld3(lx[sl]) - tij;
-            const double ax = dot_seq(v, uij);
-            const double rad = sqrt(fmax(dot_seq(v, v) - ax * ax, 0.0)) - rij;
-            const double reach = (lx[sl].w + s.rp) + 1e-4;
-            rel = rad * rad + ax * ax < reach * reach;
-          }
-          const uint64_t mk = ballot(rel);
-          if (rel) rl[nrel + lanes_below(mk)] = sl;
-          nrel += __popcll(mk);
-        }
-        __syncthreads();
-      }
-      const uint64_t lmask = masked ? w.tmask[p] : 0ull;
-      int64_t ob1 = 0, ob2 = 0;
-      double ctab = 0.0, stab = 0.0;
-      if (EMIT) {
-        const int grp = group_of_atom(b, i);
-        ob1 = w.dbase[grp * 4 + 1] + w.t1scan[p];
-        ob2 = w.dbase[grp * 4 + 2] + w.t2scan[p];
-        const double a = adelta * ((double)lane() + 0.5);
-        ctab = cos(a);
-        stab = sin(a);
-      }
-      for (int m0 = 0; m0 < M; m0 += WAVE) {
-        const int m = m0 + lane();
-        TorusSub ts_{};
-        ts_.n1 = ts_.n2 = 0;
-        bool live = false;
-        if (m < M) {
-          const V3 pm = subdiv_point(tij, cx, cy, rij, tdelta, m);
-          if (masked) {
-            live = (lmask >> lane()) & 1ull;
-          } else {
-            live = true;
-            for (int t = 0; t < nrel; ++t) {
-              const int sl = rl[t];
-              const V3 d = pm - ld3(lx[sl]);
-              const double lim = lx[sl].w + s.rp;
-              if (dot_alt(d, d) < lim * lim) { live = false; break; }
-            }
-          }
-          if (live)
-            torus_point_arcs(pm, xi, xj, eri, erj, tij, uij, rij, s.rp, between, arc1_on, arc2_on, adelta, &ts_);
-        }
-        const uint64_t lm = ballot(live);
-        if (!EMIT && m0 == 0 && lane() == 0) w.tmask[p] = lm;
-        any_live = any_live || lm != 0ull;
-        const int inc1 = wave_incl_scan(ts_.n1);
-        const int inc2 = wave_incl_scan(ts_.n2);
-        if (EMIT) {
-          const double ps1 = ts_.n1 > 0 ? (s.rp * ts_.ang1) / (double)ts_.n1 : 0.0;
-          const double ps2 = ts_.n2 > 0 ? (s.rp * ts_.ang2) / (double)ts_.n2 : 0.0;
-          emit_arc_dots(b, s, w, ob1 + tot1, ts_.n1, inc1, ts_.p, ts_.pi, ts_.y1, ps1, ts, tij, uij, rij, adelta,
-                        ctab, stab, i, ri_, make_int4(DK_ARC1, i, j, -1), m0, M);
-          emit_arc_dots(b, s, w, ob2 + tot2, ts_.n2, inc2, ts_.p, ts_.pqj, ts_.y2, ps2, ts, tij, uij, rij, adelta,
-                        ctab, stab, j, rj_, make_int4(DK_ARC2, i, j, -1), m0, M);
-        }
-        tot1 += __shfl(inc1, WAVE - 1, WAVE);
-        tot2 += __shfl(inc2, WAVE - 1, WAVE);
-      }
+      const double4 ai = b.atom[i], aj = b.atom[j], pu = w.pu[p];
+      const TorusFrame tf = torus_frame(ai.w, aj.w, s.rp, pu.w, ld3(pu), (b.dens[i] + b.dens[j]) * 0.5);
+      tb.i[a] = i;
+      tb.j[a] = j;
+      tb.ri[a] = w.rpos[i];
+      tb.rj[a] = w.rpos[j];
+      tb.flag[a] = ((w.pflag[p] & 1) ? 1 : 0) | (w.atten[i] >= ATTEN_2 ? 2 : 0) | (w.atten[j] >= ATTEN_2 ? 4 : 0);
+      tb.M[a] = tf.M;
+      tb.tot1[a] = 0;
+      tb.tot2[a] = 0;
+      tb.xi[a] = ai;
+      tb.xj[a] = aj;
+      tb.u[a] = pu;
+      tb.t[a] = w.pt[p];
+      tb.cx[a] = make_double4(tf.cx.x, tf.cx.y, tf.cx.z, 0.0);
+      tb.cy[a] = make_double4(tf.cy.x, tf.cy.y, tf.cy.z, 0.0);
+      tb.tdelta[a] = tf.tdelta;
+      tb.ts[a] = tf.ts;
+      tb.adelta[a] = tf.adelta;
+      const int grp = group_of_atom(b, i);
+      tb.ob1[a] = w.dbase[grp * 4 + 1] + w.t1scan[p];
+      tb.ob2[a] = w.dbase[grp * 4 + 2] + w.t2scan[p];
+      mw0 = w.tmask[2 * (int64_t)p];
+      mw1 = w.tmask[2 * (int64_t)p + 1];
+      lc = __popcll(mw0) + __popcll(mw1);
     }
-    if (!EMIT && lane() == 0) {
-      w.t1cnt[p] = tot1;
-      w.t2cnt[p] = tot2;
-      w.tflag[p] = any_live ? 1 : 0;
+    const int inc = wave_incl_scan(lc);
+    const int L = __shfl(inc, WAVE - 1, WAVE);
+    if (lane() < np) {
+      int o = inc - lc;
+      for (; mw0; mw0 &= mw0 - 1) items[o++] = (uint16_t)((lane() << 8) | __builtin_ctzll(mw0));
+      for (; mw1; mw1 &= mw1 - 1) items[o++] = (uint16_t)((lane() << 8) | (WAVE + __builtin_ctzll(mw1)));
     }
     __syncthreads();
+    for (int q0 = 0; q0 < L; q0 += WAVE) {
+      const int q = q0 + lane();
+      int a = -1, m = 0;
+      TorusSub t{};
+      t.n1 = t.n2 = 0;
+      if (q < L) {
+        const int e = items[q];
+        a = e >> 8;
+        m = e & 255;
+        const double4 pu = tb.u[a], pt = tb.t[a], ai = tb.xi[a], aj = tb.xj[a];
+        const int f = tb.flag[a];
+        const V3 pm = subdiv_point(ld3(pt), ld3(tb.cx[a]), ld3(tb.cy[a]), pu.w, tb.tdelta[a], m);
+        torus_point_arcs(pm, ld3(ai), ld3(aj), ai.w + s.rp, aj.w + s.rp, ld3(pt), ld3(pu), pu.w, s.rp,
+                         (f & 1) != 0, (f & 2) != 0, (f & 4) != 0, tb.adelta[a], &t);
+      }
+      // lanes of one pair are consecutive: per-lane offsets within the pair
+      const int aprev = __shfl_up(a, 1, WAVE), anext = __shfl_down(a, 1, WAVE);
+      const bool run_start = q < L && (lane() == 0 || aprev != a);
+      const bool run_end = q < L && (lane() == WAVE - 1 || q + 1 >= L || anext != a);
+      const uint64_t starts = ballot(run_start);
+      const uint64_t below = lane() == WAVE - 1 ? ~0ull : ((2ull << lane()) - 1);
+      const int first = (starts & below) ? 63 - __builtin_clzll(starts & below) : 0;
+      const int inc1 = wave_incl_scan(t.n1), inc2 = wave_incl_scan(t.n2);
+      const int b1 = __shfl(inc1 - t.n1, first, WAVE), b2 = __shfl(inc2 - t.n2, first, WAVE);
+      int off1 = 0, off2 = 0;
+      if (q < L) {
+        off1 = tb.tot1[a] + (inc1 - t.n1) - b1;
+        off2 = tb.tot2[a] + (inc2 - t.n2) - b2;
+      }
+      const double ps1 = t.n1 > 0 ? (s.rp * t.ang1) / (double)t.n1 : 0.0;
+      const double ps2 = t.n2 > 0 ? (s.rp * t.ang2) / (double)t.n2 : 0.0;
+      emit_arc_dots(b, s, w, tb, 1, t.n1, inc1, t.p, t.pi, t.y1, ps1, off1, a, m);
+      emit_arc_dots(b, s, w, tb, 2, t.n2, inc2, t.p, t.pqj, t.y2, ps2, off2, a, m);
+      __syncthreads();
+      if (run_end) {
+        tb.tot1[a] += inc1 - b1;
+        tb.tot2[a] += inc2 - b2;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -1319,8 +1404,6 @@ template __global__ void k_probes<false>(BatchView, Settings, Work);
 template __global__ void k_probes<true>(BatchView, Settings, Work);
 template __global__ void k_convex<false>(BatchView, Settings, Work);
 template __global__ void k_convex<true>(BatchView, Settings, Work);
-template __global__ void k_toroid<false>(BatchView, Settings, Work);
-template __global__ void k_toroid<true>(BatchView, Settings, Work);
 template __global__ void k_concave<false>(BatchView, Settings, Work);
 template __global__ void k_concave<true>(BatchView, Settings, Work);
 
