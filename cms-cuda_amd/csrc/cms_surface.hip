@@ -479,14 +479,16 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ uint8_t fws[KCAP];   // forward slots (atom index > i) by ascending atom index
   __shared__ uint8_t frk[KCAP];   // forward rank of a forward slot
   __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
+  // cpair (step 3) and tpre (step 5) share their words
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
+  int* const tpre = cpair;
   __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
   __shared__ uint16_t queue[PR_QCAP];
   // toroid-queue pairs: torus circle frame, spacings, point prefix, results
   __shared__ double4 tcx[PR_PAIRS], tcy[PR_PAIRS];
   __shared__ double bdens[PR_PAIRS], ttd[PR_PAIRS], tad[PR_PAIRS];
-  __shared__ int tpre[PR_PAIRS], tn1[PR_PAIRS], tn2[PR_PAIRS], tlive[PR_PAIRS];
-  __shared__ unsigned long long tmk[PR_PAIRS][2];
+  __shared__ int tn1[PR_PAIRS], tn2[PR_PAIRS];
+  __shared__ unsigned long long tmk0[PR_PAIRS], tmk1[PR_PAIRS];
   static_assert(KCAP <= 256 && PR_PAIRS <= 32, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
@@ -726,9 +728,8 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           }
           tn1[lane()] = 0;
           tn2[lane()] = 0;
-          tlive[lane()] = 0;
-          tmk[lane()][0] = 0ull;
-          tmk[lane()][1] = 0ull;
+          tmk0[lane()] = 0ull;
+          tmk1[lane()] = 0ull;
         }
         const int incM = wave_incl_scan(M);
         if (lane() < nb) tpre[lane()] = incM - M;
@@ -768,8 +769,8 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
                                (f & PB_ARC2) != 0, tad[a], &ts_);
               if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
               if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
-              atomicOr(&tlive[a], 1);
-              atomicOr(&tmk[a][m / WAVE], 1ull << (m % WAVE));
+              if (m < WAVE) atomicOr(&tmk0[a], 1ull << m);
+              else atomicOr(&tmk1[a], 1ull << (m - WAVE));
             }
           }
         }
@@ -796,9 +797,9 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         w.pflag[p] = pf;
         w.t1cnt[p] = tn1[lane()];
         w.t2cnt[p] = tn2[lane()];
-        w.tflag[p] = tlive[lane()] ? 1 : 0;
-        w.tmask[2 * (int64_t)p] = tmk[lane()][0];
-        w.tmask[2 * (int64_t)p + 1] = tmk[lane()][1];
+        w.tflag[p] = (tmk0[lane()] | tmk1[lane()]) ? 1 : 0;
+        w.tmask[2 * (int64_t)p] = tmk0[lane()];
+        w.tmask[2 * (int64_t)p + 1] = tmk1[lane()];
       }
       pb += nb;
       __syncthreads();
