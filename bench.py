@@ -195,7 +195,7 @@ def kernel_models(counts, sc, nb, nt, fp32):
     R = counts[:, 8].astype(np.float64)
     En = counts[:, 12].astype(np.float64)
     Eb = counts[:, 13].astype(np.float64)
-    Pn = counts[:, 14].astype(np.float64)
+    Np = counts[:, 9].astype(np.float64)
     B0, B1 = counts[:, 2].astype(np.float64), counts[:, 3].astype(np.float64)
     A0, A1 = counts[:, 0] - counts[:, 2], counts[:, 1] - counts[:, 3]
     T0, T1 = sc[:, 13], sc[:, 21]
@@ -205,8 +205,9 @@ def kernel_models(counts, sc, nb, nt, fp32):
                       "8 flops per binder-target atom pair (dense minimum distance)"),
         "neighbors": ("hbm", float(np.sum(34.0 * R + 12.0 * En + 4.0 * Eb)), "GB/s", PEAK_HBM_GBS,
                       "34 B per row atom + 12 B per neighbour entry + 4 B per buried entry"),
-        "probes_count": ("mfma", float(np.sum(16.0 * Pn)), "TFLOP/s", PEAK_FP64_TFLOPS,
-                         "16 flops per (pair, row slot): the two triangle distances of every triple"),
+        "probes_count": ("hbm", float(np.sum(36.0 * En + 108.0 * Np)), "GB/s", PEAK_HBM_GBS,
+                         "36 B per neighbour entry (index + coordinates, rows read once) + 108 B per torus "
+                         "pair (74 B record in, 34 B counts, flags and masks out)"),
     }
     # three searches per step (CMS, SC trim, SC partners) share the name
     if fp32:

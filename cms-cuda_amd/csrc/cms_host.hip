@@ -1012,9 +1012,13 @@ static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* 
   scan_excl(ctx->ocnt.as<int>(), ctx->ostart.as<int>(), n, ctx->part.as<int>(), st);
   k_owner_scatter<<<ig, 256, 0, st>>>(nitems, list, ctx->datom.as<int>(), ctx->ostart.as<int>(),
                                       ctx->ocur.as<int>(), ctx->operm.as<int>());
-  if (n > 0)
-    TIMED("owner_sums", k_owner_sums<<<grid_for(n, 1, 16384), 64, 0, st>>>(n, ctx->ostart.as<int>(), ctx->operm.as<int>(), val,
-                                                          vstride, per_atom));
+  if (n > 0) {
+    k_owner_small<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->ostart.as<int>(), ctx->operm.as<int>(), val, vstride,
+                                                           per_atom);
+    TIMED("owner_sums", k_owner_sums<<<grid_for(n, 64, 16384), 64, 0, st>>>(n, ctx->ostart.as<int>(),
+                                                                           ctx->operm.as<int>(), val, vstride,
+                                                                           per_atom));
+  }
   CKL();
   // np.sum of the per-atom values of every molecule group; gstride 1 keeps
   // all 2B group totals, gstride 2 only the molecule-0 ones (max-CMS)

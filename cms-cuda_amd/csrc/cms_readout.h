@@ -21,6 +21,8 @@ __global__ void k_cms_weights(Count n, const double4* bxyz, const double* d2, do
 __global__ void k_owner_count(Count n, const int* list, const int* datom, int* cnt);
 __global__ void k_owner_scatter(Count n, const int* list, const int* datom, const int* start, int* cur,
                                 int* perm);
+__global__ void k_owner_small(int natoms, const int* start, const int* perm, const double* val, int64_t vstride,
+                              double* out);
 __global__ void k_owner_sums(int natoms, const int* start, int* perm, const double* val,
                              int64_t vstride, double* out);
 __global__ void k_group_np_sum(int G, const int* start, const double* vals, int64_t stride, double* out);

@@ -150,18 +150,24 @@ __global__ void k_owner_scatter(Count cnt, const int* __restrict__ list, const i
 // One wavefront per atom: the atom's item indices (scattered in arbitrary
 // order by k_owner_scatter) are sorted in LDS with a bitonic network, their
 // values gathered in that order and summed left to right by lane 0.
+// Atoms with no item (0.0) or one (0.0 + value), one thread each.
+__global__ void k_owner_small(int natoms, const int* __restrict__ start, const int* __restrict__ perm,
+                              const double* __restrict__ val, int64_t vstride, double* __restrict__ out) {
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < natoms; a += gridDim.x * blockDim.x) {
+    const int lo = start[a], n = start[a + 1] - lo;
+    if (n <= 1) out[a] = n == 1 ? 0.0 + val[(int64_t)perm[lo] * vstride] : 0.0;
+  }
+}
 constexpr int OWN_CAP = 1024;
 __global__ __launch_bounds__(64) void k_owner_sums(int natoms, const int* __restrict__ start,
                                                    int* __restrict__ perm, const double* __restrict__ val,
                                                    int64_t vstride, double* __restrict__ out) {
   __shared__ int key[OWN_CAP];
   __shared__ double v[OWN_CAP];
-  for (int a = blockIdx.x; a < natoms; a += gridDim.x) {
+  // atoms with at most one item are summed by k_owner_small
+  auto several = [&](int64_t a) { return start[a + 1] - start[a] >= 2; };
+  CMS_SPARSE_LOOP(a, natoms, several) {
     const int lo = start[a], n = start[a + 1] - lo;
-    if (n <= 1) {
-      if (lane() == 0) out[a] = n == 1 ? 0.0 + val[(int64_t)perm[lo] * vstride] : 0.0;
-      continue;
-    }
     if (n > OWN_CAP) {
       // very long segment: in-place insertion sort by one lane
       if (lane() == 0) {
