@@ -771,7 +771,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   if (est[M_Q] > 0) TIMED("near", k_near<<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
   CKL();
   // ---- dot counts
-  if (est[M_R] > 0) TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_R] > 0) {
+    fill_int(ctx->ccnt.as<int>(), cR, ctx->cap[M_R], 0, st);
+    TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  }
   if (est[M_P] > 0 && !by_row) {
     // the count pass visits only live toroid-queue pairs
     fill_int(ctx->t1cnt.as<int>(), ctx->cap[M_P], 0, st);

@@ -891,12 +891,18 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
   __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
   __shared__ int lat_start[MAX_SUBDIV + 1];
   const int nR = *w.nR;
-  auto live = [&](int64_t q) { return !EMIT || w.ccnt[q] != 0; };
+  // count pass: accessible non-blocker atoms with neighbours (the counts of
+  // the other rows are zero-filled before); emit pass: rows with dots
+  auto live = [&](int64_t q) {
+    if (EMIT) return w.ccnt[q] != 0;
+    const int g = w.ratom[q];
+    return w.nn[q] > 0 && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER;
+  };
   CMS_SPARSE_LOOP(r, nR, live) {
     const int g = w.ratom[r];
     const int nn = w.nn[r];
     int total = 0;
-    if (nn > 0 && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER) {
+    {
       const int* row = w.nbr + (int64_t)r * KCAP;
       const double4 ai = b.atom[g];
       const V3 xi = ld3(ai);
