@@ -77,7 +77,7 @@ struct cms_ctx {
   DBuf atten, access, status, coinc, flag, nbscan, nbl, need, rpos, ratom, part;
   DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom, cxyz;
   DBuf nn, nbr, nb, bur, nbmax, fwd;
-  DBuf pcnt, pscan, pi, pj, pslot, pu, pt, pflag, rgrp, pgrp;
+  DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
   DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
@@ -215,7 +215,6 @@ static Work work(cms_ctx* ctx) {
   w.nP = ctx->meta.as<int>() + M_P;
   w.pi = ctx->pi.as<int>();
   w.pj = ctx->pj.as<int>();
-  w.pslot = ctx->pslot.as<uint8_t>();
   w.pu = ctx->pu.as<double4>();
   w.pt = ctx->pt.as<double4>();
   w.pflag = ctx->pflag.as<int8_t>();
@@ -260,7 +259,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
                   &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
-                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pslot, &ctx->pu, &ctx->pt, &ctx->pflag,
+                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
@@ -481,7 +480,6 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
 static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->pi.ensure(sizeof(int) * (c + 1)));
   CK(ctx->pj.ensure(sizeof(int) * (c + 1)));
-  CK(ctx->pslot.ensure(c + 1));
   CK(ctx->pu.ensure(sizeof(double4) * (c + 1)));
   CK(ctx->pt.ensure(sizeof(double4) * (c + 1)));
   CK(ctx->pflag.ensure(c + 1));
@@ -743,9 +741,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     if ((r = size_pairs(ctx, headroom(v)))) return r;
     w = work(ctx);
   }
-  if (est[M_R] > 0) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
-  // large batches count probes and torus points per neighbour row
+  // large batches count probes and torus points per neighbour row, and that
+  // pass writes the pair records itself
   const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
+  if (est[M_R] > 0 && !by_row) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
   if (est[M_P] > 0) {
     if (by_row)
       TIMED("probes_count", k_probe_rows<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));

@@ -76,7 +76,6 @@ struct Work {
   const int* nP;
   int* pi;
   int* pj;
-  uint8_t* pslot;          // row slot of j
   double4* pu;
   double4* pt;
   int8_t* pflag;          // bit0 between, bit1 toroid queue

@@ -26,6 +26,37 @@ __global__ void k_compact_index(int n, const int* __restrict__ flag, const int* 
 // ------------------------------------------------------------ pairs
 // _core.py:1351-1442.  Forward pairs (j > i) of atoms with neighbours, torus
 // centre/radius; the single-neighbour shortcut marks access and emits none.
+
+// Torus of the pair (i, j): axis u, centre t, radius rij, whether the centre
+// lies between the atoms; false when the probe cannot touch both spheres.
+__device__ __forceinline__ bool pair_torus(V3 xi, double ri, V3 xj, double rj, double rp, V3* u, V3* t,
+                                           double* rij, bool* between) {
+  const double eri = ri + rp, erj = rj + rp;
+  const V3 diff = xj - xi;
+  const double dsq = dot_alt(diff, diff);
+  const double dij = sqrt(dsq);
+  const double asymm = (eri * eri - erj * erj) / dij;
+  *between = fabs(asymm) < dij;
+  *u = diff / dij;
+  *t = (xi + xj) * 0.5 + *u * (asymm * 0.5);
+  const double far_sq = (eri + erj) * (eri + erj) - dsq;
+  const double cont_sq = dsq - (ri - rj) * (ri - rj);
+  const bool ok = (far_sq > 0.0) && (cont_sq > 0.0);
+  *rij = ok ? (0.5 * sqrt(far_sq)) * sqrt(cont_sq) / dij : 0.0;
+  return ok;
+}
+
+// A pair matters only for the toroid queue (a non-blocker end) or for probes
+// (some atom of a triple i, j, k > j is a non-blocker); the toroid queue takes
+// pairs with a non-blocker first atom, or a non-blocker second atom when the
+// probe has a radius.
+__device__ __forceinline__ bool pair_relevant(int8_t atti, int8_t attj, int nbmax, int j) {
+  return atti > ATTEN_BLOCKER || attj > ATTEN_BLOCKER || nbmax > j;
+}
+__device__ __forceinline__ bool pair_toroid(int8_t atti, int8_t attj, double rp) {
+  return (atti > ATTEN_BLOCKER) || ((attj > ATTEN_BLOCKER) && (rp > 0.0));
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
   const int nR = *w.nR;
@@ -36,7 +67,7 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
     const int nn = w.nn[r];
     const double4 ai = b.atom[g];
     const V3 xi = ld3(ai);
-    const double ri = ai.w, eri = ri + s.rp;
+    const double ri = ai.w;
     const int* row = w.nbr + (int64_t)r * KCAP;
     const int nbmax = w.nbmax[r];
     int base = EMIT ? w.pscan[r] : 0;
@@ -53,22 +84,8 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
         j = row[sl];
         if (j > g) {
           const double4 aj = b.atom[j];
-          const V3 xj = ld3(aj);
-          const double rj = aj.w, erj = rj + s.rp;
-          const V3 diff = xj - xi;
-          const double dsq = dot_alt(diff, diff);
-          const double dij = sqrt(dsq);
-          const double asymm = (eri * eri - erj * erj) / dij;
-          between = fabs(asymm) < dij;
-          u = diff / dij;
-          t = (xi + xj) * 0.5 + u * (asymm * 0.5);
-          const double far_sq = (eri + erj) * (eri + erj) - dsq;
-          const double cont_sq = dsq - (ri - rj) * (ri - rj);
-          ok = (far_sq > 0.0) && (cont_sq > 0.0);
-          if (ok) rij = (0.5 * sqrt(far_sq)) * sqrt(cont_sq) / dij;
-          // a pair matters only for the toroid queue (a non-blocker end) or
-          // for probes (some atom of a triple i, j, k > j is a non-blocker)
-          relevant = ok && (w.atten[g] > ATTEN_BLOCKER || w.atten[j] > ATTEN_BLOCKER || nbmax > j);
+          ok = pair_torus(xi, ri, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between);
+          relevant = ok && pair_relevant(w.atten[g], w.atten[j], nbmax, j);
         }
       }
       if (nn <= 1) {
@@ -87,11 +104,9 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
         const int o = base + total + lanes_below(mk);
         w.pi[o] = g;
         w.pj[o] = j;
-        w.pslot[o] = (uint8_t)sl;
         w.pu[o] = make_double4(u.x, u.y, u.z, rij);
         w.pt[o] = make_double4(t.x, t.y, t.z, 0.0);
-        const int8_t ai8 = w.atten[g], aj8 = w.atten[j];
-        const bool tor = (ai8 > ATTEN_BLOCKER) || ((aj8 > ATTEN_BLOCKER) && (s.rp > 0.0));
+        const bool tor = pair_toroid(w.atten[g], w.atten[j], s.rp);
         w.pflag[o] = (int8_t)((between ? 1 : 0) | (tor ? 2 : 0));
       }
       total += __popcll(mk);
@@ -461,7 +476,7 @@ __device__ __forceinline__ void torus_point_arcs(V3 pm, V3 xi, V3 xj, double eri
 // atoms, whether any torus point is free and the free points among the
 // first 64 (_core.py:1847-2044).
 constexpr int PR_PAIRS = 32;            // pairs per batch (one per lane in the per-pair steps)
-constexpr int PR_QCAP = 1024;           // triangle-test items per batch, bounds the triple queue
+constexpr int PR_QCAP = 896;            // triangle-test items per batch, bounds the triple queue
 constexpr int PR_RELW = KCAP / WAVE;    // reach mask words per pair
 // Reach margin: a computed probe of a non-near-degenerate triple lies within
 // 1e-4 A of its pair's torus circle (see circle_in_one_sphere), and the
@@ -478,6 +493,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
   __shared__ int8_t latt[KCAP];
   __shared__ uint8_t fws[KCAP];   // forward slots (atom index > i) by ascending atom index
   __shared__ uint8_t frk[KCAP];   // forward rank of a forward slot
+  __shared__ uint8_t pslots[KCAP];   // row slots of the row's pairs, in pair order
   __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
   // cpair (step 3) and tpre (step 5) share their words
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
@@ -532,32 +548,58 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
     const bool iblock = iatt <= ATTEN_BLOCKER, iarc = iatt >= ATTEN_2;
     const double idens = b.dens[i];
     const int nw = (nn + WAVE - 1) / WAVE;
+    // the row's pairs as the pair pass numbers them: relevant forward
+    // neighbours in row order (the pass's count for the row is p_hi - p_lo)
+    {
+      const int nbmax = w.nbmax[r];
+      int np = 0;
+      for (int s0 = 0; s0 < nn; s0 += WAVE) {
+        const int sl = s0 + lane();
+        bool rel = false;
+        if (sl < nn && lj[sl] > i) {
+          V3 u, t;
+          double rij;
+          bool between;
+          const double4 aj = lx[sl];
+          rel = pair_torus(xi, ai.w, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between) &&
+                pair_relevant(iatt, latt[sl], nbmax, lj[sl]);
+        }
+        const uint64_t m = ballot(rel);
+        if (rel) pslots[np + lanes_below(m)] = (uint8_t)sl;
+        np += __popcll(m);
+      }
+    }
     __syncthreads();
     for (int pb = p_lo; pb < p_hi;) {
       // the batch: consecutive pairs whose triangle items fit the queue
       const int avail = min(PR_PAIRS, p_hi - pb);
       int j = -1, first = F, cnt = 0;
+      int slot = 0;
       if (lane() < avail) {
         // forward atoms after j: positions first .. F-1 of fws
-        j = w.pj[pb + lane()];
-        first = frk[w.pslot[pb + lane()]] + 1;
+        slot = pslots[pb - p_lo + lane()];
+        j = lj[slot];
+        first = frk[slot] + 1;
         cnt = F - first;
       }
       const int inc = wave_incl_scan(cnt);
       const int nb = __popcll(ballot(lane() < avail && inc <= PR_QCAP));   // >= 1: cnt < KCAP <= PR_QCAP
       const int items = __shfl(inc, nb - 1, WAVE);
       if (lane() < nb) {
-        const int p = pb + lane();
+        const double4 aj = lx[slot];
+        V3 u, t;
+        double rij;
+        bool between;
+        pair_torus(xi, ai.w, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between);
         bj[lane()] = j;
-        bu[lane()] = w.pu[p];
-        bt[lane()] = w.pt[p];
-        bx[lane()] = b.atom[j];
+        bu[lane()] = make_double4(u.x, u.y, u.z, rij);
+        bt[lane()] = make_double4(t.x, t.y, t.z, 0.0);
+        bx[lane()] = aj;
         bdens[lane()] = b.dens[j];
-        const int8_t jatt = w.atten[j];
-        const int8_t pf = w.pflag[p];
+        const int8_t jatt = latt[slot];
         const bool bij = iblock && jatt <= ATTEN_BLOCKER;
-        bflag[lane()] = ((pf & 2) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0) | ((pf & 1) ? PB_BETWEEN : 0) |
-                        (jatt >= ATTEN_2 ? PB_ARC2 : 0);
+        bflag[lane()] = (pair_toroid(iatt, jatt, s.rp) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0) |
+                        (between ? PB_BETWEEN : 0) | (jatt >= ATTEN_2 ? PB_ARC2 : 0);
         bnp[lane()] = 0;
         bnm[lane()] = 0;
         for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = 0ull;
@@ -783,18 +825,27 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         const int64_t vp = 2ll * g0 + (p - g0);
         const int64_t vm = 2ll * g0 + (g1 - g0) + (p - g0);
         const int f = bflag[lane()];
-        int8_t pf = w.pflag[p];
+        int pf = ((f & PB_BETWEEN) ? 1 : 0) | ((f & PB_TOR) ? 2 : 0);
         int np = 0, nm = 0;
         if (f & PB_COVERED) {
-          pf = (int8_t)(pf | 4);
+          pf |= 4;
         } else if (!(f & PB_KILLED)) {
           np = bnp[lane()];
           nm = bnm[lane()];
-          if (np + nm > 0) pf = (int8_t)(pf | 8);
+          if (np + nm > 0) pf |= 8;
+        }
+        // the pair record (what the pair pass's emit would write); the
+        // emit passes read the geometry only of pairs with probes or
+        // toroidal dots
+        w.pi[p] = i;
+        w.pj[p] = bj[lane()];
+        w.pflag[p] = (int8_t)pf;
+        if (np + nm + tn1[lane()] + tn2[lane()] > 0) {
+          w.pu[p] = bu[lane()];
+          w.pt[p] = bt[lane()];
         }
         w.prcnt[vp] = np;
         w.prcnt[vm] = nm;
-        w.pflag[p] = pf;
         w.t1cnt[p] = tn1[lane()];
         w.t2cnt[p] = tn2[lane()];
         w.tflag[p] = (tmk0[lane()] | tmk1[lane()]) ? 1 : 0;
