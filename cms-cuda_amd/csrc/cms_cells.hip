@@ -397,40 +397,46 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    cells_around(gi, ix, iy, iz, cstart, 2, 0x7ffffffu, cbeg, cpre);
-    const int T = cpre[27];
     int cnt = 0, cntb = 0, nbmax = -1;
-    for (int i0 = 0; i0 < T; i0 += WAVE) {
-      const int idx = i0 + lane();
-      bool isn = false, isb = false;
-      double d2 = 0.0;
-      int j = -1;
-      if (idx < T) {
-        const int t = cell_slot(cbeg, cpre, idx);
-        j = catom[t];
-        const double4 aj = cxyz[t];
-        const V3 d = xi - ld3(aj);
-        d2 = dot_seq(d, d);
-        const double br = (ai.w + aj.w) + 2.0 * s.rp;
-        const bool inb = d2 < br * br;
-        if (b.amol[j] == m) {
-          isn = inb && j != g;
-          if (isn && w.atten[j] > ATTEN_BLOCKER) nbmax = max(nbmax, j);
-        } else {
-          isb = inb && w.atten[j] >= ATTEN_BURIED;
+    // two sweeps: the atom's own molecule (neighbour row), then the other
+    // molecule (buried row); the cell slot gives the molecule
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      const bool own = sweep == 0;
+      cells_around(gi, ix, iy, iz, cstart, own ? m : 1 - m, 0x7ffffffu, cbeg, cpre);
+      const int T = cpre[27];
+      for (int i0 = 0; i0 < T; i0 += WAVE) {
+        const int idx = i0 + lane();
+        bool isn = false, isb = false;
+        double d2 = 0.0;
+        int j = -1;
+        if (idx < T) {
+          const int t = cell_slot(cbeg, cpre, idx);
+          j = catom[t];
+          const double4 aj = cxyz[t];
+          const V3 d = xi - ld3(aj);
+          d2 = dot_seq(d, d);
+          const double br = (ai.w + aj.w) + 2.0 * s.rp;
+          const bool inb = d2 < br * br;
+          if (own) {
+            isn = inb && j != g;
+            if (isn && w.atten[j] > ATTEN_BLOCKER) nbmax = max(nbmax, j);
+          } else {
+            isb = inb && w.atten[j] >= ATTEN_BURIED;
+          }
         }
+        const uint64_t mn = ballot(isn), mb = ballot(isb);
+        if (isn) {
+          const int p = cnt + lanes_below(mn);
+          if (p < KCAP) { cj[p] = j; cd[p] = d2; }
+        }
+        if (isb) {
+          const int p = cntb + lanes_below(mb);
+          if (p < BCAP) bj[p] = j;
+        }
+        cnt += __popcll(mn);
+        cntb += __popcll(mb);
       }
-      const uint64_t mn = ballot(isn), mb = ballot(isb);
-      if (isn) {
-        const int p = cnt + lanes_below(mn);
-        if (p < KCAP) { cj[p] = j; cd[p] = d2; }
-      }
-      if (isb) {
-        const int p = cntb + lanes_below(mb);
-        if (p < BCAP) bj[p] = j;
-      }
-      cnt += __popcll(mn);
-      cntb += __popcll(mb);
+      __syncthreads();
     }
     __syncthreads();
     if (cnt > KCAP || cntb > BCAP) {
