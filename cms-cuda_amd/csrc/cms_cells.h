@@ -10,7 +10,29 @@ struct GridInfo {
   double x0, y0, z0, h;  // grid origin and cell edge
   int nx, ny, nz;        // cells per axis
   int cell0;             // first cell of this complex in the batch-wide cell array
+  double rmax;           // largest atom radius of the complex
 };
+
+// Cells of the 3x3x3 block around (ix, iy, iz) (bit k = 9 (dx + 1) + 3 (dy + 1) + dz + 1) whose
+// box comes within lim of the point (x, y, z) of cell (ix, iy, iz): atoms
+// farther than lim lie only in the other cells.  The point's offsets from its
+// cell's faces are clamped at 0 and the test carries a 1e-6 A margin, far
+// above the rounding of the offsets and of the cell index.
+__device__ __forceinline__ uint32_t cells_within(const GridInfo& gi, double x, double y, double z, int ix, int iy,
+                                                 int iz, double lim) {
+  const double fx = x - (gi.x0 + ix * gi.h), fy = y - (gi.y0 + iy * gi.h), fz = z - (gi.z0 + iz * gi.h);
+  const double gx[3] = {fmax(fx, 0.0), 0.0, fmax(gi.h - fx, 0.0)};
+  const double gy[3] = {fmax(fy, 0.0), 0.0, fmax(gi.h - fy, 0.0)};
+  const double gz[3] = {fmax(fz, 0.0), 0.0, fmax(gi.h - fz, 0.0)};
+  const double l2 = (lim + 1e-6) * (lim + 1e-6);
+  uint32_t mask = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const double a = gx[k / 9], b = gy[(k / 3) % 3], c = gz[k % 3];
+    if ((a * a + b * b) + c * c <= l2) mask |= 1u << k;
+  }
+  return mask;
+}
 
 __global__ void k_grid_setup(int B, const int* aoff, const double4* atom, Settings s, GridInfo* grid,
                              double4* frame, int* ncell);

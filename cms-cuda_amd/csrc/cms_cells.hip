@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(int B, const int* __restrict
       gi.x0 = lo[0]; gi.y0 = lo[1]; gi.z0 = lo[2]; gi.h = h;
       gi.nx = d[0]; gi.ny = d[1]; gi.nz = d[2];
       gi.cell0 = 0;
+      gi.rmax = rm;
       grid[c] = gi;
       ncell[c] = d[0] * d[1] * d[2];
       const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
@@ -285,7 +286,9 @@ __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, co
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    cells_around(gi, ix, iy, iz, cstart, m, 0x7ffffffu, cbeg, cpre);
+    // the bridge distance is at most r_i + r_max + 2 rp
+    const double reach = (ai.w + gi.rmax) + 2.0 * s.rp;
+    cells_around(gi, ix, iy, iz, cstart, m, cells_within(gi, ai.x, ai.y, ai.z, ix, iy, iz, reach), cbeg, cpre);
     const int T = cpre[27];
     for (int idx = lane(); idx < T; idx += WAVE) {
       const int t = cell_slot(cbeg, cpre, idx);
@@ -362,7 +365,9 @@ __global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s,
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, 0x7ffffffu, cell) {
+    // the bridge distance is at most r_i + r_max + 2 rp
+    const uint32_t within = cells_within(gi, ai.x, ai.y, ai.z, ix, iy, iz, (ai.w + gi.rmax) + 2.0 * s.rp);
+    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, within, cell) {
 #pragma unroll 4
       for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
         const int j = catom[t];

@@ -605,6 +605,7 @@ __global__ __launch_bounds__(256) void k_dot_grid_setup(int G, const int* __rest
       gi.x0 = lo[0]; gi.y0 = lo[1]; gi.z0 = lo[2]; gi.h = h;
       gi.nx = d[0]; gi.ny = d[1]; gi.nz = d[2];
       gi.cell0 = 0;
+      gi.rmax = 0.0;
       grid[g] = gi;
       ncell[g] = d[0] * d[1] * d[2];
     }
@@ -692,14 +693,18 @@ __global__ __launch_bounds__(256) void k_trim_grid(Count nb, const double4* __re
     int ix, iy, iz;
     dot_cell(gi, q.x, q.y, q.z, &ix, &iy, &iz);
     bool hit = false;
+    // only cells whose box comes within the band of the dot
+    const uint32_t within = cells_within(gi, q.x, q.y, q.z, ix, iy, iz, sqrt(band2));
     for (int dx = -1; dx <= 1 && !hit; ++dx) {
       const int cx = ix + dx;
       if (cx < 0 || cx >= gi.nx) continue;
       for (int dy = -1; dy <= 1 && !hit; ++dy) {
         const int cy = iy + dy;
         if (cy < 0 || cy >= gi.ny) continue;
-        // the three cells along z are contiguous in the cell array
-        const int zlo = max(iz - 1, 0), zhi = min(iz + 1, gi.nz - 1);
+        // the cells along z are contiguous in the cell array
+        const uint32_t col = (within >> (9 * (dx + 1) + 3 * (dy + 1))) & 7u;
+        if (!col) continue;
+        const int zlo = max(iz - ((col & 1u) ? 1 : 0), 0), zhi = min(iz + ((col & 4u) ? 1 : 0), gi.nz - 1);
         const int c0 = gi.cell0 + (cx * gi.ny + cy) * gi.nz;
         const int lo = cstart[c0 + zlo], hi = cstart[c0 + zhi + 1];
         if constexpr (L == 1) {
