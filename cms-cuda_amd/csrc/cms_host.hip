@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kscan, dbase, dgrp;
+  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kstage, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -238,6 +238,7 @@ static Work work(cms_ctx* ctx) {
   w.tflag = ctx->tflag.as<int8_t>();
   w.tmask = ctx->tmask.as<uint64_t>();
   w.kcnt = ctx->kcnt.as<int>();
+  w.kstage = ctx->kstage.as<DotRec>();
   w.kscan = ctx->kscan.as<int>();
   w.dbase = ctx->dbase.as<int64_t>();
   w.dpos = ctx->dpos.as<double4>();
@@ -263,7 +264,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kstage, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
                   &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -505,6 +506,7 @@ static int size_probes(cms_ctx* ctx, int64_t c) {
   CK(ctx->nearl.ensure(sizeof(int) * (c * NEARCAP + 1)));
   CK(ctx->nnear.ensure(sizeof(int) * (c + 1)));
   CK(ctx->kcnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->kstage.ensure(sizeof(DotRec) * KSLOTS * (c + 1)));
   CK(ctx->kscan.ensure(sizeof(int) * (c + 1)));
   ctx->cap[M_Q] = std::max(ctx->cap[M_Q], c);
   return CMS_OK;
@@ -822,8 +824,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     TIMED("toroid_emit", k_toroid_emit<<<grid_for(est[M_P] / 16 + 1, 1), 64, 0, st>>>(b, s, w, ctx->tel.as<int>(),
                                                                                       meta + M_TE));
   }
-  if (est[M_Q] > 0 && s.rp > 0.0)
-    TIMED("concave_emit", k_concave<true><<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_Q] > 0 && s.rp > 0.0) {
+    TIMED("concave_emit", (k_concave_place<<<grid_for(est[M_Q] * KSLOTS, 256, 8192), 256, 0, st>>>(b, w),
+                           k_concave<true><<<grid_for(est[M_Q] / 64 + 1, 1), 64, 0, st>>>(b, s, w)));
+  }
   CKL();
   ctx->generated = true;
   ctx->gen_all_atoms = mode;

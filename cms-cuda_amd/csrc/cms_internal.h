@@ -55,6 +55,20 @@ struct BatchView {
   int B;
 };
 
+// One dot's output record (the fields of the dot arrays).
+struct DotRec {
+  double4 pos;    // x, y, z, area
+  double4 nml;    // outward normal, 0
+  int4 desc;
+  int4 desc2;
+  int atom;
+  int8_t bur;
+  int8_t type;
+};
+// Concave dots computed by the count pass per probe before the scan places
+// them (probes with more are emitted again after the scan).
+constexpr int KSLOTS = 64;
+
 struct Work {
   // per atom
   int8_t* atten;
@@ -101,6 +115,7 @@ struct Work {
   int8_t* tflag;
   uint64_t* tmask;        // count pass: free torus points of each pair (two words)
   int* kcnt;
+  DotRec* kstage;         // KSLOTS staged concave dots per probe
   int* kscan;
   int64_t* dbase;         // G*4 stage bases
   // dots
@@ -127,6 +142,7 @@ __global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);
 __global__ void k_compact_count(Count, const int*, const int*, int*);
 __global__ void k_toroid_emit(BatchView, Settings, Work, const int*, const int*);
 template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);
+__global__ void k_concave_place(BatchView, Work);
 __global__ void k_compact_index(int, const int*, const int*, int*);
 __global__ void k_near(BatchView, Settings, Work);
 

@@ -901,9 +901,9 @@ __global__ __launch_bounds__(64) void k_near(BatchView b, Settings s, Work w) {
 // ------------------------------------------------------------ dot output
 // Outward normal and burial test against the owning atom's buried row
 // (_core.py:2337-2377), then the store.
-__device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s, const Work& w,
-                                          int64_t o, V3 pt, V3 pc, double area, int owner,
-                                          int rowner, int8_t type, int4 desc, int4 desc2) {
+__device__ __forceinline__ DotRec make_dot(const BatchView& b, const Settings& s, const Work& w, V3 pt, V3 pc,
+                                           double area, int owner, int rowner, int8_t type, int4 desc,
+                                           int4 desc2) {
   V3 nml = (s.rp <= 0.0) ? pt - ld3(b.atom[owner]) : (pc - pt) / s.rp;
   // every dot owner has a neighbour row; a missing one would be a bug, so
   // it is flagged (pool status) instead of read out of bounds
@@ -921,14 +921,30 @@ __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s,
     const double lim = ab.w + s.rp;
     if (dot_seq(d, d) <= lim * lim) { buried = 1; break; }
   }
+  DotRec d;
+  d.pos = make_double4(pt.x, pt.y, pt.z, area);
+  d.nml = make_double4(nml.x, nml.y, nml.z, 0.0);
+  d.desc = desc;
+  d.desc2 = desc2;
+  d.atom = owner;
+  d.bur = buried;
+  d.type = type;
+  return d;
+}
+__device__ __forceinline__ void write_dot(const Work& w, int64_t o, const DotRec& d) {
   if (o >= w.dot_cap) return;
-  w.dpos[o] = make_double4(pt.x, pt.y, pt.z, area);
-  w.dnml[o] = make_double4(nml.x, nml.y, nml.z, 0.0);
-  w.dbur[o] = buried;
-  w.dtype[o] = type;
-  w.datom[o] = owner;
-  w.ddesc[o] = desc;
-  w.ddesc2[o] = desc2;
+  w.dpos[o] = d.pos;
+  w.dnml[o] = d.nml;
+  w.dbur[o] = d.bur;
+  w.dtype[o] = d.type;
+  w.datom[o] = d.atom;
+  w.ddesc[o] = d.desc;
+  w.ddesc2[o] = d.desc2;
+}
+__device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s, const Work& w,
+                                          int64_t o, V3 pt, V3 pc, double area, int owner,
+                                          int rowner, int8_t type, int4 desc, int4 desc2) {
+  write_dot(w, o, make_dot(b, s, w, pt, pc, area, owner, rowner, type, desc, desc2));
 }
 
 // ------------------------------------------------------------ convex
@@ -1341,7 +1357,10 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
   __shared__ double4 lnear[NEARCAP];
   const int nQ = *w.nQ;
   const double rp = s.rp, rp2 = s.rp * s.rp;
-  auto live = [&](int64_t u) { return !EMIT || w.kcnt[u] != 0; };
+  // count pass: every probe, its first KSLOTS dots computed into the
+  // probe's staging slots (placed by k_concave_place after the scan); emit
+  // pass: only probes with more dots than that
+  auto live = [&](int64_t u) { return !EMIT || w.kcnt[u] > KSLOTS; };
   CMS_SPARSE_LOOP(q, nQ, live) {
     const int4 at = w.patom[q];
     const double4 pq = w.ppos[q];
@@ -1435,7 +1454,8 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
         }
       }
       const uint64_t mk = ballot(keep);
-      if (EMIT && keep) {
+      const int k = total + lanes_below(mk);
+      if (keep && (EMIT || k < KSLOTS)) {
         int mc = 0;
         double dbest = norm_seq(pts - xa[0]) - ra[0];
         for (int a = 1; a < 3; ++a) {
@@ -1443,10 +1463,13 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
           if (d < dbest) { dbest = d; mc = a; }
         }
         const int owner = aid[mc];
-        const int64_t o = obase + total + lanes_below(mk);
-        store_dot(b, s, w, o, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
-                  make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
-                  make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
+        const DotRec d = make_dot(b, s, w, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
+                                  make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
+                                  make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
+        if (EMIT)
+          write_dot(w, obase + k, d);
+        else
+          w.kstage[q * KSLOTS + k] = d;
       }
       total += __popcll(mk);
     }
@@ -1464,5 +1487,19 @@ template __global__ void k_convex<false>(BatchView, Settings, Work);
 template __global__ void k_convex<true>(BatchView, Settings, Work);
 template __global__ void k_concave<false>(BatchView, Settings, Work);
 template __global__ void k_concave<true>(BatchView, Settings, Work);
+
+// Staged concave dots to their place in the dot arrays: one thread per
+// staging slot of probes with at most KSLOTS dots.
+__global__ void k_concave_place(BatchView b, Work w) {
+  const int nQ = *w.nQ;
+  const int64_t n = (int64_t)nQ * KSLOTS;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(i / KSLOTS), k = (int)(i % KSLOTS);
+    const int kc = w.kcnt[q];
+    if (kc > KSLOTS || k >= kc) continue;
+    const int grp = group_of_atom(b, w.patom[q].x);
+    write_dot(w, w.dbase[grp * 4 + 3] + w.kscan[q] + k, w.kstage[i]);
+  }
+}
 
 }  // namespace cms
