@@ -24,9 +24,31 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total) {
   return off + inc - v;
 }
 
-// Multi-block scan: nb blocks, each owns a contiguous chunk of
-// ceil(n / nb) items, n read on the device.
-__device__ __forceinline__ int64_t scan_chunk(int64_t n, int nb) { return (n + nb - 1) / nb; }
+// Multi-block scan: nb blocks, each owns a contiguous chunk of whole tiles
+// (SCAN_PER items per thread, 4,096 per tile) covering n, read on the device.
+constexpr int SCAN_PER = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_PER;
+__device__ __forceinline__ int64_t scan_chunk(int64_t n, int nb) {
+  const int64_t c = (n + nb - 1) / nb;
+  return (c + SCAN_TILE - 1) / SCAN_TILE * SCAN_TILE;
+}
+
+// The SCAN_PER items of thread t in the tile at base (zero past hi); vector
+// loads when the run is whole (base and the buffers are 16-byte aligned).
+__device__ __forceinline__ void scan_load(const int* __restrict__ in, int64_t base, int64_t hi, int* v) {
+  const int64_t i0 = base + (int64_t)threadIdx.x * SCAN_PER;
+  if (i0 + SCAN_PER <= hi) {
+    const int4* p = reinterpret_cast<const int4*>(in + i0);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER / 4; ++k) {
+      const int4 q = p[k];
+      v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) v[k] = i0 + k < hi ? in[i0 + k] : 0;
+  }
+}
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partial(const int* __restrict__ in, Count cnt, int nb,
                                                               int* __restrict__ part) {
@@ -35,7 +57,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partial(const int* __rest
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   int acc = 0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += SCAN_THREADS) acc += in[i];
+  for (int64_t base = lo; base < hi; base += SCAN_TILE) {
+    int v[SCAN_PER];
+    scan_load(in, base, hi, v);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) acc += v[k];
+  }
   int tot;
   block_excl_scan(acc, &tot);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
@@ -62,12 +89,32 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restri
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   int carry = part[blockIdx.x];
-  for (int64_t base = lo; base < hi; base += SCAN_THREADS) {
-    const int64_t i = base + threadIdx.x;
-    const int v = i < hi ? in[i] : 0;
+  for (int64_t base = lo; base < hi; base += SCAN_TILE) {
+    int v[SCAN_PER];
+    scan_load(in, base, hi, v);
+    int run = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) run += v[k];
     int tot;
-    const int e = block_excl_scan(v, &tot);
-    if (i < hi) out[i] = carry + e;
+    int e = carry + block_excl_scan(run, &tot);
+    const int64_t i0 = base + (int64_t)threadIdx.x * SCAN_PER;
+    if (i0 + SCAN_PER <= hi) {
+      int4* p = reinterpret_cast<int4*>(out + i0);
+#pragma unroll
+      for (int k = 0; k < SCAN_PER / 4; ++k) {
+        int4 q;
+        q.x = e; e += v[4 * k];
+        q.y = e; e += v[4 * k + 1];
+        q.z = e; e += v[4 * k + 2];
+        q.w = e; e += v[4 * k + 3];
+        p[k] = q;
+      }
+    } else {
+      for (int k = 0; k < SCAN_PER && i0 + k < hi; ++k) {
+        out[i0 + k] = e;
+        e += v[k];
+      }
+    }
     carry += tot;
   }
   if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = part[nb];
