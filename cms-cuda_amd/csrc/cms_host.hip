@@ -84,11 +84,11 @@ struct cms_ctx {
   // readout state
   bool lists_ready = false;
   int nB = 0, nA = 0;
-  DBuf bflag, bscan, blist, bxyz, bgrp, aflag, ascan, alist, axyz, agrpd;
+  DBuf bflag, bscan, blist, bxyz, bgrp, axyz, agrpd;
   DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm;
   DBuf trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
-  DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;
+  DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;   // frame: cms_nearest's point frame
   DBuf bidx, bgof, adj, eatom, evpos, chk, qlast;
   DBuf tgrid, tncell, tncscan, tacell, tcount, tstart, tcur, tsorted;   // trim grid
   bool cms_ready = false;
@@ -105,6 +105,9 @@ struct cms_ctx {
   uint64_t layout_epoch = 0;        // bumped when the batch layout changes
   // one captured cms_run step
   bool use_graph = false;
+  // the packed per-atom ids / densities came from default densities
+  bool packed_default = false;
+  double packed_density = 0.0;
   // probes count pass: one wavefront per pair suits small batches (more
   // units than rows), one per neighbour row large ones (full lanes); the
   // environment variable CMS_PROBE_COUNT = "pairs" / "rows" fixes the choice
@@ -266,8 +269,8 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
                   &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kstage, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
-                  &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp, &ctx->aflag,
-                  &ctx->ascan, &ctx->alist, &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
+                  &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
+                  &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
                   &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
                   &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
@@ -429,10 +432,16 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
                       hipMemcpyHostToDevice, ctx->stream));
   }
   if (n > 0) {
+    // complex ids, molecules and default densities of the same layout are
+    // already in place: only coordinates and radii are packed again
+    const bool coords_only = same_layout && !density && ctx->packed_default && ctx->packed_density == ctx->density;
     k_pack_atoms<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(
         n, xyz, radius, density, ctx->density, ctx->aoff.as<int>(), ctx->nb0.as<int>(), n_complex,
-        ctx->atom.as<double4>(), ctx->dens.as<double>(), ctx->acplx.as<int>(), ctx->amol.as<int8_t>());
+        coords_only ? 0 : 1, ctx->atom.as<double4>(), ctx->dens.as<double>(), ctx->acplx.as<int>(),
+        ctx->amol.as<int8_t>());
     CKL();
+    ctx->packed_default = !density;
+    ctx->packed_density = ctx->density;
   }
   CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
   fill_u64(ctx->nresc.as<unsigned long long>(), 2, 0ull, ctx->stream);
@@ -520,7 +529,7 @@ static int size_dots(cms_ctx* ctx, int64_t c) {
   for (DBuf* b : i4) CK(b->ensure(sizeof(int4) * (c + 1)));
   DBuf* f4[] = {&ctx->bxyz32, &ctx->txyz32};
   for (DBuf* b : f4) CK(b->ensure(sizeof(float4) * (c + 1)));
-  DBuf* i1[] = {&ctx->datom, &ctx->bflag, &ctx->bscan, &ctx->aflag, &ctx->ascan, &ctx->blist, &ctx->alist,
+  DBuf* i1[] = {&ctx->datom, &ctx->bflag, &ctx->bscan, &ctx->blist,
                 &ctx->bidx, &ctx->operm, &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->tidx, &ctx->tgof,
                 &ctx->bgof};
   for (DBuf* b : i1) CK(b->ensure(sizeof(int) * (c + 1)));
@@ -900,25 +909,18 @@ static int prepare_lists(cms_ctx* ctx) {
   const Count cD = dev_count(meta + M_D);
   CK(ctx->bgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->agrpd.ensure(sizeof(int) * (G + 1)));
-  if (estD > 0) {
+  if (estD > 0)
     k_dot_flag<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->dbur.as<int8_t>(), 1, ctx->bflag.as<int>());
-    k_dot_flag<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->dbur.as<int8_t>(), 0, ctx->aflag.as<int>());
-  }
   scan_excl(ctx->bflag.as<int>(), ctx->bscan.as<int>(), cD, capD, part, st);
   k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->dgrp.as<int>(), 1, ctx->bscan.as<int>(),
                                                              ctx->bgrp.as<int>());
-  scan_excl(ctx->aflag.as<int>(), ctx->ascan.as<int>(), cD, capD, part, st);
-  k_gather_scan<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->dgrp.as<int>(), 1, ctx->ascan.as<int>(),
-                                                             ctx->agrpd.as<int>());
+  k_gather_compl<<<grid_for(G + 1, 256, 4096), 256, 0, st>>>(G, ctx->dgrp.as<int>(), ctx->bscan.as<int>(),
+                                                              ctx->agrpd.as<int>());
   k_list_totals<<<1, 64, 0, st>>>(ctx->bscan.as<int>(), meta);
-  if (estD > 0) {
-    k_dot_compact<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->bflag.as<int>(), ctx->bscan.as<int>(),
-                                                              ctx->dpos.as<double4>(), ctx->blist.as<int>(),
-                                                              ctx->bxyz.as<double4>());
-    k_dot_compact<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->aflag.as<int>(), ctx->ascan.as<int>(),
-                                                              ctx->dpos.as<double4>(), ctx->alist.as<int>(),
-                                                              ctx->axyz.as<double4>());
-  }
+  if (estD > 0)
+    k_dot_split<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->dbur.as<int8_t>(), ctx->bscan.as<int>(),
+                                                            ctx->dpos.as<double4>(), ctx->blist.as<int>(),
+                                                            ctx->bxyz.as<double4>(), ctx->axyz.as<double4>());
   CKL();
   ctx->lists_ready = true;
   return CMS_OK;
@@ -931,7 +933,8 @@ static int prepare_lists(cms_ctx* ctx) {
 // nq_hint / nr_hint; any split gives the same result).
 static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const float4* Q32, const int* qgrp,
                           Count nq, int64_t qcap, int64_t nq_hint, int64_t nr_hint, const double4* R,
-                          const float4* R32, const int* rgrp, int xmask, int G, double* out_d2, int* out_idx) {
+                          const float4* R32, const int* rgrp, int xmask, int G, const double4* frame,
+                          double* out_d2, int* out_idx) {
   hipStream_t st = ctx->stream;
   CK(ctx->tcnt.ensure(sizeof(int) * (G + 1)));
   CK(ctx->toff.ensure(sizeof(int) * (G + 1)));
@@ -961,10 +964,10 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
     unsigned long long* nr = ctx->nresc.as<unsigned long long>();
     if (ido)
       TIMED("nearest32", k_nearest32<true><<<grid, 256, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
-                                                                 ctx->frame.as<double4>(), d2o, ido, nr));
+                                                                 frame, d2o, ido, nr));
     else
       TIMED("nearest32", k_nearest32<false><<<grid, 256, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
-                                                                  ctx->frame.as<double4>(), d2o, nullptr, nr));
+                                                                  frame, d2o, nullptr, nr));
   } else {
     if (ido)
       TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, ido));
@@ -984,13 +987,11 @@ static int prepare_float_lists(cms_ctx* ctx) {
   hipStream_t st = ctx->stream;
   const int B = ctx->B, G = 2 * B;
   const int* meta = ctx->meta.as<int>();
-  CK(ctx->frame.ensure(sizeof(double4) * (B + 1)));
   CK(ctx->nresc.ensure(sizeof(unsigned long long) * 2));
-  k_complex_frame<<<grid_for(B, 1, 4096), 256, 0, st>>>(B, ctx->aoff.as<int>(), ctx->atom.as<double4>(), ctx->s.rp,
-                                                         ctx->frame.as<double4>());
+  // the per-complex frames come from the grid setup of the generation
   if (ctx->est[M_D] > 0) {
     k_to_float<<<grid_for(ctx->est[M_B] + 1, 256, 4096), 256, 0, st>>>(
-        dev_count(meta + M_B), ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, 1, ctx->frame.as<double4>(),
+        dev_count(meta + M_B), ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, 1, ctx->gframe.as<double4>(),
         ctx->bxyz32.as<float4>());
   }
   CKL();
@@ -1048,7 +1049,8 @@ static int contact_ms_pass(cms_ctx* ctx, int precision, double* per_atom, double
   const int64_t estB = ctx->est[M_B];
   r = launch_nearest(ctx, precision, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(), ctx->bgrp.as<int>(), cB,
                      ctx->cap[M_D], estB, estB, ctx->bxyz.as<double4>(), ctx->bxyz32.as<float4>(),
-                     ctx->bgrp.as<int>(), 1, G, ctx->bmin.as<double>(), ctx->bidx.as<int>());
+                     ctx->bgrp.as<int>(), 1, G, ctx->gframe.as<double4>(), ctx->bmin.as<double>(),
+                     ctx->bidx.as<int>());
   if (r) return r;
   k_cms_weights<<<grid_for(std::max<int64_t>(estB, 1), 256, 4096), 256, 0, st>>>(
       cB, ctx->bxyz.as<double4>(), ctx->bmin.as<double>(), ctx->s.weight, ctx->wv.as<double>());
@@ -1143,10 +1145,10 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   CKL();
   const int gT = grid_for(std::max<int64_t>(estT, 1), 256, 4096);
   if (precision == CMS_PREC_FP32_RESCORE)
-    k_to_float<<<gT, 256, 0, st>>>(cT, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), G, 1, ctx->frame.as<double4>(),
+    k_to_float<<<gT, 256, 0, st>>>(cT, ctx->txyz.as<double4>(), ctx->tgrp.as<int>(), G, 1, ctx->gframe.as<double4>(),
                                    ctx->txyz32.as<float4>());
   r = launch_nearest(ctx, precision, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), cT, capD,
-                     estT, estT, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), 1, G,
+                     estT, estT, ctx->txyz.as<double4>(), ctx->txyz32.as<float4>(), ctx->tgrp.as<int>(), 1, G, ctx->gframe.as<double4>(),
                      ctx->tmin.as<double>(), ctx->tidx.as<int>());
   if (r) return r;
   k_group_of_items<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->tgrp.as<int>(), ctx->tgof.as<int>());
@@ -1596,7 +1598,7 @@ int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const 
   }
   int rc = launch_nearest(ctx, precision, ctx->qpack.as<double4>(), ctx->qpack32.as<float4>(), ctx->grp1.as<int>(),
                           host_count((int)nq), nq, nq, nr, ctx->rpack.as<double4>(), ctx->rpack32.as<float4>(),
-                          ctx->grp1.as<int>() + 2, 0, 1, out_d2, out_idx);
+                          ctx->grp1.as<int>() + 2, 0, 1, ctx->frame.as<double4>(), out_d2, out_idx);
   if (rc) return rc;
   CK(hipStreamSynchronize(st));
   return CMS_OK;

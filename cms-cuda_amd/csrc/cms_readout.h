@@ -8,8 +8,9 @@
 namespace cms {
 
 __global__ void k_dot_flag(Count n, const int8_t* bur, int want, int* flag);
-__global__ void k_dot_compact(Count n, const int* flag, const int* scan, const double4* dpos, int* list,
-                              double4* xyz);
+__global__ void k_dot_split(Count cnt, const int8_t* dbur, const int* bscan, const double4* dpos, int* blist,
+                            double4* bxyz, double4* axyz);
+__global__ void k_gather_compl(int G, const int* idx, const int* scan, int* out);
 __global__ void k_tile_counts(int G, const int* qgrp, int tq, int* cnt);
 template <bool IDX>
 __global__ void k_nearest64(const double4* Q, const int* qgrp, const double4* R, const int* rgrp,
@@ -51,7 +52,6 @@ __global__ void k_dot_cell_scatter(Count na, const int* acell, const int* cstart
 template <int L>
 __global__ void k_trim_grid(Count nb, const double4* bxyz, const int* bgrp, int G, const GridInfo* grid,
                             const int* cstart, const double4* sorted, double band2, int* flag);
-__global__ void k_complex_frame(int B, const int* aoff, const double4* atom, double rp, double4* frame);
 __global__ void k_to_float(Count n, const double4* in, const int* grp, int G, int percomplex,
                            const double4* frame, float4* out);
 

@@ -15,16 +15,29 @@ __global__ void k_dot_flag(Count cnt, const int8_t* __restrict__ bur, int want, 
 }
 
 // list[scan[i]] = i for flagged i; also gathers the dot position record
-__global__ void k_dot_compact(Count cnt, const int* __restrict__ flag, const int* __restrict__ scan,
-                              const double4* __restrict__ dpos, int* __restrict__ list,
-                              double4* __restrict__ xyz) {
+// Buried and accessible dot lists in one pass: the buried ones at the
+// exclusive scan of the burial flags, the accessible ones at i - that scan.
+__global__ void k_dot_split(Count cnt, const int8_t* __restrict__ dbur, const int* __restrict__ bscan,
+                            const double4* __restrict__ dpos, int* __restrict__ blist, double4* __restrict__ bxyz,
+                            double4* __restrict__ axyz) {
   const int n = cnt.get();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (flag[i]) {
-      const int o = scan[i];
-      list[o] = i;
-      xyz[o] = dpos[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int b = bscan[i];
+    const double4 p = dpos[i];
+    if (dbur[i]) {
+      blist[b] = i;
+      bxyz[b] = p;
+    } else {
+      axyz[i - b] = p;
     }
+  }
+}
+// Accessible-list group starts: idx[g] - scan[idx[g]].
+__global__ void k_gather_compl(int G, const int* __restrict__ idx, const int* __restrict__ scan,
+                               int* __restrict__ out) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) out[g] = idx[g] - scan[idx[g]];
+}
+
 }
 
 // ------------------------------------------------------------ search

@@ -102,41 +102,6 @@ __global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q
   if (nrescored && resc) atomicAdd(nrescored, resc);
 }
 
-// Per-complex frame (centre of the atom bounding box, bound R on |x - c| per
-// component for anything within max radius + 2 rp + 1 A of an atom).
-__global__ __launch_bounds__(256) void k_complex_frame(int B, const int* __restrict__ aoff,
-                                                       const double4* __restrict__ atom, double rp,
-                                                       double4* __restrict__ frame) {
-  __shared__ double s[8][256];
-  for (int c = blockIdx.x; c < B; c += gridDim.x) {
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    double rmax = 0.0;
-    for (int i = aoff[c] + threadIdx.x; i < aoff[c + 1]; i += blockDim.x) {
-      const double4 a = atom[i];
-      mn[0] = fmin(mn[0], a.x); mn[1] = fmin(mn[1], a.y); mn[2] = fmin(mn[2], a.z);
-      mx[0] = fmax(mx[0], a.x); mx[1] = fmax(mx[1], a.y); mx[2] = fmax(mx[2], a.z);
-      rmax = fmax(rmax, a.w);
-    }
-    for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = mn[k]; s[3 + k][threadIdx.x] = mx[k]; }
-    s[6][threadIdx.x] = rmax;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (threadIdx.x < o) {
-        for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + o]);
-        for (int k = 3; k < 7; ++k) s[k][threadIdx.x] = fmax(s[k][threadIdx.x], s[k][threadIdx.x + o]);
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      double cx = 0.5 * (s[0][0] + s[3][0]), cy = 0.5 * (s[1][0] + s[4][0]), cz = 0.5 * (s[2][0] + s[5][0]);
-      double half = fmax(fmax(s[3][0] - cx, s[4][0] - cy), s[5][0] - cz);
-      if (!(half >= 0.0)) { cx = cy = cz = 0.0; half = 0.0; }
-      frame[c] = make_double4(cx, cy, cz, half + s[6][0] + 2.0 * rp + 1.0);
-    }
-    __syncthreads();
-  }
-}
-
 // float copy of a double4 point list, recentred on its complex's frame;
 // group of item i found from grp (G+1 starts), complex = group / 2 (or 0
 // when percomplex == 0).

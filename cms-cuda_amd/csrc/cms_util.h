@@ -22,7 +22,7 @@ void fill_int(int* p, Count n, int64_t cap, int v, hipStream_t st);
 void fill_double(double* p, int64_t n, double v, hipStream_t st);
 
 __global__ void k_pack_atoms(int ntot, const double* xyz, const double* rad, const double* dens_in,
-                             double dens_default, const int* aoff, const int* nb0, int B,
+                             double dens_default, const int* aoff, const int* nb0, int B, int full,
                              double4* atom, double* dens, int* acplx, int8_t* amol);
 __global__ void k_group_starts_from_atoms(Count n, const int* items, const int* acplx,
                                           const int8_t* amol, int G, int* start);

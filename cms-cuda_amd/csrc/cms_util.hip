@@ -259,10 +259,14 @@ void fill_double(double* p, int64_t n, double v, hipStream_t st) {
 // molecule from the complex offsets and binder counts.
 __global__ void k_pack_atoms(int ntot, const double* __restrict__ xyz, const double* __restrict__ rad,
                              const double* __restrict__ dens_in, double dens_default,
-                             const int* __restrict__ aoff, const int* __restrict__ nb0, int B,
+                             const int* __restrict__ aoff, const int* __restrict__ nb0, int B, int full,
                              double4* __restrict__ atom, double* __restrict__ dens,
                              int* __restrict__ acplx, int8_t* __restrict__ amol) {
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ntot; g += gridDim.x * blockDim.x) {
+    if (!full) {
+      atom[g] = make_double4(xyz[3 * (int64_t)g], xyz[3 * (int64_t)g + 1], xyz[3 * (int64_t)g + 2], rad[g]);
+      continue;
+    }
     int lo = 0, hi = B;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
