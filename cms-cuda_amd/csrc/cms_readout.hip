@@ -14,7 +14,6 @@ __global__ void k_dot_flag(Count cnt, const int8_t* __restrict__ bur, int want, 
     flag[i] = (bur[i] != 0) == (want != 0) ? 1 : 0;
 }
 
-// list[scan[i]] = i for flagged i; also gathers the dot position record
 // Buried and accessible dot lists in one pass: the buried ones at the
 // exclusive scan of the burial flags, the accessible ones at i - that scan.
 __global__ void k_dot_split(Count cnt, const int8_t* __restrict__ dbur, const int* __restrict__ bscan,
@@ -36,8 +35,6 @@ __global__ void k_dot_split(Count cnt, const int8_t* __restrict__ dbur, const in
 __global__ void k_gather_compl(int G, const int* __restrict__ idx, const int* __restrict__ scan,
                                int* __restrict__ out) {
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) out[g] = idx[g] - scan[idx[g]];
-}
-
 }
 
 // ------------------------------------------------------------ search
