@@ -94,8 +94,10 @@ __device__ __forceinline__ void cell_coords(const GridInfo& gi, double x, double
   *iz = min(max((int)floor((z - gi.z0) / gi.h), 0), gi.nz - 1);
 }
 
-__global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int* __restrict__ acell,
-                             int* __restrict__ ccount) {
+// Cell counts and the cell-sorted atom lists for small batches, one thread
+// per atom (wavefront-aggregated device atomics).
+__global__ void k_cell_count_atoms(BatchView b, const GridInfo* __restrict__ grid, int* __restrict__ acell,
+                                   int* __restrict__ ccount) {
   CMS_WAVE_LOOP(g, b.ntot) {
     int cell = 0;
     if (g < b.ntot) {
@@ -110,9 +112,9 @@ __global__ void k_cell_count(BatchView b, const GridInfo* __restrict__ grid, int
   }
 }
 
-__global__ void k_cell_scatter(int n, const int* __restrict__ acell, const int* __restrict__ cstart,
-                               int* __restrict__ cursor, int* __restrict__ catom,
-                               const double4* __restrict__ atom, double4* __restrict__ cxyz) {
+__global__ void k_cell_scatter_atoms(int n, const int* __restrict__ acell, const int* __restrict__ cstart,
+                                     int* __restrict__ cursor, int* __restrict__ catom,
+                                     const double4* __restrict__ atom, double4* __restrict__ cxyz) {
   CMS_WAVE_LOOP(g, n) {
     const int cell = g < n ? acell[g] : 0;
     const int slot = wave_counter_add(cursor, cell, g < n);
@@ -120,6 +122,108 @@ __global__ void k_cell_scatter(int n, const int* __restrict__ acell, const int* 
       catom[cstart[cell] + slot] = (int)g;
       cxyz[cstart[cell] + slot] = atom[g];
     }
+  }
+}
+
+// The same for large batches, one workgroup per complex:
+// the (cell, molecule) counters and cursors live in LDS when the complex's
+// grid has at most CELL_LDS of them (every grid of the benchmark batches),
+// otherwise device atomics on the zero-filled ccount / cursor arrays.  The
+// order of atoms within a cell is whatever the atomics give; nothing that
+// reads the lists depends on it.
+constexpr int CELL_LDS = 4096;
+constexpr int CELL_NT = 512;    // threads per complex
+constexpr int CELL_U = 4;       // atoms per thread per round (loads in flight)
+
+__global__ __launch_bounds__(CELL_NT) void k_cell_count(BatchView b, const GridInfo* __restrict__ grid,
+                                                        int* __restrict__ acell, int* __restrict__ ccount) {
+  __shared__ int cnt[CELL_LDS];
+  for (int c = blockIdx.x; c < b.B; c += gridDim.x) {
+    const GridInfo gi = grid[c];
+    const int ns = 2 * gi.nx * gi.ny * gi.nz;
+    const int base = 2 * gi.cell0;
+    const bool lds = ns <= CELL_LDS;
+    if (lds)
+      for (int k = threadIdx.x; k < ns; k += CELL_NT) cnt[k] = 0;
+    __syncthreads();
+    const int a1 = b.aoff[c + 1];
+    for (int g0 = b.aoff[c] + threadIdx.x; g0 < a1; g0 += CELL_U * CELL_NT) {
+      double4 a[CELL_U];
+      int8_t mol[CELL_U];
+#pragma unroll
+      for (int u = 0; u < CELL_U; ++u) {
+        const int g = g0 + u * CELL_NT;
+        if (g < a1) { a[u] = b.atom[g]; mol[u] = b.amol[g]; }
+      }
+#pragma unroll
+      for (int u = 0; u < CELL_U; ++u) {
+        const int g = g0 + u * CELL_NT;
+        if (g < a1) {
+          int ix, iy, iz;
+          cell_coords(gi, a[u].x, a[u].y, a[u].z, &ix, &iy, &iz);
+          const int k = 2 * ((ix * gi.ny + iy) * gi.nz + iz) + mol[u];
+          acell[g] = base + k;
+          if (lds) atomicAdd(&cnt[k], 1);
+          else atomicAdd(&ccount[base + k], 1);
+        }
+      }
+    }
+    __syncthreads();
+    if (lds)
+      for (int k = threadIdx.x; k < ns; k += CELL_NT) ccount[base + k] = cnt[k];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(CELL_NT) void k_cell_scatter(BatchView b, const GridInfo* __restrict__ grid,
+                                                          const int* __restrict__ acell,
+                                                          const int* __restrict__ cstart, int* __restrict__ cursor,
+                                                          int* __restrict__ catom, double4* __restrict__ cxyz) {
+  __shared__ int cur[CELL_LDS];
+  for (int c = blockIdx.x; c < b.B; c += gridDim.x) {
+    const GridInfo gi = grid[c];
+    const int ns = 2 * gi.nx * gi.ny * gi.nz;
+    const int base = 2 * gi.cell0;
+    const bool lds = ns <= CELL_LDS;
+    if (lds)
+      for (int k = threadIdx.x; k < ns; k += CELL_NT) cur[k] = cstart[base + k];
+    __syncthreads();
+    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
+    for (int g0 = a0 + threadIdx.x; g0 < a1; g0 += CELL_U * CELL_NT) {
+      int cell[CELL_U];
+#pragma unroll
+      for (int u = 0; u < CELL_U; ++u) {
+        const int g = g0 + u * CELL_NT;
+        cell[u] = g < a1 ? acell[g] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < CELL_U; ++u) {
+        const int g = g0 + u * CELL_NT;
+        if (g < a1) {
+          const int pos = lds ? atomicAdd(&cur[cell[u] - base], 1) : cstart[cell[u]] + atomicAdd(&cursor[cell[u]], 1);
+          catom[pos] = g;
+        }
+      }
+    }
+    // the complex's slots hold its atoms at cstart[base] .. + atom count;
+    // their coordinates are copied in slot order (coalesced stores) once
+    // this workgroup's index stores are visible to it
+    __syncthreads();
+    const int p0 = cstart[base], np = a1 - a0;
+    for (int t0 = threadIdx.x; t0 < np; t0 += CELL_U * CELL_NT) {
+      int src[CELL_U];
+#pragma unroll
+      for (int u = 0; u < CELL_U; ++u) {
+        const int t = t0 + u * CELL_NT;
+        src[u] = t < np ? catom[p0 + t] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < CELL_U; ++u) {
+        const int t = t0 + u * CELL_NT;
+        if (t < np) cxyz[p0 + t] = b.atom[src[u]];
+      }
+    }
+    __syncthreads();
   }
 }
 

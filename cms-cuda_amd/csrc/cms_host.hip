@@ -681,14 +681,25 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     const Count cCells = dev_count(ctx->ncscan.as<int>() + B, 2);   // two slots per cell
     fill_int(ctx->ccount.as<int>(), cCells, cell_cap, 0, st);
     fill_int(ctx->ccur.as<int>(), cCells, cell_cap, 0, st);
-    k_cell_count<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
-    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cCells, cell_cap, part, st);
-    k_cell_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
-                                                            ctx->ccur.as<int>(), ctx->catom.as<int>(),
-                                                            ctx->atom.as<double4>(), ctx->cxyz.as<double4>());
-    // small batches: one wavefront per atom (parallelism); large batches:
-    // one thread per cell-sorted atom (cell data shared within a wavefront)
+    // small batches: cell lists thread per atom and one wavefront per atom
+    // in the sweeps (parallelism); large batches: cell lists workgroup per
+    // complex (LDS counters) and one thread per cell-sorted atom in the
+    // sweeps (cell data shared within a wavefront)
     const bool per_thread = n >= 200000;
+    if (per_thread)
+      k_cell_count<<<grid_for(B, 1, 65536), 512, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
+    else
+      k_cell_count_atoms<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, grid, ctx->acell.as<int>(),
+                                                                 ctx->ccount.as<int>());
+    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cCells, cell_cap, part, st);
+    if (per_thread)
+      k_cell_scatter<<<grid_for(B, 1, 65536), 512, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->cstart.as<int>(),
+                                                             ctx->ccur.as<int>(), ctx->catom.as<int>(),
+                                                             ctx->cxyz.as<double4>());
+    else
+      k_cell_scatter_atoms<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
+                                                                   ctx->ccur.as<int>(), ctx->catom.as<int>(),
+                                                                   ctx->atom.as<double4>(), ctx->cxyz.as<double4>());
     const int* cst = ctx->cstart.as<int>();
     const int* cat = ctx->catom.as<int>();
     const double4* cxyz = ctx->cxyz.as<double4>();
