@@ -158,8 +158,10 @@ __global__ void k_owner_scatter(Count cnt, const int* __restrict__ list, const i
   }
 }
 // One wavefront per atom: the atom's item indices (scattered in arbitrary
-// order by k_owner_scatter) are sorted in LDS with a bitonic network, their
-// values gathered in that order and summed left to right by lane 0.
+// order by k_owner_scatter) are put in ascending order -- up to 64 of them
+// by rank (each lane counts the smaller indices), longer segments with a
+// bitonic network in LDS -- their values gathered in that order and summed
+// left to right by lane 0.
 // Atoms with no item (0.0) or one (0.0 + value), one thread each.
 __global__ void k_owner_small(int natoms, const int* __restrict__ start, const int* __restrict__ perm,
                               const double* __restrict__ val, int64_t vstride, double* __restrict__ out) {
@@ -191,6 +193,22 @@ __global__ __launch_bounds__(64) void k_owner_sums(int natoms, const int* __rest
         for (int i = lo; i < lo + n; ++i) s += val[(int64_t)perm[i] * vstride];
         out[a] = s;
       }
+      continue;
+    }
+    if (n <= WAVE) {
+      // item indices are distinct, so the ranks are a permutation of 0..n-1
+      const int k = lane() < n ? perm[lo + lane()] : 0x7fffffff;
+      const double x = lane() < n ? val[(int64_t)k * vstride] : 0.0;
+      int rank = 0;
+      for (int t = 0; t < n; ++t) rank += __builtin_amdgcn_readlane(k, t) < k;
+      if (lane() < n) v[rank] = x;
+      __syncthreads();
+      if (lane() == 0) {
+        double s = 0.0;
+        for (int t = 0; t < n; ++t) s += v[t];
+        out[a] = s;
+      }
+      __syncthreads();
       continue;
     }
     int P = 1;
