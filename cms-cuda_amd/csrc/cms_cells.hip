@@ -308,11 +308,15 @@ __device__ __forceinline__ int cell_slot(const int* cbeg, const int* cpre, int i
 __device__ __forceinline__ void attention_store(int g, int c, double mn, unsigned long long ckey, int all_atoms,
                                                 int full, const Settings& s, int8_t* atten, int8_t* access,
                                                 int* need, unsigned long long* coinc, int* status) {
+  // atten, access and need come pre-filled in atom order (atten: blocker,
+  // or buried in all-atoms mode; access 0; need 1 in full mode, else 0), so
+  // only the non-blockers of a normal run are stored here: few scattered
+  // stores instead of three per atom
   const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
-  const int8_t at = all_atoms ? ATTEN_BURIED : (dmin >= s.sep ? ATTEN_BLOCKER : ATTEN_BURIED);
-  atten[g] = at;
-  access[g] = 0;
-  need[g] = (full || at > ATTEN_BLOCKER) ? 1 : 0;
+  if (!all_atoms && !(dmin >= s.sep)) {
+    atten[g] = ATTEN_BURIED;
+    if (!full) need[g] = 1;
+  }
   if (ckey != ~0ull) {
     atomicMin(&coinc[c], ckey);
     atomicOr(&status[c], ST_COINCIDENT);

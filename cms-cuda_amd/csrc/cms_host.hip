@@ -705,6 +705,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     const double4* cxyz = ctx->cxyz.as<double4>();
     // every atom is in the region in all-atoms / full-state runs
     const int full = (full_state || all_atoms) ? 1 : 0;
+    // the attention passes store only the non-blockers over these
+    fill_int8(ctx->atten.as<int8_t>(), n, all_atoms ? ATTEN_BURIED : ATTEN_BLOCKER, st);
+    fill_int8(ctx->access.as<int8_t>(), n, 0, st);
+    fill_int(ctx->need.as<int>(), (int64_t)n, full, st);
     if (per_thread) {
       TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
           b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
