@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kstage, kscan, dbase, dgrp;
+  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -241,7 +241,8 @@ static Work work(cms_ctx* ctx) {
   w.tflag = ctx->tflag.as<int8_t>();
   w.tmask = ctx->tmask.as<uint64_t>();
   w.kcnt = ctx->kcnt.as<int>();
-  w.kstage = ctx->kstage.as<DotRec>();
+  w.kpos = ctx->kpos.as<double4>();
+  w.kinfo = ctx->kinfo.as<uint64_t>();
   w.kscan = ctx->kscan.as<int>();
   w.dbase = ctx->dbase.as<int64_t>();
   w.dpos = ctx->dpos.as<double4>();
@@ -267,7 +268,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kstage, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
                   &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -515,7 +516,8 @@ static int size_probes(cms_ctx* ctx, int64_t c) {
   CK(ctx->nearl.ensure(sizeof(int) * (c * NEARCAP + 1)));
   CK(ctx->nnear.ensure(sizeof(int) * (c + 1)));
   CK(ctx->kcnt.ensure(sizeof(int) * (c + 1)));
-  CK(ctx->kstage.ensure(sizeof(DotRec) * KSLOTS * (c + 1)));
+  CK(ctx->kpos.ensure(sizeof(double4) * KSLOTS * (c + 1)));
+  CK(ctx->kinfo.ensure(sizeof(uint64_t) * KSLOTS * (c + 1)));
   CK(ctx->kscan.ensure(sizeof(int) * (c + 1)));
   ctx->cap[M_Q] = std::max(ctx->cap[M_Q], c);
   return CMS_OK;
@@ -849,7 +851,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
                                                                                       meta + M_TE));
   }
   if (est[M_Q] > 0 && s.rp > 0.0) {
-    TIMED("concave_emit", (k_concave_place<<<grid_for(est[M_Q] * KSLOTS, 256, 8192), 256, 0, st>>>(b, w),
+    TIMED("concave_emit", (k_concave_place<<<grid_for(est[M_Q] * KSLOTS, 256, 8192), 256, 0, st>>>(b, s, w),
                            k_concave<true><<<grid_for(est[M_Q] / 64 + 1, 1), 64, 0, st>>>(b, s, w)));
   }
   CKL();

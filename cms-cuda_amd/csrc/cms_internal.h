@@ -66,7 +66,11 @@ struct DotRec {
   int8_t type;
 };
 // Concave dots computed by the count pass per probe before the scan places
-// them (probes with more are emitted again after the scan).
+// them (probes with more are emitted again after the scan).  A staged dot
+// is its position and area (kpos) and one word (kinfo) with what the
+// placement needs besides the probe: owner choice (bits 0-1), burial (2),
+// top vertex mm (3-4), latitude t (8-15), point on the latitude (16-23),
+// latitude count L (24-31) and the latitude's point count (32-39).
 constexpr int KSLOTS = 64;
 
 struct Work {
@@ -115,7 +119,8 @@ struct Work {
   int8_t* tflag;
   uint64_t* tmask;        // count pass: free torus points of each pair (two words)
   int* kcnt;
-  DotRec* kstage;         // KSLOTS staged concave dots per probe
+  double4* kpos;          // KSLOTS staged concave dots per probe
+  uint64_t* kinfo;
   int* kscan;
   int64_t* dbase;         // G*4 stage bases
   // dots
@@ -142,7 +147,7 @@ __global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);
 __global__ void k_compact_count(Count, const int*, const int*, int*);
 __global__ void k_toroid_emit(BatchView, Settings, Work, const int*, const int*);
 template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);
-__global__ void k_concave_place(BatchView, Work);
+__global__ void k_concave_place(BatchView, Settings, Work);
 __global__ void k_compact_index(int, const int*, const int*, int*);
 __global__ void k_near(BatchView, Settings, Work);
 

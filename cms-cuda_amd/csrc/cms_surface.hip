@@ -1466,10 +1466,15 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
         const DotRec d = make_dot(b, s, w, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
                                   make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
                                   make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
-        if (EMIT)
+        if (EMIT) {
           write_dot(w, obase + k, d);
-        else
-          w.kstage[q * KSLOTS + k] = d;
+        } else {
+          w.kpos[q * KSLOTS + k] = d.pos;
+          w.kinfo[q * KSLOTS + k] =
+              (uint64_t)mc | ((uint64_t)(d.bur != 0) << 2) | ((uint64_t)mm << 3) | ((uint64_t)t << 8) |
+              ((uint64_t)(qq - lat_start[t]) << 16) | ((uint64_t)L << 24) |
+              ((uint64_t)(lat_start[t + 1] - lat_start[t]) << 32);
+        }
       }
       total += __popcll(mk);
     }
@@ -1490,15 +1495,31 @@ template __global__ void k_concave<true>(BatchView, Settings, Work);
 
 // Staged concave dots to their place in the dot arrays: one thread per
 // staging slot of probes with at most KSLOTS dots.
-__global__ void k_concave_place(BatchView b, Work w) {
+__global__ void k_concave_place(BatchView b, Settings s, Work w) {
   const int nQ = *w.nQ;
   const int64_t n = (int64_t)nQ * KSLOTS;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int q = (int)(i / KSLOTS), k = (int)(i % KSLOTS);
     const int kc = w.kcnt[q];
     if (kc > KSLOTS || k >= kc) continue;
-    const int grp = group_of_atom(b, w.patom[q].x);
-    write_dot(w, w.dbase[grp * 4 + 3] + w.kscan[q] + k, w.kstage[i]);
+    const int4 at = w.patom[q];
+    const int grp = group_of_atom(b, at.x);
+    const uint64_t f = w.kinfo[i];
+    const int mc = (int)(f & 3), mm = (int)((f >> 3) & 3);
+    // the fields of make_dot: normal (pc - pt) / rp towards the probe
+    // centre (concave dots exist only for rp > 0), descriptors, owner
+    DotRec d;
+    d.pos = w.kpos[i];
+    const V3 pt = ld3(d.pos), pc = ld3(w.ppos[q]);
+    const V3 nml = (pc - pt) / s.rp;
+    d.nml = make_double4(nml.x, nml.y, nml.z, 0.0);
+    d.desc = make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z);
+    d.desc2 = make_int4((int)((f >> 8) & 255), (int)((f >> 16) & 255), (int)((f >> 24) & 255),
+                        (int)((f >> 32) & 255));
+    d.atom = mc == 0 ? at.x : mc == 1 ? at.y : at.z;
+    d.bur = (int8_t)((f >> 2) & 1);
+    d.type = 3;
+    write_dot(w, w.dbase[grp * 4 + 3] + w.kscan[q] + k, d);
   }
 }
 
