@@ -496,8 +496,9 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
                                                         const double4* __restrict__ cxyz) {
-  __shared__ int cj[KCAP];
-  __shared__ double cd[KCAP];
+  // row entries (d^2, j) in one 16 B word each: one LDS read per entry in
+  // the rank loop
+  __shared__ double2 ce[KCAP];
   __shared__ int bj[BCAP];
   __shared__ int cbeg[27], cpre[28];
   const int nR = *w.nR;
@@ -540,7 +541,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
         const uint64_t mn = ballot(isn), mb = ballot(isb);
         if (isn) {
           const int p = cnt + lanes_below(mn);
-          if (p < KCAP) { cj[p] = j; cd[p] = d2; }
+          if (p < KCAP) ce[p] = make_double2(d2, __longlong_as_double((long long)j));
         }
         if (isb) {
           const int p = cntb + lanes_below(mb);
@@ -560,12 +561,14 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     int* row = w.nbr + (int64_t)r * KCAP;
     uint8_t* fwd = w.fwd + (int64_t)r * KCAP;
     for (int p = lane(); p < cnt; p += WAVE) {
-      const double dp = cd[p];
-      const int jp = cj[p];
+      const double dp = ce[p].x;
+      const int jp = (int)__double_as_longlong(ce[p].y);
       int rank = 0, frank = 0;
+#pragma unroll 4
       for (int t = 0; t < cnt; ++t) {
-        const double dt = cd[t];
-        const int jt = cj[t];
+        const double2 e = ce[t];
+        const double dt = e.x;
+        const int jt = (int)__double_as_longlong(e.y);
         rank += (dt < dp) || (dt == dp && jt < jp);
         frank += (jt > g) & (jt < jp);
       }
