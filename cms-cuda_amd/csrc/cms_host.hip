@@ -77,7 +77,7 @@ struct cms_ctx {
   DBuf atten, access, status, coinc, flag, nbscan, nbl, need, rpos, ratom, part;
   DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom, cxyz;
   DBuf nn, nbr, nb, bur, nbmax, fwd;
-  DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp;
+  DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
   DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
@@ -231,6 +231,8 @@ static Work work(cms_ctx* ctx) {
   w.prgrp = ctx->prgrp.as<int>();
   w.near = ctx->nearl.as<int>();
   w.nnear = ctx->nnear.as<int>();
+  w.pstage = ctx->pstage.as<ProbeRec>();
+  w.nps = ctx->nps.as<int>();
   w.ccnt = ctx->ccnt.as<int>();
   w.cscan = ctx->cscan.as<int>();
   w.cmask = ctx->cmask.as<uint64_t>();
@@ -264,7 +266,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
                   &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
-                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag,
+                  &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag, &ctx->pstage, &ctx->nps,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
@@ -484,6 +486,8 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->ccnt.ensure(sizeof(int) * (c + 1)));
   CK(ctx->cscan.ensure(sizeof(int) * (c + 1)));
   CK(ctx->cmask.ensure(sizeof(uint64_t) * (c * CMASK_WORDS + 1)));
+  CK(ctx->pstage.ensure(sizeof(ProbeRec) * PSLOTS * (c + 1)));
+  CK(ctx->nps.ensure(sizeof(int) * (c + 1)));
   ctx->cap[M_R] = std::max(ctx->cap[M_R], c);
   return CMS_OK;
 }
@@ -774,8 +778,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
   if (est[M_R] > 0 && !by_row) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
   if (est[M_P] > 0) {
-    if (by_row)
+    if (by_row) {
+      fill_int(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0, st);
       TIMED("probes_count", k_probe_rows<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+    }
     else
       TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
   }
@@ -794,7 +800,11 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     if ((r = size_probes(ctx, headroom(v)))) return r;
     w = work(ctx);
   }
-  if (est[M_P] > 0) TIMED("probes_emit", k_probes<true><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+  // the row pass staged the probes of large batches (pairs it could not
+  // stage keep flag bit 8 for the emit pass)
+  if (est[M_P] > 0)
+    TIMED("probes_emit", if (by_row) k_probe_place<<<grid_for(est[M_R] * PSLOTS, 256, 16384), 256, 0, st>>>(b, w);
+                         k_probes<true><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
   if (est[M_Q] > 0) TIMED("near", k_near<<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
   CKL();
   // ---- dot counts

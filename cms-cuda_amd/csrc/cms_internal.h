@@ -73,6 +73,19 @@ struct DotRec {
 // latitude count L (24-31) and the latitude's point count (32-39).
 constexpr int KSLOTS = 64;
 
+// Surviving probe candidate staged by the per-row probe pass (large
+// batches) for k_probe_place: both probe positions and the height, the
+// unit normal of the triple, pair, third atom, its row slot and which signs
+// survive (bit 0 "+", bit 1 "-").  PSLOTS per neighbour row; a pair with a
+// survivor past them is left to the probe emit pass (pflag bit 8).
+struct ProbeRec {
+  double4 pp;   // + probe, height
+  double4 pm;   // - probe, 0
+  double4 u;    // uijk, 0
+  int p, k, sl, f;
+};
+constexpr int PSLOTS = 8;
+
 struct Work {
   // per atom
   int8_t* atten;
@@ -108,6 +121,8 @@ struct Work {
   int* prgrp;             // G+1 probe offsets per group
   int* near;
   int* nnear;
+  ProbeRec* pstage;       // PSLOTS staged probe candidates per row (large batches)
+  int* nps;               // per row: slots used
   // dot counts
   int* ccnt;
   int* cscan;
@@ -141,6 +156,7 @@ struct Work {
 template <bool EMIT> __global__ void k_pairs(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
 __global__ void k_probe_rows(BatchView, Settings, Work);
+__global__ void k_probe_place(BatchView, Work);
 template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
 __global__ void k_toroid_count(BatchView, Settings, Work);
 __global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);

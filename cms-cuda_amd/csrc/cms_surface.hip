@@ -485,7 +485,7 @@ constexpr double REACH_MARGIN = 0.01;
 
 // pair-batch flag bits
 constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32,
-              PB_BETWEEN = 64, PB_ARC2 = 128;
+              PB_BETWEEN = 64, PB_ARC2 = 128, PB_EMIT = 256;
 
 __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work w) {
   __shared__ int lj[KCAP];
@@ -570,6 +570,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       }
     }
     __syncthreads();
+    int prow = 0;   // surviving candidates offered to the row's probe slots
     for (int pb = p_lo; pb < p_hi;) {
       // the batch: consecutive pairs whose triangle items fit the queue
       const int avail = min(PR_PAIRS, p_hi - pb);
@@ -713,17 +714,16 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       for (int q0 = 0; q0 < nc; q0 += WAVE) {
         const int q = q0 + lane();
         bool okp = false, okm = false;
-        int a = 0, j = -1, k = -1;
-        V3 pp{0, 0, 0}, pm{0, 0, 0};
+        int a = 0, j = -1, k = -1, sl = 0;
+        V3 pp{0, 0, 0}, pm{0, 0, 0}, uijk{0, 0, 0};
+        double h = 0.0;
         if (q < nc) {
           const int e = queue[q];
           a = e >> 8;
-          const int sl = e & 255;
+          sl = e & 255;
           if (!(bflag[a] & (PB_KILLED | PB_COVERED))) {
             j = bj[a];
             k = lj[sl];
-            V3 uijk;
-            double h;
             const double4 ak = lx[sl], pu = bu[a], pt = bt[a];
             okp = okm = probe_points(xi, eri, ld3(ak), ak.w + s.rp, ld3(pu), ld3(pt), &pp, &pm, &uijk, &h);
           }
@@ -750,6 +750,28 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         }
         if (okp) atomicAdd(&bnp[a], 1);
         if (okm) atomicAdd(&bnm[a], 1);
+        // survivors go to the row's probe slots for k_probe_place; a pair
+        // with one past them is written by the emit pass instead
+        const uint64_t sm = ballot(okp || okm);
+        if (sm) {
+          const int idx = prow + lanes_below(sm);
+          prow += __popcll(sm);
+          if (okp || okm) {
+            if (idx < PSLOTS) {
+              ProbeRec pr;
+              pr.pp = make_double4(pp.x, pp.y, pp.z, h);
+              pr.pm = make_double4(pm.x, pm.y, pm.z, 0.0);
+              pr.u = make_double4(uijk.x, uijk.y, uijk.z, 0.0);
+              pr.p = pb + a;
+              pr.k = k;
+              pr.sl = sl;
+              pr.f = (okp ? 1 : 0) | (okm ? 2 : 0);
+              w.pstage[r * PSLOTS + idx] = pr;
+            } else {
+              atomicOr(&bflag[a], PB_EMIT);
+            }
+          }
+        }
       }
       __syncthreads();
 
@@ -832,7 +854,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         } else if (!(f & PB_KILLED)) {
           np = bnp[lane()];
           nm = bnm[lane()];
-          if (np + nm > 0) pf |= 8;
+          if (np + nm > 0 && (f & PB_EMIT)) pf |= 8;
         }
         // the pair record (what the pair pass's emit would write); the
         // emit passes read the geometry only of pairs with probes or
@@ -854,6 +876,53 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       }
       pb += nb;
       __syncthreads();
+    }
+    if (lane() == 0) w.nps[r] = min(prow, PSLOTS);
+  }
+}
+
+// Probes staged by k_probe_rows, one thread per row slot, written where the
+// emit pass would write them: the pair's "+" / "-" probes in row-slot order
+// of k after the pair's scanned offsets, with the atom order, height and
+// normal sign of k_probes<true> (_core.py:1590-1643).  Pairs flagged for the
+// emit pass (a survivor past the slots) are left to it.
+__global__ __launch_bounds__(256) void k_probe_place(BatchView b, Work w) {
+  const int64_t n = (int64_t)*w.nR * PSLOTS;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = c / PSLOTS;
+    const int e = (int)(c % PSLOTS);
+    const int cnt = w.nps[r];
+    if (e >= cnt) continue;
+    const ProbeRec* rec = w.pstage + r * PSLOTS;
+    const ProbeRec pr = rec[e];
+    const int p = pr.p;
+    if (w.pflag[p] & 8) continue;
+    int rkp = 0, rkm = 0;   // earlier survivors of the pair (by row slot)
+    for (int t = 0; t < cnt; ++t) {
+      const ProbeRec& o = rec[t];
+      if (o.p == p && o.sl < pr.sl) {
+        rkp += o.f & 1;
+        rkm += (o.f >> 1) & 1;
+      }
+    }
+    const int i = w.pi[p], j = w.pj[p];
+    const int grp = group_of_atom(b, i);
+    const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
+    if (pr.f & 1) {
+      const int o = w.prscan[2ll * g0 + (p - g0)] + rkp;
+      if (o < w.probe_cap) {
+        w.patom[o] = make_int4(i, j, pr.k, 0);
+        w.ppos[o] = pr.pp;
+        w.palt[o] = make_double4(pr.u.x, pr.u.y, pr.u.z, 0.0);
+      }
+    }
+    if (pr.f & 2) {
+      const int o = w.prscan[2ll * g0 + (g1 - g0) + (p - g0)] + rkm;
+      if (o < w.probe_cap) {
+        w.patom[o] = make_int4(j, i, pr.k, 0);
+        w.ppos[o] = make_double4(pr.pm.x, pr.pm.y, pr.pm.z, pr.pp.w);
+        w.palt[o] = make_double4(-pr.u.x, -pr.u.y, -pr.u.z, 0.0);
+      }
     }
   }
 }
