@@ -112,6 +112,9 @@ struct cms_ctx {
   // units than rows), one per neighbour row large ones (full lanes); the
   // environment variable CMS_PROBE_COUNT = "pairs" / "rows" fixes the choice
   int probe_count_mode = 0;   // 0 by size, 1 pairs, 2 rows
+  // CMS_PROBE_SLOTS = 0..PSLOTS: probe slots per row the per-row pass uses
+  // (fewer sends more pairs through the emit pass; same results)
+  int probe_slots = PSLOTS;
   bool capturing = false;
   hipStream_t gstream = nullptr;    // stream the graph is captured on and launched to
   hipEvent_t gev0 = nullptr, gev1 = nullptr;
@@ -233,6 +236,7 @@ static Work work(cms_ctx* ctx) {
   w.nnear = ctx->nnear.as<int>();
   w.pstage = ctx->pstage.as<ProbeRec>();
   w.nps = ctx->nps.as<int>();
+  w.pslots = ctx->probe_slots;
   w.ccnt = ctx->ccnt.as<int>();
   w.cscan = ctx->cscan.as<int>();
   w.cmask = ctx->cmask.as<uint64_t>();
@@ -331,6 +335,9 @@ int cms_create(int device, void* stream, cms_ctx** out) {
   if (const char* pm = std::getenv("CMS_PROBE_COUNT")) {
     if (!std::strcmp(pm, "pairs")) ctx->probe_count_mode = 1;
     if (!std::strcmp(pm, "rows")) ctx->probe_count_mode = 2;
+  }
+  if (const char* ps = std::getenv("CMS_PROBE_SLOTS")) {
+    ctx->probe_slots = std::min(std::max(std::atoi(ps), 0), PSLOTS);
   }
   ctx->stream = static_cast<hipStream_t>(stream);
   if (ctx->nresc.ensure(sizeof(unsigned long long) * 2) != hipSuccess ||

@@ -123,6 +123,7 @@ struct Work {
   int* nnear;
   ProbeRec* pstage;       // PSLOTS staged probe candidates per row (large batches)
   int* nps;               // per row: slots used
+  int pslots;             // slots per row in use (<= PSLOTS)
   // dot counts
   int* ccnt;
   int* cscan;

@@ -757,7 +757,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           const int idx = prow + lanes_below(sm);
           prow += __popcll(sm);
           if (okp || okm) {
-            if (idx < PSLOTS) {
+            if (idx < w.pslots) {
               ProbeRec pr;
               pr.pp = make_double4(pp.x, pp.y, pp.z, h);
               pr.pm = make_double4(pm.x, pm.y, pm.z, 0.0);
@@ -877,7 +877,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       pb += nb;
       __syncthreads();
     }
-    if (lane() == 0) w.nps[r] = min(prow, PSLOTS);
+    if (lane() == 0) w.nps[r] = min(prow, w.pslots);
   }
 }
 

@@ -157,22 +157,56 @@ def test_graph_replays_match_eager(golden_complex):
     bc.close()
 
 
+def _probe_lists(bc, counts):
+    """Every complex's probes (atoms, height, position, normal) via the C ABI."""
+    import ctypes
+    import torch
+    out = []
+    for c in range(bc.B):
+        nn = ctypes.c_int64()
+        bc.ctx("cms_probe_count", c, ctypes.byref(nn))
+        n = nn.value
+        assert n == counts[c, 7]
+        atoms = torch.empty((max(n, 1), 3), dtype=torch.int32, device="cuda")
+        h = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+        p = torch.empty((max(n, 1), 3), dtype=torch.float64, device="cuda")
+        a = torch.empty((max(n, 1), 3), dtype=torch.float64, device="cuda")
+        if n:
+            bc.ctx("cms_copy_probes", c, ctypes.c_void_p(atoms.data_ptr()), ctypes.c_void_p(h.data_ptr()),
+                   ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(a.data_ptr()))
+        out.append(tuple(t[:n].cpu().numpy() for t in (atoms, h, p, a)))
+    return out
+
+
 def test_probe_count_kernels_agree(golden_complex, monkeypatch):
     """The probes count pass has a per-pair and a per-row form (chosen by
-    batch size); both must give identical counts and results."""
+    batch size); the per-row form stages up to 8 surviving probes per row
+    for placement after the scan and leaves pairs past them to the emit
+    pass.  Per pair, per row, and per row with one slot (staged and emitted
+    probes mixed) must give identical counts, probe lists and results."""
     from cms_cuda.batch import BatchCalculator
     cs = _batch(golden_complex, [(400, 40), (700, 200), (300, 300), (900, 500)], 700)
     outs = []
-    for mode in ("pairs", "rows"):
+    for mode, slots in (("pairs", None), ("rows", None), ("rows", "1")):
         monkeypatch.setenv("CMS_PROBE_COUNT", mode)
-        say("probes count pass per %s, %d complexes" % (mode[:-1], len(cs)))
+        if slots is None:
+            monkeypatch.delenv("CMS_PROBE_SLOTS", raising=False)
+        else:
+            monkeypatch.setenv("CMS_PROBE_SLOTS", slots)
+        say("probes count pass per %s (slots %s), %d complexes" % (mode[:-1], slots, len(cs)))
         bc = BatchCalculator()
         bc.load(cs)
         o = bc.contact_ms(0)
         o["sc"] = bc.shape_complementarity(0)
         o["counts"] = bc.counts()
+        o["probes"] = _probe_lists(bc, o["counts"])
         bc.close()
         outs.append(o)
-    a, b = outs
-    assert np.array_equal(a["counts"], b["counts"])
-    assert _same(a, b)
+    a = outs[0]
+    assert sum(len(pl[1]) for pl in a["probes"]) > 100
+    for b in outs[1:]:
+        assert np.array_equal(a["counts"], b["counts"])
+        for pa, pb in zip(a["probes"], b["probes"]):
+            for x, y in zip(pa, pb):
+                assert np.array_equal(x, y)
+        assert _same({k: v for k, v in a.items() if k != "probes"}, {k: v for k, v in b.items() if k != "probes"})
