@@ -1023,6 +1023,7 @@ __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s,
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) {
   __shared__ double4 lx[KCAP];
+  __shared__ uint16_t klist[CMASK_WORDS * WAVE];   // emit: kept points under the count pass's mask
   __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
   __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
   __shared__ int lat_start[MAX_SUBDIV + 1];
@@ -1113,12 +1114,42 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
         for (int sl = lane(); sl < nn; sl += WAVE) lx[sl] = b.atom[row[sl]];
         __syncthreads();
       }
-      for (int q0 = 0; q0 < C; q0 += WAVE) {
+      // the emit pass takes the collision outcome of the first
+      // CMASK_WORDS * 64 points from the count pass: their kept points are
+      // listed (in point order) and placed one per lane; the points past
+      // them are tested again below
+      int q_from = 0;
+      if (EMIT) {
+        const int cm = min(C, CMASK_WORDS * WAVE);
+        int nk = 0;
+        for (int wd = 0; wd * WAVE < cm; ++wd) {
+          const uint64_t m = w.cmask[(int64_t)r * CMASK_WORDS + wd];
+          if ((m >> lane()) & 1ull) klist[nk + lanes_below(m)] = (uint16_t)(wd * WAVE + lane());
+          nk += __popcll(m);
+        }
+        __syncthreads();
+        for (int k0 = 0; k0 < nk; k0 += WAVE) {
+          const int kq = k0 + lane();
+          if (kq < nk) {
+            const int q = klist[kq];
+            int lo = 0, hi = L;  // find t with lat_start[t] <= q < lat_start[t+1]
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (lat_start[mid] <= q) lo = mid; else hi = mid;
+            }
+            const int t = lo;
+            const V3 pt = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], lat_delta[t],
+                                       q - lat_start[t]);
+            const V3 pc = xi + (pt - xi) * scale;
+            store_dot(b, s, w, obase + kq, pt, pc, lat_ps[t] * cs, g, r, 1, make_int4(DK_CONVEX, g, row[0], -1),
+                      make_int4(t, q - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
+          }
+        }
+        total = nk;
+        q_from = cm;
+      }
+      for (int q0 = q_from; q0 < C; q0 += WAVE) {
         const int q = q0 + lane();
-        // the emit pass takes the collision outcome of the first
-        // CMASK_WORDS * 64 points from the count pass
-        const bool masked = EMIT && q0 < CMASK_WORDS * WAVE;
-        const uint64_t kmask = masked ? w.cmask[(int64_t)r * CMASK_WORDS + q0 / WAVE] : 0ull;
         bool keep = false;
         V3 pt{0, 0, 0}, pc{0, 0, 0};
         int t = 0;
@@ -1132,9 +1163,7 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
           const int kk = q - lat_start[t];
           pt = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], lat_delta[t], kk);
           pc = xi + (pt - xi) * scale;
-          if (masked) {
-            keep = (kmask >> lane()) & 1ull;
-          } else {
+          {
             keep = true;
             for (int sl = 1; sl < nn; ++sl) {
               const V3 d = ld3(lx[sl]) - pc;
