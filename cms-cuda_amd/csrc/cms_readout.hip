@@ -258,39 +258,66 @@ __device__ double np_leaf_sum(const double* a, int64_t stride, int n) {
   return res;
 }
 
-constexpr int PW_MAXTOK = 2048;
+constexpr int PW_MAXN = 1024;   // tree nodes held in LDS (n up to ~32k values)
+constexpr int PW_MAXLEV = 32;
 
-// Block-wide np.sum of a[0..n) (stride in doubles).  Thread 0 lays out the
-// pairwise tree as postfix tokens (leaf / add), all threads sum the leaves,
-// thread 0 evaluates the postfix.  Returns the sum in every thread.
+// Block-wide np.sum of a[0..n) (stride in doubles), the pairwise tree of
+// np_leaf_sum leaves.  Wavefront 0 lays the tree out level by level (the
+// children of a level's split nodes, in order, form the next level), all
+// threads sum the leaves, and the internal nodes are added bottom-up one
+// level at a time as left + right -- the association of the recursion.  A
+// tree with more than PW_MAXN nodes is followed by one thread instead.
+// Returns the sum in every thread.
 __device__ double block_np_sum(const double* a, int64_t stride, int n) {
-  __shared__ int tok_lo[PW_MAXTOK];
-  __shared__ int tok_n[PW_MAXTOK];   // > 0 leaf of that length, 0 = add, -1 empty leaf
-  __shared__ double leaf[PW_MAXTOK];
+  __shared__ int nd_lo[PW_MAXN], nd_m[PW_MAXN], nd_ch[PW_MAXN];
+  __shared__ double nd_v[PW_MAXN];
+  __shared__ int lev_start[PW_MAXLEV + 1];
+  __shared__ int nlev, overflow;
   __shared__ int stk_n[64], stk_lo[64], stk_v[64];
   __shared__ double vstk[64];
-  __shared__ int ntok, overflow;
   __shared__ double result;
-  if (threadIdx.x == 0) {
-    int sp = 0, nt = 0, of = 0;
-    stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
-    while (sp >= 0) {
-      const int m = stk_n[sp], lo = stk_lo[sp], v = stk_v[sp];
-      --sp;
-      if (nt >= PW_MAXTOK || sp > 60) { of = 1; break; }
-      if (m <= 128) {
-        tok_lo[nt] = lo; tok_n[nt] = m > 0 ? m : -1; ++nt;
-      } else if (!v) {
-        int n2 = m / 2;
-        n2 -= n2 % 8;
-        ++sp; stk_n[sp] = m; stk_lo[sp] = lo; stk_v[sp] = 1;
-        ++sp; stk_n[sp] = m - n2; stk_lo[sp] = lo + n2; stk_v[sp] = 0;
-        ++sp; stk_n[sp] = n2; stk_lo[sp] = lo; stk_v[sp] = 0;
-      } else {
-        tok_lo[nt] = 0; tok_n[nt] = 0; ++nt;
-      }
+  if (threadIdx.x < WAVE) {
+    int s0 = 0, cnt = 1, lev = 0, of = 0;
+    if (lane() == 0) {
+      nd_lo[0] = 0;
+      nd_m[0] = n;
+      lev_start[0] = 0;
     }
-    ntok = nt; overflow = of;
+    for (;;) {
+      // the split nodes of level lev get two children each at the next level
+      const int next = s0 + cnt;
+      int ncnt = 0;
+      for (int b0 = 0; b0 < cnt; b0 += WAVE) {
+        const int k = b0 + lane();
+        const int m = k < cnt ? nd_m[s0 + k] : 0;
+        const bool split = k < cnt && m > 128;
+        const int c = split ? 2 : 0;
+        const int inc = wave_incl_scan(c);
+        const int pos = next + ncnt + inc - c;
+        if (k < cnt) nd_ch[s0 + k] = split ? pos : -1;
+        if (split && pos + 1 < PW_MAXN) {
+          const int lo = nd_lo[s0 + k];
+          int n2 = m / 2;
+          n2 -= n2 % 8;
+          nd_lo[pos] = lo;
+          nd_m[pos] = n2;
+          nd_lo[pos + 1] = lo + n2;
+          nd_m[pos + 1] = m - n2;
+        }
+        ncnt += __shfl(inc, WAVE - 1, WAVE);
+      }
+      if (ncnt == 0) break;
+      if (next + ncnt > PW_MAXN || lev + 1 >= PW_MAXLEV) { of = 1; break; }
+      s0 = next;
+      cnt = ncnt;
+      ++lev;
+      if (lane() == 0) lev_start[lev] = s0;
+    }
+    if (lane() == 0) {
+      nlev = lev + 1;
+      lev_start[lev + 1] = s0 + cnt;
+      overflow = of;
+    }
   }
   __syncthreads();
   if (overflow) {
@@ -322,26 +349,18 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
     __syncthreads();
     return r;
   }
-  for (int t = threadIdx.x; t < ntok; t += blockDim.x) {
-    const int m = tok_n[t];
-    if (m != 0) leaf[t] = np_leaf_sum(a + (int64_t)tok_lo[t] * stride, stride, m > 0 ? m : 0);
-  }
+  const int nodes = lev_start[nlev];
+  for (int k = threadIdx.x; k < nodes; k += blockDim.x)
+    if (nd_ch[k] < 0) nd_v[k] = np_leaf_sum(a + (int64_t)nd_lo[k] * stride, stride, nd_m[k]);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int vp = 0;
-    for (int t = 0; t < ntok; ++t) {
-      if (tok_n[t] != 0) {
-        vstk[vp++] = leaf[t];
-      } else {
-        const double b2 = vstk[--vp];
-        const double b1 = vstk[--vp];
-        vstk[vp++] = b1 + b2;
-      }
+  for (int lev = nlev - 2; lev >= 0; --lev) {
+    for (int k = lev_start[lev] + threadIdx.x; k < lev_start[lev + 1]; k += blockDim.x) {
+      const int c = nd_ch[k];
+      if (c >= 0) nd_v[k] = nd_v[c] + nd_v[c + 1];
     }
-    result = vp > 0 ? vstk[0] : 0.0;
+    __syncthreads();
   }
-  __syncthreads();
-  const double r = result;
+  const double r = nd_v[0];
   __syncthreads();
   return r;
 }

@@ -210,3 +210,29 @@ def test_probe_count_kernels_agree(golden_complex, monkeypatch):
             for x, y in zip(pa, pb):
                 assert np.array_equal(x, y)
         assert _same({k: v for k, v in a.items() if k != "probes"}, {k: v for k, v in b.items() if k != "probes"})
+
+
+def test_group_totals_numpy_order(golden_complex):
+    """Every molecule's CMS total is np.sum of its per-atom values, bit for
+    bit: NumPy's pairwise tree (leaves of at most 128 values with eight
+    interleaved accumulators) evaluated on the GPU, for groups from under
+    128 atoms (one leaf) to thousands (several tree levels)."""
+    from cms_cuda.batch import BatchCalculator
+    cs = _batch(golden_complex, [(60, 20), (3000, 900), (1500, 130), (4200, 1700)], 910)
+    say("group totals vs np.sum, %d complexes" % len(cs))
+    bc = BatchCalculator()
+    bc.load(cs)
+    o = bc.contact_ms(0)
+    bc.close()
+    per = o["per_atom"]
+    off = 0
+    sizes = []
+    for c, (bx, br, tx, tr) in enumerate(cs):
+        nb, nt = len(br), len(tr)
+        b_seg = per[off:off + nb]
+        t_seg = per[off + nb:off + nb + nt]
+        assert o["binder"][c] == np.sum(b_seg), c
+        assert o["target"][c] == np.sum(t_seg), c
+        sizes += [nb, nt]
+        off += nb + nt
+    assert min(sizes) < 128 and max(sizes) > 4000
