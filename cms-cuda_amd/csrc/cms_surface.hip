@@ -484,6 +484,9 @@ constexpr int PR_RELW = KCAP / WAVE;    // reach mask words per pair
 constexpr double REACH_MARGIN = 0.01;
 
 // pair-batch flag bits
+// latt: attention value of the row atom (low bits) and whether it passes
+// the triangle test's d(i, k) < eri + erk (forward atoms only)
+constexpr int LATT_ATT = 0x3f, LATT_IK = 0x40;
 constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32,
               PB_BETWEEN = 64, PB_ARC2 = 128, PB_EMIT = 256;
 
@@ -525,7 +528,13 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       }
     }
     __syncthreads();
-    // forward atoms by ascending index (ordered by the neighbour pass)
+    // forward atoms by ascending index (ordered by the neighbour pass); the
+    // triangle test's d(i, k) < eri + erk depends on k alone, so it is
+    // taken once per forward atom (bit LATT_IK of latt, above the
+    // attention values)
+    const double4 ai = b.atom[i];
+    const V3 xi = ld3(ai);
+    const double eri = ai.w + s.rp;
     int F = 0;
     for (int s0 = 0; s0 < nn; s0 += WAVE) {
       const int sl = s0 + lane();
@@ -537,11 +546,10 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         const uint8_t sl = fw[f];
         fws[f] = sl;
         frk[sl] = (uint8_t)f;
+        const double4 ak = lx[sl];
+        if (norm_lt(ld3(ak) - xi, eri + (ak.w + s.rp))) latt[sl] = (int8_t)(latt[sl] | LATT_IK);
       }
     }
-    const double4 ai = b.atom[i];
-    const V3 xi = ld3(ai);
-    const double eri = ai.w + s.rp;
     const int grp = group_of_atom(b, i);
     const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
     const int8_t iatt = w.atten[i];
@@ -562,7 +570,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           bool between;
           const double4 aj = lx[sl];
           rel = pair_torus(xi, ai.w, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between) &&
-                pair_relevant(iatt, latt[sl], nbmax, lj[sl]);
+                pair_relevant(iatt, (int8_t)(latt[sl] & LATT_ATT), nbmax, lj[sl]);
         }
         const uint64_t m = ballot(rel);
         if (rel) pslots[np + lanes_below(m)] = (uint8_t)sl;
@@ -597,7 +605,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         bt[lane()] = make_double4(t.x, t.y, t.z, 0.0);
         bx[lane()] = aj;
         bdens[lane()] = b.dens[j];
-        const int8_t jatt = latt[slot];
+        const int8_t jatt = (int8_t)(latt[slot] & LATT_ATT);
         const bool bij = iblock && jatt <= ATTEN_BLOCKER;
         bflag[lane()] = (pair_toroid(iatt, jatt, s.rp) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0) |
                         (between ? PB_BETWEEN : 0) | (jatt >= ATTEN_2 ? PB_ARC2 : 0);
@@ -619,11 +627,11 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         if (it < items) {
           e = queue[it];
           const int a = e >> 8, sl = e & 255;
+          const int lk = latt[sl];
           const double4 ak = lx[sl], aj = bx[a];
-          const V3 xk = ld3(ak);
           const double erk = ak.w + s.rp, erj = aj.w + s.rp;
-          valid = norm_lt(xk - ld3(aj), erj + erk) && norm_lt(xk - xi, eri + erk) &&
-                  !((bflag[a] & PB_BLOCKERS) && latt[sl] <= ATTEN_BLOCKER);
+          valid = (lk & LATT_IK) && !((bflag[a] & PB_BLOCKERS) && (lk & LATT_ATT) <= ATTEN_BLOCKER) &&
+                  norm_lt(ld3(ak) - ld3(aj), erj + erk);
         }
         const uint64_t m = ballot(valid);
         if (valid) queue[nq + lanes_below(m)] = (uint16_t)e;
