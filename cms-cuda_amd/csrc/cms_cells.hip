@@ -267,6 +267,32 @@ __device__ __forceinline__ uint32_t coincident_cells(const GridInfo& gi, double 
     if (((mask) >> k_) & 1u)                                                         \
       for (int cellvar = cell_of((gi), (ix), (iy), (iz), k_), once_ = 1; once_ && cellvar >= 0; once_ = 0)
 
+// Atom ranges of molecule slot ms (0 or 1) in the cells z0 .. z1 (at most
+// three, inside the grid) of column (cx, cy): consecutive cells, so their
+// bounds come from one run of cstart and are loaded together; col_index
+// maps a flattened position over the ranges to a cell-list slot.
+struct ColRanges {
+  int b[3], n[3], total;
+};
+__device__ __forceinline__ ColRanges column_ranges(const GridInfo& gi, const int* __restrict__ cstart, int cx,
+                                                   int cy, int z0, int z1, int ms) {
+  ColRanges r;
+  r.total = 0;
+  const int base = 2 * (gi.cell0 + (cx * gi.ny + cy) * gi.nz + z0) + ms;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bool ok = z0 + k <= z1;
+    const int bb = ok ? cstart[base + 2 * k] : 0, ee = ok ? cstart[base + 2 * k + 1] : 0;
+    r.b[k] = bb;
+    r.n[k] = ee - bb;
+    r.total += ee - bb;
+  }
+  return r;
+}
+__device__ __forceinline__ int col_index(const ColRanges& r, int t) {
+  return t < r.n[0] ? r.b[0] + t : (t < r.n[0] + r.n[1] ? r.b[1] + (t - r.n[0]) : r.b[2] + (t - r.n[0] - r.n[1]));
+}
+
 // Flattened candidate list of the cells around (ix, iy, iz) selected by
 // mask, molecule range sel: cbeg[k] / cpre[k] are the start in catom / cxyz and the
 // prefix count of cell k; cpre[27] is the total.  Wave-cooperative (lanes
@@ -434,11 +460,15 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     double mn = INFINITY;
     if (!all_atoms) {
-      FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, 0x7ffffffu, cell) {
-        const int lo = cstart[2 * cell + 1 - m], hi = cstart[2 * cell + 2 - m];
+      // the other molecule's atoms of the 27 cells, by (x, y) column
+      const int z0 = max(iz - 1, 0), z1 = min(iz + 1, gi.nz - 1);
+      for (int col = 0; col < 9; ++col) {
+        const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
+        if (cx < 0 || cx >= gi.nx || cy < 0 || cy >= gi.ny) continue;
+        const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, 1 - m);
 #pragma unroll 4
-        for (int t = lo; t < hi; ++t) {
-          const V3 d = xi - ld3(cxyz[t]);
+        for (int t = 0; t < cr.total; ++t) {
+          const V3 d = xi - ld3(cxyz[col_index(cr, t)]);
           mn = fmin(mn, dot_seq(d, d));
         }
       }
@@ -473,13 +503,22 @@ __global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s,
     const V3 xi = ld3(ai);
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    // the bridge distance is at most r_i + r_max + 2 rp
+    // the bridge distance is at most r_i + r_max + 2 rp; the cells within
+    // it form one z-run per (x, y) column (the column's middle cell is the
+    // nearest)
     const uint32_t within = cells_within(gi, ai.x, ai.y, ai.z, ix, iy, iz, (ai.w + gi.rmax) + 2.0 * s.rp);
-    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, within, cell) {
+    for (int col = 0; col < 9; ++col) {
+      const uint32_t zb = (within >> (3 * col)) & 7u;
+      const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
+      if (!zb || cx < 0 || cx >= gi.nx || cy < 0 || cy >= gi.ny) continue;
+      const int z0 = max(iz + __builtin_ctz(zb) - 1, 0), z1 = min(iz + 31 - __builtin_clz(zb) - 1, gi.nz - 1);
+      if (z0 > z1) continue;
+      const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, m);
 #pragma unroll 4
-      for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
-        const int j = catom[t];
-        const double4 aj = cxyz[t];
+      for (int t = 0; t < cr.total; ++t) {
+        const int slot = col_index(cr, t);
+        const int j = catom[slot];
+        const double4 aj = cxyz[slot];
         const V3 d = xi - ld3(aj);
         const double br = (ai.w + aj.w) + 2.0 * s.rp;
         if (dot_seq(d, d) < br * br) need[j] = 1;
