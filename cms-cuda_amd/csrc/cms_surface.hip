@@ -1455,8 +1455,11 @@ __global__ __launch_bounds__(64) void k_toroid_emit(BatchView b, Settings s, Wor
 // _core.py:2069-2333.  Spherical triangle of each probe: latitude arcs from
 // the highest vertex towards -alt, circles, the three plane tests, low-probe
 // rejection, and the closest of the three atoms as owner.
+constexpr int CC_BLOCK = 512;   // concave candidate points per prefilter block
+
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w) {
+  __shared__ uint16_t clist[CC_BLOCK];
   __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
   __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
   __shared__ int lat_start[MAX_SUBDIV + 1];
@@ -1533,56 +1536,91 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
       obase = w.dbase[grp * 4 + 3] + w.kscan[q];
     }
     int total = 0;
-    for (int q0 = 0; q0 < C; q0 += WAVE) {
-      const int qq = q0 + lane();
-      bool keep = false;
-      V3 pts{0, 0, 0};
-      int t = 0;
-      if (qq < C) {
-        int lo = 0, hi = L;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (lat_start[mid] <= qq) lo = mid; else hi = mid;
+    // candidates in blocks of CC_BLOCK: a float pass drops the points that
+    // lie clearly outside one of the three planes, the rest are listed in
+    // order in LDS and go through the exact tests.  The float point differs
+    // from the computed one by at most rad * (|dcos| + |dsin|) <= 1.3e-6 rp
+    // (float rounding of the angle and float cos / sin), far inside the
+    // margin 1e-4 (1 + rp), so every dropped point fails a plane test.
+    const double pf = 1e-4 * (1.0 + rp);
+    for (int c0 = 0; c0 < C; c0 += CC_BLOCK) {
+      const int c1 = min(C, c0 + CC_BLOCK);
+      int ns = 0;
+      for (int q0 = c0; q0 < c1; q0 += WAVE) {
+        const int qq = q0 + lane();
+        bool cand = false;
+        if (qq < c1) {
+          int lo = 0, hi = L;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lat_start[mid] <= qq) lo = mid; else hi = mid;
+          }
+          const int t = lo;
+          const double ang = lat_delta[t] * ((double)(qq - lat_start[t]) + 0.5);
+          float sa, ca;
+          sincosf((float)ang, &sa, &ca);
+          const V3 rel = subdiv_point_cs(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], (double)ca,
+                                         (double)sa);
+          cand = !(dot_seq(rel, vec0) > pf || dot_seq(rel, vec1) > pf || dot_seq(rel, vec2) > pf);
         }
-        t = lo;
-        const int kk = qq - lat_start[t];
-        const V3 rel = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t],
-                                    lat_delta[t], kk);
-        const bool bail = dot_seq(rel, vec0) >= 0.0 || dot_seq(rel, vec1) >= 0.0 ||
-                          dot_seq(rel, vec2) >= 0.0;
-        if (!bail) {
-          pts = rel + p;
-          keep = true;
-          for (int l = 0; l < nnear; ++l) {
-            const V3 d = pts - ld3(lnear[l]);
-            if (dot_alt(d, d) < rp2) { keep = false; break; }
+        const uint64_t cm = ballot(cand);
+        if (cand) clist[ns + lanes_below(cm)] = (uint16_t)(qq - c0);
+        ns += __popcll(cm);
+      }
+      __syncthreads();
+      for (int s0 = 0; s0 < ns; s0 += WAVE) {
+        const int si = s0 + lane();
+        bool keep = false;
+        V3 pts{0, 0, 0};
+        int t = 0, qq = 0;
+        if (si < ns) {
+          qq = c0 + clist[si];
+          int lo = 0, hi = L;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lat_start[mid] <= qq) lo = mid; else hi = mid;
+          }
+          t = lo;
+          const int kk = qq - lat_start[t];
+          const V3 rel = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t],
+                                      lat_delta[t], kk);
+          const bool bail = dot_seq(rel, vec0) >= 0.0 || dot_seq(rel, vec1) >= 0.0 ||
+                            dot_seq(rel, vec2) >= 0.0;
+          if (!bail) {
+            pts = rel + p;
+            keep = true;
+            for (int l = 0; l < nnear; ++l) {
+              const V3 d = pts - ld3(lnear[l]);
+              if (dot_alt(d, d) < rp2) { keep = false; break; }
+            }
           }
         }
-      }
-      const uint64_t mk = ballot(keep);
-      const int k = total + lanes_below(mk);
-      if (keep && (EMIT || k < KSLOTS)) {
-        int mc = 0;
-        double dbest = norm_seq(pts - xa[0]) - ra[0];
-        for (int a = 1; a < 3; ++a) {
-          const double d = norm_seq(pts - xa[a]) - ra[a];
-          if (d < dbest) { dbest = d; mc = a; }
+        const uint64_t mk = ballot(keep);
+        const int k = total + lanes_below(mk);
+        if (keep && (EMIT || k < KSLOTS)) {
+          int mc = 0;
+          double dbest = norm_seq(pts - xa[0]) - ra[0];
+          for (int a = 1; a < 3; ++a) {
+            const double d = norm_seq(pts - xa[a]) - ra[a];
+            if (d < dbest) { dbest = d; mc = a; }
+          }
+          const int owner = aid[mc];
+          const DotRec d = make_dot(b, s, w, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
+                                    make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
+                                    make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
+          if (EMIT) {
+            write_dot(w, obase + k, d);
+          } else {
+            w.kpos[q * KSLOTS + k] = d.pos;
+            w.kinfo[q * KSLOTS + k] =
+                (uint64_t)mc | ((uint64_t)(d.bur != 0) << 2) | ((uint64_t)mm << 3) | ((uint64_t)t << 8) |
+                ((uint64_t)(qq - lat_start[t]) << 16) | ((uint64_t)L << 24) |
+                ((uint64_t)(lat_start[t + 1] - lat_start[t]) << 32);
+          }
         }
-        const int owner = aid[mc];
-        const DotRec d = make_dot(b, s, w, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
-                                  make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
-                                  make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
-        if (EMIT) {
-          write_dot(w, obase + k, d);
-        } else {
-          w.kpos[q * KSLOTS + k] = d.pos;
-          w.kinfo[q * KSLOTS + k] =
-              (uint64_t)mc | ((uint64_t)(d.bur != 0) << 2) | ((uint64_t)mm << 3) | ((uint64_t)t << 8) |
-              ((uint64_t)(qq - lat_start[t]) << 16) | ((uint64_t)L << 24) |
-              ((uint64_t)(lat_start[t + 1] - lat_start[t]) << 32);
-        }
+        total += __popcll(mk);
       }
-      total += __popcll(mk);
+      __syncthreads();
     }
     if (!EMIT && lane() == 0) w.kcnt[q] = total;
     __syncthreads();
