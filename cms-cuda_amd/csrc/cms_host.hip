@@ -1137,8 +1137,9 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   const int gA = grid_for(std::max<int64_t>(estA, 1), 256, 4096);
   const int gB = grid_for(std::max<int64_t>(estB, 1), 256, 4096);
   GridInfo* tg = ctx->tgrid.as<GridInfo>();
-  k_dot_grid_setup<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->agrpd.as<int>(), ctx->axyz.as<double4>(), ctx->s.band,
-                                                          tg, ctx->tncell.as<int>());
+  k_dot_grid_setup<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->agrpd.as<int>(), ctx->bgrp.as<int>(),
+                                                          ctx->bxyz.as<double4>(), ctx->s.band, tg,
+                                                          ctx->tncell.as<int>());
   scan_excl(ctx->tncell.as<int>(), ctx->tncscan.as<int>(), G, part, st);
   k_grid_offsets<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, ctx->tncscan.as<int>(), tg);
   const Count cCells = dev_count(ctx->tncscan.as<int>() + G);

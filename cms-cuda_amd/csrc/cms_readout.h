@@ -43,8 +43,8 @@ __global__ void k_nearest32(const float4* Q32, const double4* Q64, const int* qg
                             const double4* R64, const int* rgrp, int xmask, const int* toff, int G, int nsplit,
                             int64_t pstride, const double4* frame, double* out_d2, int* out_idx,
                             unsigned long long* nrescored);
-__global__ void k_dot_grid_setup(int G, const int* agrp, const double4* axyz, double band, GridInfo* grid,
-                                 int* ncell);
+__global__ void k_dot_grid_setup(int G, const int* agrp, const int* bgrp, const double4* bxyz, double band,
+                                 GridInfo* grid, int* ncell);
 __global__ void k_dot_cell_count(Count na, const double4* axyz, const int* agrp, int G, const GridInfo* grid,
                                  int* acell, int* ccount);
 __global__ void k_dot_cell_scatter(Count na, const int* acell, const int* cstart, int* cursor,

@@ -606,14 +606,21 @@ namespace cms {
 
 // Per group: bounding box of its accessible dots, cell edge (>= band, grown
 // until the group has at most 2 n + 64 cells) and dims.
+// Trim grid of each group: cells of edge >= band over the box of the
+// group's buried dots widened by band + 1e-6 A (at most 2 n + 64 cells for
+// n accessible dots, the edge grows otherwise).  An accessible dot outside
+// that box is farther than the band from every buried dot along one axis,
+// so only the dots inside are binned.  A group without buried dots gets an
+// empty grid (x0 = +inf: no dot is inside).
 __global__ __launch_bounds__(256) void k_dot_grid_setup(int G, const int* __restrict__ agrp,
-                                                        const double4* __restrict__ axyz, double band,
+                                                        const int* __restrict__ bgrp,
+                                                        const double4* __restrict__ bxyz, double band,
                                                         GridInfo* __restrict__ grid, int* __restrict__ ncell) {
   __shared__ double red[6][256];
   for (int g = blockIdx.x; g < G; g += gridDim.x) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int i = agrp[g] + threadIdx.x; i < agrp[g + 1]; i += blockDim.x) {
-      const double4 a = axyz[i];
+    for (int i = bgrp[g] + threadIdx.x; i < bgrp[g + 1]; i += blockDim.x) {
+      const double4 a = bxyz[i];
       mn[0] = fmin(mn[0], a.x); mn[1] = fmin(mn[1], a.y); mn[2] = fmin(mn[2], a.z);
       mx[0] = fmax(mx[0], a.x); mx[1] = fmax(mx[1], a.y); mx[2] = fmax(mx[2], a.z);
     }
@@ -628,9 +635,11 @@ __global__ __launch_bounds__(256) void k_dot_grid_setup(int G, const int* __rest
     }
     if (threadIdx.x == 0) {
       const int n = agrp[g + 1] - agrp[g];
-      double lo[3] = {red[0][0], red[1][0], red[2][0]};
-      double hi[3] = {red[3][0], red[4][0], red[5][0]};
-      if (n == 0 || !(hi[0] >= lo[0]) || !(hi[1] >= lo[1]) || !(hi[2] >= lo[2])) {
+      const bool none = bgrp[g + 1] <= bgrp[g];
+      const double wide = band + 1e-6;
+      double lo[3] = {red[0][0] - wide, red[1][0] - wide, red[2][0] - wide};
+      double hi[3] = {red[3][0] + wide, red[4][0] + wide, red[5][0] + wide};
+      if (none || !(hi[0] >= lo[0]) || !(hi[1] >= lo[1]) || !(hi[2] >= lo[2])) {
         lo[0] = lo[1] = lo[2] = 0.0;
         hi[0] = hi[1] = hi[2] = 0.0;
       }
@@ -649,7 +658,7 @@ __global__ __launch_bounds__(256) void k_dot_grid_setup(int G, const int* __rest
         h *= 1.25;
       }
       GridInfo gi;
-      gi.x0 = lo[0]; gi.y0 = lo[1]; gi.z0 = lo[2]; gi.h = h;
+      gi.x0 = none ? INFINITY : lo[0]; gi.y0 = lo[1]; gi.z0 = lo[2]; gi.h = h;
       gi.nx = d[0]; gi.ny = d[1]; gi.nz = d[2];
       gi.cell0 = 0;
       gi.rmax = 0.0;
@@ -682,22 +691,33 @@ __device__ __forceinline__ void dot_cell(const GridInfo& gi, double x, double y,
   *iy = min(max((int)floor((y - gi.y0) / gi.h), 0), gi.ny - 1);
   *iz = min(max((int)floor((z - gi.z0) / gi.h), 0), gi.nz - 1);
 }
+// The cell of a point inside the grid's box (false outside it, or for an
+// empty grid).
+__device__ __forceinline__ bool dot_cell_in(const GridInfo& gi, double x, double y, double z, int* ix, int* iy,
+                                            int* iz) {
+  const double fx = floor((x - gi.x0) / gi.h), fy = floor((y - gi.y0) / gi.h), fz = floor((z - gi.z0) / gi.h);
+  if (!(fx >= 0.0 && fx < (double)gi.nx && fy >= 0.0 && fy < (double)gi.ny && fz >= 0.0 && fz < (double)gi.nz))
+    return false;
+  *ix = (int)fx;
+  *iy = (int)fy;
+  *iz = (int)fz;
+  return true;
+}
 
 __global__ void k_dot_cell_count(Count na, const double4* __restrict__ axyz, const int* __restrict__ agrp, int G,
                                  const GridInfo* __restrict__ grid, int* __restrict__ acell,
                                  int* __restrict__ ccount) {
   const int n = na.get();
   CMS_WAVE_LOOP(i, n) {
-    int cell = 0;
+    int cell = -1;
     if (i < n) {
       const GridInfo gi = grid[group_of_item_wave(agrp, G, (int)i, (int)(i - lane()))];
       const double4 a = axyz[i];
       int ix, iy, iz;
-      dot_cell(gi, a.x, a.y, a.z, &ix, &iy, &iz);
-      cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+      if (dot_cell_in(gi, a.x, a.y, a.z, &ix, &iy, &iz)) cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
       acell[i] = cell;
     }
-    wave_counter_inc(ccount, cell, i < n);
+    wave_counter_inc(ccount, max(cell, 0), cell >= 0);
   }
 }
 
@@ -706,9 +726,9 @@ __global__ void k_dot_cell_scatter(Count na, const int* __restrict__ acell, cons
                                    double4* __restrict__ sorted) {
   const int n = na.get();
   CMS_WAVE_LOOP(i, n) {
-    const int cell = i < n ? acell[i] : 0;
-    const int slot = wave_counter_add(cursor, cell, i < n);
-    if (i < n) sorted[cstart[cell] + slot] = axyz[i];
+    const int cell = i < n ? acell[i] : -1;
+    const int slot = wave_counter_add(cursor, max(cell, 0), cell >= 0);
+    if (cell >= 0) sorted[cstart[cell] + slot] = axyz[i];
   }
 }
 
