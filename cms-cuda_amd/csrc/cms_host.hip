@@ -536,7 +536,7 @@ static int size_probes(cms_ctx* ctx, int64_t c) {
 
 // dots and every per-dot list of the readouts
 static int size_dots(cms_ctx* ctx, int64_t c) {
-  DBuf* d4[] = {&ctx->dpos, &ctx->dnml, &ctx->bxyz, &ctx->axyz, &ctx->txyz, &ctx->tnml};
+  DBuf* d4[] = {&ctx->dpos, &ctx->dnml, &ctx->bxyz, &ctx->txyz, &ctx->tnml};
   for (DBuf* b : d4) CK(b->ensure(sizeof(double4) * (c + 1)));
   DBuf* i4[] = {&ctx->ddesc, &ctx->ddesc2};
   for (DBuf* b : i4) CK(b->ensure(sizeof(int4) * (c + 1)));
@@ -954,7 +954,7 @@ static int prepare_lists(cms_ctx* ctx) {
   if (estD > 0)
     k_dot_split<<<grid_for(estD, 256, 4096), 256, 0, st>>>(cD, ctx->dbur.as<int8_t>(), ctx->bscan.as<int>(),
                                                             ctx->dpos.as<double4>(), ctx->blist.as<int>(),
-                                                            ctx->bxyz.as<double4>(), ctx->axyz.as<double4>());
+                                                            ctx->bxyz.as<double4>());
   CKL();
   ctx->lists_ready = true;
   return CMS_OK;
@@ -1125,16 +1125,14 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   int* part = ctx->part.as<int>();
   int* meta = ctx->meta.as<int>();
   const Count cB = dev_count(meta + M_B), cT = dev_count(meta + M_T);
-  const int64_t capD = ctx->cap[M_D], estB = ctx->est[M_B], estA = ctx->est[M_A], estT = ctx->est[M_T];
+  const int64_t capD = ctx->cap[M_D], estB = ctx->est[M_B], estT = ctx->est[M_T];
   CK(ctx->tgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->side.ensure(sizeof(double) * 8 * G));
   // trim: buried dots of g against a grid of the accessible dots of g
   CK(ctx->tgrid.ensure(sizeof(GridInfo) * (G + 1)));
   CK(ctx->tncell.ensure(sizeof(int) * (G + 1)));
   CK(ctx->tncscan.ensure(sizeof(int) * (G + 1)));
-  const Count cA = dev_count(meta + M_A);
   const int64_t cells_cap = 2 * capD + 64ll * G;
-  const int gA = grid_for(std::max<int64_t>(estA, 1), 256, 4096);
   const int gB = grid_for(std::max<int64_t>(estB, 1), 256, 4096);
   GridInfo* tg = ctx->tgrid.as<GridInfo>();
   k_dot_grid_setup<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->agrpd.as<int>(), ctx->bgrp.as<int>(),
@@ -1145,11 +1143,14 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   const Count cCells = dev_count(ctx->tncscan.as<int>() + G);
   fill_int(ctx->tcount.as<int>(), cCells, cells_cap, 0, st);
   fill_int(ctx->tcur.as<int>(), cCells, cells_cap, 0, st);
-  k_dot_cell_count<<<gA, 256, 0, st>>>(cA, ctx->axyz.as<double4>(), ctx->agrpd.as<int>(), G, tg,
-                                       ctx->tacell.as<int>(), ctx->tcount.as<int>());
+  // the accessible dots read in place from the dot arrays
+  const Count cD = dev_count(meta + M_D);
+  const int gD = grid_for(std::max<int64_t>(ctx->est[M_D], 1), 256, 4096);
+  k_dot_cell_count<<<gD, 256, 0, st>>>(cD, ctx->dpos.as<double4>(), ctx->dbur.as<int8_t>(), ctx->dgrp.as<int>(), G,
+                                       tg, ctx->tacell.as<int>(), ctx->tcount.as<int>());
   scan_excl(ctx->tcount.as<int>(), ctx->tstart.as<int>(), cCells, cells_cap, part, st);
-  k_dot_cell_scatter<<<gA, 256, 0, st>>>(cA, ctx->tacell.as<int>(), ctx->tstart.as<int>(), ctx->tcur.as<int>(),
-                                         ctx->axyz.as<double4>(), ctx->tsorted.as<double4>());
+  k_dot_cell_scatter<<<gD, 256, 0, st>>>(cD, ctx->tacell.as<int>(), ctx->tstart.as<int>(), ctx->tcur.as<int>(),
+                                         ctx->dpos.as<double4>(), ctx->tsorted.as<double4>());
   {
     // lanes per buried dot: enough threads to fill the GPU for small batches
     const int64_t nb = std::max<int64_t>(estB, 1);

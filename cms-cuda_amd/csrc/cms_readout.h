@@ -9,7 +9,7 @@ namespace cms {
 
 __global__ void k_dot_flag(Count n, const int8_t* bur, int want, int* flag);
 __global__ void k_dot_split(Count cnt, const int8_t* dbur, const int* bscan, const double4* dpos, int* blist,
-                            double4* bxyz, double4* axyz);
+                            double4* bxyz);
 __global__ void k_gather_compl(int G, const int* idx, const int* scan, int* out);
 __global__ void k_tile_counts(int G, const int* qgrp, int tq, int* cnt);
 template <bool IDX>
@@ -45,10 +45,10 @@ __global__ void k_nearest32(const float4* Q32, const double4* Q64, const int* qg
                             unsigned long long* nrescored);
 __global__ void k_dot_grid_setup(int G, const int* agrp, const int* bgrp, const double4* bxyz, double band,
                                  GridInfo* grid, int* ncell);
-__global__ void k_dot_cell_count(Count na, const double4* axyz, const int* agrp, int G, const GridInfo* grid,
-                                 int* acell, int* ccount);
-__global__ void k_dot_cell_scatter(Count na, const int* acell, const int* cstart, int* cursor,
-                                   const double4* axyz, double4* sorted);
+__global__ void k_dot_cell_count(Count nd, const double4* dpos, const int8_t* dbur, const int* dgrp, int G,
+                                 const GridInfo* grid, int* acell, int* ccount);
+__global__ void k_dot_cell_scatter(Count nd, const int* acell, const int* cstart, int* cursor,
+                                   const double4* dpos, double4* sorted);
 template <int L>
 __global__ void k_trim_grid(Count nb, const double4* bxyz, const int* bgrp, int G, const GridInfo* grid,
                             const int* cstart, const double4* sorted, double band2, int* flag);

@@ -14,20 +14,16 @@ __global__ void k_dot_flag(Count cnt, const int8_t* __restrict__ bur, int want, 
     flag[i] = (bur[i] != 0) == (want != 0) ? 1 : 0;
 }
 
-// Buried and accessible dot lists in one pass: the buried ones at the
-// exclusive scan of the burial flags, the accessible ones at i - that scan.
+// Buried dot list in one pass: index and position at the exclusive scan
+// of the burial flags (the trim grid reads the accessible dots in place).
 __global__ void k_dot_split(Count cnt, const int8_t* __restrict__ dbur, const int* __restrict__ bscan,
-                            const double4* __restrict__ dpos, int* __restrict__ blist, double4* __restrict__ bxyz,
-                            double4* __restrict__ axyz) {
+                            const double4* __restrict__ dpos, int* __restrict__ blist, double4* __restrict__ bxyz) {
   const int n = cnt.get();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int b = bscan[i];
-    const double4 p = dpos[i];
     if (dbur[i]) {
+      const int b = bscan[i];
       blist[b] = i;
-      bxyz[b] = p;
-    } else {
-      axyz[i - b] = p;
+      bxyz[b] = dpos[i];
     }
   }
 }
@@ -704,31 +700,35 @@ __device__ __forceinline__ bool dot_cell_in(const GridInfo& gi, double x, double
   return true;
 }
 
-__global__ void k_dot_cell_count(Count na, const double4* __restrict__ axyz, const int* __restrict__ agrp, int G,
-                                 const GridInfo* __restrict__ grid, int* __restrict__ acell,
-                                 int* __restrict__ ccount) {
-  const int n = na.get();
+// Cells of the accessible dots (all dots, the buried ones skipped; dgrp:
+// the dots' group offsets) inside their group's trim grid, -1 otherwise.
+__global__ void k_dot_cell_count(Count nd, const double4* __restrict__ dpos, const int8_t* __restrict__ dbur,
+                                 const int* __restrict__ dgrp, int G, const GridInfo* __restrict__ grid,
+                                 int* __restrict__ acell, int* __restrict__ ccount) {
+  const int n = nd.get();
   CMS_WAVE_LOOP(i, n) {
     int cell = -1;
     if (i < n) {
-      const GridInfo gi = grid[group_of_item_wave(agrp, G, (int)i, (int)(i - lane()))];
-      const double4 a = axyz[i];
-      int ix, iy, iz;
-      if (dot_cell_in(gi, a.x, a.y, a.z, &ix, &iy, &iz)) cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+      const GridInfo gi = grid[group_of_item_wave(dgrp, G, (int)i, (int)(i - lane()))];
+      if (!dbur[i]) {
+        const double4 a = dpos[i];
+        int ix, iy, iz;
+        if (dot_cell_in(gi, a.x, a.y, a.z, &ix, &iy, &iz)) cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+      }
       acell[i] = cell;
     }
     wave_counter_inc(ccount, max(cell, 0), cell >= 0);
   }
 }
 
-__global__ void k_dot_cell_scatter(Count na, const int* __restrict__ acell, const int* __restrict__ cstart,
-                                   int* __restrict__ cursor, const double4* __restrict__ axyz,
+__global__ void k_dot_cell_scatter(Count nd, const int* __restrict__ acell, const int* __restrict__ cstart,
+                                   int* __restrict__ cursor, const double4* __restrict__ dpos,
                                    double4* __restrict__ sorted) {
-  const int n = na.get();
+  const int n = nd.get();
   CMS_WAVE_LOOP(i, n) {
     const int cell = i < n ? acell[i] : -1;
     const int slot = wave_counter_add(cursor, max(cell, 0), cell >= 0);
-    if (cell >= 0) sorted[cstart[cell] + slot] = axyz[i];
+    if (cell >= 0) sorted[cstart[cell] + slot] = dpos[i];
   }
 }
 
