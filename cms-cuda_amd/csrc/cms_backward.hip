@@ -196,7 +196,7 @@ template <int N> struct Seed<Dual<N>> {
   }
 };
 
-// Torus geometry of pair (i, j), as k_pairs computes it.
+// Torus geometry of pair (i, j), as pair_torus computes it.
 template <class T>
 __device__ __forceinline__ void pair_geom(const V3T<T>& xi, const V3T<T>& xj, double ri, double rj, double rp,
                                           V3T<T>* u, V3T<T>* t, T* rij, bool* between) {

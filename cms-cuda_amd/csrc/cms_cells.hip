@@ -623,10 +623,41 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
       brow[rank] = jp;
     }
     for (int o = 32; o > 0; o >>= 1) nbmax = max(nbmax, __shfl_xor(nbmax, o, WAVE));
+    // the row's forward pairs that matter (what the pair pass numbers,
+    // _core.py:1351-1442): valid torus and relevant; an atom whose only
+    // neighbour forms a valid forward pair takes the single-neighbour
+    // shortcut (both atoms accessible, no pairs)
+    int npair = 0;
+    bool shortcut = false;
+    for (int p0 = 0; p0 < cnt; p0 += WAVE) {
+      const int p = p0 + lane();
+      bool ok = false, rel = false;
+      int jp = -1;
+      if (p < cnt) {
+        jp = (int)__double_as_longlong(ce[p].y);
+        if (jp > g) {
+          V3 u, t;
+          double rij;
+          bool between;
+          const double4 aj = b.atom[jp];
+          ok = pair_torus(xi, ai.w, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between);
+          rel = ok && pair_relevant(w.atten[g], w.atten[jp], nbmax, jp);
+        }
+      }
+      if (cnt == 1 && wave_any(ok)) {
+        shortcut = true;
+        if (ok) {
+          w.access[g] = 1;
+          w.access[jp] = 1;
+        }
+      }
+      npair += __popcll(ballot(rel));
+    }
     if (lane() == 0) {
       w.nn[r] = cnt;
       w.nb[r] = cntb;
       w.nbmax[r] = nbmax;
+      w.pcnt[r] = shortcut ? 0 : npair;
     }
     __syncthreads();
   }

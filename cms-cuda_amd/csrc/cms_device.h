@@ -104,6 +104,37 @@ __device__ __forceinline__ double wave_min_double(double v) {
   return v;
 }
 
+// ------------------------------------------------------------ pairs
+// Torus of the pair (i, j): axis u, centre t, radius rij, whether the centre
+// lies between the atoms; false when the probe cannot touch both spheres.
+__device__ __forceinline__ bool pair_torus(V3 xi, double ri, V3 xj, double rj, double rp, V3* u, V3* t,
+                                           double* rij, bool* between) {
+  const double eri = ri + rp, erj = rj + rp;
+  const V3 diff = xj - xi;
+  const double dsq = dot_alt(diff, diff);
+  const double dij = sqrt(dsq);
+  const double asymm = (eri * eri - erj * erj) / dij;
+  *between = fabs(asymm) < dij;
+  *u = diff / dij;
+  *t = (xi + xj) * 0.5 + *u * (asymm * 0.5);
+  const double far_sq = (eri + erj) * (eri + erj) - dsq;
+  const double cont_sq = dsq - (ri - rj) * (ri - rj);
+  const bool ok = (far_sq > 0.0) && (cont_sq > 0.0);
+  *rij = ok ? (0.5 * sqrt(far_sq)) * sqrt(cont_sq) / dij : 0.0;
+  return ok;
+}
+
+// A pair matters only for the toroid queue (a non-blocker end) or for probes
+// (some atom of a triple i, j, k > j is a non-blocker); the toroid queue takes
+// pairs with a non-blocker first atom, or a non-blocker second atom when the
+// probe has a radius.
+__device__ __forceinline__ bool pair_relevant(int8_t atti, int8_t attj, int nbmax, int j) {
+  return atti > ATTEN_BLOCKER || attj > ATTEN_BLOCKER || nbmax > j;
+}
+__device__ __forceinline__ bool pair_toroid(int8_t atti, int8_t attj, double rp) {
+  return (atti > ATTEN_BLOCKER) || ((attj > ATTEN_BLOCKER) && (rp > 0.0));
+}
+
 // ------------------------------------------------------- arc subdivision
 // Number of points a_i = delta*(i+1/2) <= angle with i < MAX_SUBDIV, where
 // delta = 1/(sqrt(density)*rad).  NaN angle or infinite delta give 0.

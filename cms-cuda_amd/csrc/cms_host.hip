@@ -761,7 +761,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
                                                                                 ctx->cstart.as<int>(),
                                                                                 ctx->catom.as<int>(),
                                                                                 ctx->cxyz.as<double4>()));
-    TIMED("pairs_count", k_pairs<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+    // (the neighbour pass counted each row's pairs)
     CKL();
   }
   scan_excl(ctx->pcnt.as<int>(), ctx->pscan.as<int>(), cR, ctx->cap[M_R], part, st);
@@ -783,7 +783,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   // large batches count probes and torus points per neighbour row, and that
   // pass writes the pair records itself
   const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
-  if (est[M_R] > 0 && !by_row) TIMED("pairs_emit", k_pairs<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_R] > 0 && !by_row) TIMED("pairs_emit", k_pairs<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
   if (est[M_P] > 0) {
     if (by_row) {
       fill_int(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0, st);

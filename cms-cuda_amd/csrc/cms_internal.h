@@ -154,7 +154,7 @@ struct Work {
   int64_t dot_cap;
 };
 
-template <bool EMIT> __global__ void k_pairs(BatchView, Settings, Work);
+__global__ void k_pairs(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
 __global__ void k_probe_rows(BatchView, Settings, Work);
 __global__ void k_probe_place(BatchView, Work);

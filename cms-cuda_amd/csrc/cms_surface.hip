@@ -24,44 +24,14 @@ __global__ void k_compact_index(int n, const int* __restrict__ flag, const int* 
 }
 
 // ------------------------------------------------------------ pairs
-// _core.py:1351-1442.  Forward pairs (j > i) of atoms with neighbours, torus
-// centre/radius; the single-neighbour shortcut marks access and emits none.
-
-// Torus of the pair (i, j): axis u, centre t, radius rij, whether the centre
-// lies between the atoms; false when the probe cannot touch both spheres.
-__device__ __forceinline__ bool pair_torus(V3 xi, double ri, V3 xj, double rj, double rp, V3* u, V3* t,
-                                           double* rij, bool* between) {
-  const double eri = ri + rp, erj = rj + rp;
-  const V3 diff = xj - xi;
-  const double dsq = dot_alt(diff, diff);
-  const double dij = sqrt(dsq);
-  const double asymm = (eri * eri - erj * erj) / dij;
-  *between = fabs(asymm) < dij;
-  *u = diff / dij;
-  *t = (xi + xj) * 0.5 + *u * (asymm * 0.5);
-  const double far_sq = (eri + erj) * (eri + erj) - dsq;
-  const double cont_sq = dsq - (ri - rj) * (ri - rj);
-  const bool ok = (far_sq > 0.0) && (cont_sq > 0.0);
-  *rij = ok ? (0.5 * sqrt(far_sq)) * sqrt(cont_sq) / dij : 0.0;
-  return ok;
-}
-
-// A pair matters only for the toroid queue (a non-blocker end) or for probes
-// (some atom of a triple i, j, k > j is a non-blocker); the toroid queue takes
-// pairs with a non-blocker first atom, or a non-blocker second atom when the
-// probe has a radius.
-__device__ __forceinline__ bool pair_relevant(int8_t atti, int8_t attj, int nbmax, int j) {
-  return atti > ATTEN_BLOCKER || attj > ATTEN_BLOCKER || nbmax > j;
-}
-__device__ __forceinline__ bool pair_toroid(int8_t atti, int8_t attj, double rp) {
-  return (atti > ATTEN_BLOCKER) || ((attj > ATTEN_BLOCKER) && (rp > 0.0));
-}
-
-template <bool EMIT>
+// _core.py:1351-1442.  Pair records of small batches: the forward pairs
+// (j > i) of each row that the neighbour pass counted (valid torus,
+// relevant), at the scanned offsets, with torus centre/radius and flags.
+// Rows that take the single-neighbour shortcut have a count of 0 and are
+// not visited.
 __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
   const int nR = *w.nR;
-  // the emit pass visits only rows whose count pass found pairs
-  auto live = [&](int64_t q) { return !EMIT || w.pcnt[q] != 0; };
+  auto live = [&](int64_t q) { return w.pcnt[q] != 0; };
   CMS_SPARSE_LOOP(r, nR, live) {
     const int g = w.ratom[r];
     const int nn = w.nn[r];
@@ -70,12 +40,11 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
     const double ri = ai.w;
     const int* row = w.nbr + (int64_t)r * KCAP;
     const int nbmax = w.nbmax[r];
-    int base = EMIT ? w.pscan[r] : 0;
+    const int base = w.pscan[r];
     int total = 0;
-    bool shortcut = false;
     for (int s0 = 0; s0 < nn; s0 += WAVE) {
       const int sl = s0 + lane();
-      bool ok = false, relevant = false;
+      bool relevant = false;
       int j = -1;
       V3 u{0, 0, 0}, t{0, 0, 0};
       double rij = 0.0;
@@ -84,23 +53,12 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
         j = row[sl];
         if (j > g) {
           const double4 aj = b.atom[j];
-          ok = pair_torus(xi, ri, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between);
-          relevant = ok && pair_relevant(w.atten[g], w.atten[j], nbmax, j);
-        }
-      }
-      if (nn <= 1) {
-        // nn == 1 here: shortcut when slot 0 is a valid forward pair
-        shortcut = wave_any(ok);
-        if (shortcut) {
-          if (!EMIT && sl == 0) {
-            w.access[g] = 1;
-            w.access[j] = 1;
-          }
-          break;
+          relevant = pair_torus(xi, ri, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between) &&
+                     pair_relevant(w.atten[g], w.atten[j], nbmax, j);
         }
       }
       const uint64_t mk = ballot(relevant);
-      if (EMIT && relevant && base + total + lanes_below(mk) < w.pair_cap) {
+      if (relevant && base + total + lanes_below(mk) < w.pair_cap) {
         const int o = base + total + lanes_below(mk);
         w.pi[o] = g;
         w.pj[o] = j;
@@ -111,7 +69,6 @@ __global__ __launch_bounds__(64) void k_pairs(BatchView b, Settings s, Work w) {
       }
       total += __popcll(mk);
     }
-    if (!EMIT && lane() == 0) w.pcnt[r] = shortcut ? 0 : total;
   }
 }
 
@@ -1628,8 +1585,6 @@ __global__ __launch_bounds__(64) void k_concave(BatchView b, Settings s, Work w)
 }
 
 // explicit instantiations used by the host pipeline
-template __global__ void k_pairs<false>(BatchView, Settings, Work);
-template __global__ void k_pairs<true>(BatchView, Settings, Work);
 template __global__ void k_probes<false>(BatchView, Settings, Work);
 template __global__ void k_probes<true>(BatchView, Settings, Work);
 template __global__ void k_convex<false>(BatchView, Settings, Work);
