@@ -460,9 +460,15 @@ __global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     double mn = INFINITY;
     if (!all_atoms) {
-      // the other molecule's atoms of the 27 cells, by (x, y) column
+      // the other molecule's atoms of the 27 cells, by (x, y) column, the
+      // atom's own column first.  Only sqrt(min d^2) < sep is used, so the
+      // sweep stops once a partial minimum lies below sep^2 (1 - 1e-12):
+      // its square root, and so that of the full minimum, rounds below sep
+      // (the margin is 1e4 times the roundings of sep^2 and of the root)
+      const double stop = (s.sep * s.sep) * (1.0 - 1e-12);
       const int z0 = max(iz - 1, 0), z1 = min(iz + 1, gi.nz - 1);
-      for (int col = 0; col < 9; ++col) {
+      for (int k = 0; k < 9 && !(mn < stop); ++k) {
+        const int col = k == 0 ? 4 : (k <= 4 ? k - 1 : k);
         const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
         if (cx < 0 || cx >= gi.nx || cy < 0 || cy >= gi.ny) continue;
         const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, 1 - m);
