@@ -1,19 +1,26 @@
 // FP32 nearest-dot search with FP64 rescoring.
 //
 // Coordinates are recentred per complex (centre c = middle of the complex's
-// atom bounding box) and rounded to float.  With R bounding |x - c| per
-// component for every dot of the complex and u = 2^-24, the float squared
-// distance of a pair at true distance d satisfies
-//     |d2f - d^2| <= delta(d) = 4 sqrt(3) R u d + 7 u d^2 + 12 R^2 u^2
-// (recentring and rounding of each coordinate: |e| <= R u; rounding of the
-// difference: |dx| u; three roundings in (dx^2 + dy^2) + dz^2: 3u d^2).
+// atom bounding box) and rounded to float; R bounds |x - c| per component
+// for every dot of the complex and u = 2^-24.  Each float point carries
+// its squared norm rounded to float in .w, and a (query q, reference r)
+// pair is filtered on
+//     e = |r|^2 - 2 r.q       (three fused multiply-adds)
+// which differs from d^2 - |q|^2 (d = distance of the FP64 points, |q|^2 a
+// per-query constant) by at most 64 u R^2:
+//   - float rounding of the coordinates moves |r - q| by at most 2 sqrt(3) u R,
+//     and d <= 2 sqrt(3) R, so |(r - q)_float|^2 is within 24 u R^2 (+ O(u^2)) of d^2;
+//   - rounding |r|^2 to float costs 3 u R^2, the three fused steps, whose
+//     partial sums stay below 5, 7 and 9 R^2, cost 21 u R^2.
 // While streaming over the references, every reference with
-//     d2f <= m + 2 * 1.5 * delta(sqrt(m) + 0.05)
-// (m = running float minimum) is rescored in FP64 exactly as the FP64 path
+//     e <= m + 1.5 * 2 * 64 u R^2
+// (m = running minimum of e) is rescored in FP64 exactly as the FP64 path
 // computes it, and the (d^2, index) minimum is kept among rescored ones.  A
-// reference left out has d^2 > (true minimum), so the FP64 argmin -- and
-// every reference tied with it -- is always rescored: distances and partner
-// indices are identical to the FP64 search.
+// reference left out has d^2 above that of the one that set m, so the FP64
+// argmin -- and every reference tied with it -- is always rescored:
+// distances and partner indices are identical to the FP64 search.  (The 1.5
+// covers the float rounding of the threshold, at most 9 u R^2, and the
+// FP64 rounding of d^2.)
 #include "cms_device.h"
 #include "cms_internal.h"
 #include "cms_readout.h"
@@ -31,7 +38,7 @@ __global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q
                                                     int64_t pstride, const double4* __restrict__ frame,
                                                     double* __restrict__ out_d2, int* __restrict__ out_idx,
                                                     unsigned long long* __restrict__ nrescored) {
-  __shared__ float rx[NT32], ry[NT32], rz[NT32];
+  __shared__ float4 rs[NT32];
   const int64_t nwork = (int64_t)toff[G] * nsplit;
   unsigned long long resc = 0;
   for (int64_t wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
@@ -50,14 +57,12 @@ __global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q
     const int r0 = rg0 + (int)(((int64_t)rn * sp) / nsplit);
     const int r1 = rg0 + (int)(((int64_t)rn * (sp + 1)) / nsplit);
     const float Rb = (float)frame[g >> 1].w;
-    const float cA = 1.5f * 4.0f * 1.7320508f * Rb * U32;
-    const float cB = 1.5f * 7.0f * U32;
-    const float cC = 1.5f * 12.0f * Rb * Rb * U32 * U32;
-    float qx = 0.f, qy = 0.f, qz = 0.f;
+    const float band = (1.5f * 2.0f * 64.0f * U32) * (Rb * Rb);
+    float qx = 0.f, qy = 0.f, qz = 0.f;   // -2 q
     double qx64 = 0.0, qy64 = 0.0, qz64 = 0.0;
     if (live) {
       const float4 a = Q32[q];
-      qx = a.x; qy = a.y; qz = a.z;
+      qx = -2.0f * a.x; qy = -2.0f * a.y; qz = -2.0f * a.z;
       const double4 b = Q64[q];
       qx64 = b.x; qy64 = b.y; qz64 = b.z;
     }
@@ -67,28 +72,41 @@ __global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q
     for (int rb = r0; rb < r1; rb += NT32) {
       __syncthreads();
       const int rr = rb + threadIdx.x;
-      if (rr < r1) {
-        const float4 a = R32[rr];
-        rx[threadIdx.x] = a.x; ry[threadIdx.x] = a.y; rz[threadIdx.x] = a.z;
-      }
+      if (rr < r1) rs[threadIdx.x] = R32[rr];
       __syncthreads();
       const int n = min(NT32, r1 - rb);
       if (live) {
-        for (int t = 0; t < n; ++t) {
-          const float dx = rx[t] - qx, dy = ry[t] - qy, dz = rz[t] - qz;
-          const float d2f = (dx * dx + dy * dy) + dz * dz;
-          if (d2f <= thr) {
+        auto take = [&](float e, int t) {
+          if (e <= thr) {
             ++resc;
-            if (d2f < m) {
-              m = d2f;
-              const float de = sqrtf(m) + 0.05f;
-              thr = m + 2.0f * ((cA * de + cB * de * de) + cC);
+            if (e < m) {
+              m = e;
+              thr = m + band;
             }
             const double4 a = R64[rb + t];
             const double ex = a.x - qx64, ey = a.y - qy64, ez = a.z - qz64;
             const double d2 = (ex * ex + ey * ey) + ez * ez;
             if (d2 < best || (d2 == best && rb - rg0 + t < bi)) { best = d2; bi = rb - rg0 + t; }
           }
+        };
+        int t = 0;
+        // four references per test: the threshold only falls, so a group
+        // whose smallest e is above it holds no candidate
+        for (; t + 4 <= n; t += 4) {
+          float e[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const float4 r = rs[t + k];
+            e[k] = fmaf(r.x, qx, fmaf(r.y, qy, fmaf(r.z, qz, r.w)));
+          }
+          if (fminf(fminf(e[0], e[1]), fminf(e[2], e[3])) <= thr) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) take(e[k], t + k);
+          }
+        }
+        for (; t < n; ++t) {
+          const float4 r = rs[t];
+          take(fmaf(r.x, qx, fmaf(r.y, qy, fmaf(r.z, qz, r.w))), t);
         }
       }
     }
@@ -102,9 +120,10 @@ __global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q
   if (nrescored && resc) atomicAdd(nrescored, resc);
 }
 
-// float copy of a double4 point list, recentred on its complex's frame;
-// group of item i found from grp (G+1 starts), complex = group / 2 (or 0
-// when percomplex == 0).
+// float copy of a double4 point list, recentred on its complex's frame,
+// with the float point's squared norm (rounded once) in .w; group of item i
+// found from grp (G+1 starts), complex = group / 2 (or 0 when
+// percomplex == 0).
 __global__ void k_to_float(Count cnt, const double4* __restrict__ in, const int* __restrict__ grp, int G,
                            int percomplex, const double4* __restrict__ frame, float4* __restrict__ out) {
   const int n = cnt.get();
@@ -120,7 +139,10 @@ __global__ void k_to_float(Count cnt, const double4* __restrict__ in, const int*
     }
     const double4 f = frame[c];
     const double4 a = in[i];
-    out[i] = make_float4((float)(a.x - f.x), (float)(a.y - f.y), (float)(a.z - f.z), 0.f);
+    const float x = (float)(a.x - f.x), y = (float)(a.y - f.y), z = (float)(a.z - f.z);
+    // float products are exact in double
+    const double n2 = ((double)x * x + (double)y * y) + (double)z * z;
+    out[i] = make_float4(x, y, z, (float)n2);
   }
 }
 

@@ -290,6 +290,44 @@ def test_fp32_search_matches_fp64(cms_cuda):
     ctx.close()
 
 
+def test_fp32_search_equidistant_shell(cms_cuda):
+    """References on a sphere of radius 55 A around the queries, radii
+    spread by at most 1e-5 A: the float filter cannot order them (its error
+    at this frame size is ~1e-2 A^2), so only a rescoring band as wide as the
+    filter's error bound finds the FP64 nearest reference and its
+    first-index ties."""
+    import ctypes
+    import torch
+    from cms_cuda import _lib
+    say("fp32 vs fp64 nearest, equidistant shell")
+    rng = np.random.default_rng(11)
+    c0 = np.array([12.5, -7.25, 3.0])
+    u = rng.normal(size=(3000, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    r = c0 + (55.0 + rng.uniform(-1e-5, 1e-5, size=(3000, 1))) * u
+    r[2500:2600] = r[1000:1100]                   # exact ties
+    q = c0 + rng.normal(scale=1e-7, size=(64, 3))
+    q[:8] = c0
+    ctx = _lib.Context(0, torch.cuda.current_stream().cuda_stream)
+    dq = torch.from_numpy(q).cuda()
+    dr = torch.from_numpy(r).cuda()
+    res = {}
+    for prec in (_lib.PREC_FP64, _lib.PREC_FP32_RESCORE):
+        d2 = torch.empty(len(q), dtype=torch.float64, device="cuda")
+        idx = torch.empty(len(q), dtype=torch.int32, device="cuda")
+        ctx("cms_nearest", prec, ctypes.c_void_p(dq.data_ptr()), len(q), ctypes.c_void_p(dr.data_ptr()), len(r),
+            ctypes.c_void_p(d2.data_ptr()), ctypes.c_void_p(idx.data_ptr()))
+        res[prec] = (d2.cpu().numpy(), idx.cpu().numpy())
+    # FP64 partners against numpy's squared distances (x^2 + y^2 first)
+    e = r[None, :, :] - q[:, None, :]
+    dn = (e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2]
+    assert np.array_equal(res[0][0], dn.min(axis=1))
+    assert np.array_equal(res[0][1], dn.argmin(axis=1))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    ctx.close()
+
+
 def test_fp32_mode_batch_identical(cms_cuda, golden_complex):
     """Batched CMS / SC in FP32-search mode equal the FP64 mode bit for bit."""
     from cms_cuda import synth
