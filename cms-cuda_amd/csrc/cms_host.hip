@@ -997,10 +997,11 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
   if (precision == CMS_PREC_FP32_RESCORE) {
     unsigned long long* nr = ctx->nresc.as<unsigned long long>();
     if (ido)
-      TIMED("nearest32", k_nearest32<true><<<grid, 256, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+      // 128 threads per block, two queries per thread (256-query tiles)
+      TIMED("nearest32", k_nearest32<true><<<grid, 128, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
                                                                  frame, d2o, ido, nr));
     else
-      TIMED("nearest32", k_nearest32<false><<<grid, 256, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+      TIMED("nearest32", k_nearest32<false><<<grid, 128, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
                                                                   frame, d2o, nullptr, nr));
   } else {
     if (ido)

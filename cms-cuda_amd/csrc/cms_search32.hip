@@ -27,17 +27,27 @@
 
 namespace cms {
 
-constexpr int NT32 = 256;
+constexpr int NT32 = 256;          // queries per tile, references staged per round
+constexpr int TPB32 = NT32 / 2;    // threads per block: two queries per thread
 constexpr float U32 = 5.9604645e-08f;  // 2^-24
 
+// One query's running state: -2 q (float), q (FP64), the float minimum of
+// e, the rescoring threshold, and the (d^2, index) minimum among rescored
+// references.
+struct Near32 {
+  float qx, qy, qz, m, thr;
+  double x, y, z, best;
+  int bi;
+};
+
 template <bool IDX>
-__global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q32, const double4* __restrict__ Q64,
-                                                    const int* __restrict__ qgrp, const float4* __restrict__ R32,
-                                                    const double4* __restrict__ R64, const int* __restrict__ rgrp,
-                                                    int xmask, const int* __restrict__ toff, int G, int nsplit,
-                                                    int64_t pstride, const double4* __restrict__ frame,
-                                                    double* __restrict__ out_d2, int* __restrict__ out_idx,
-                                                    unsigned long long* __restrict__ nrescored) {
+__global__ __launch_bounds__(TPB32) void k_nearest32(const float4* __restrict__ Q32, const double4* __restrict__ Q64,
+                                                     const int* __restrict__ qgrp, const float4* __restrict__ R32,
+                                                     const double4* __restrict__ R64, const int* __restrict__ rgrp,
+                                                     int xmask, const int* __restrict__ toff, int G, int nsplit,
+                                                     int64_t pstride, const double4* __restrict__ frame,
+                                                     double* __restrict__ out_d2, int* __restrict__ out_idx,
+                                                     unsigned long long* __restrict__ nrescored) {
   __shared__ float4 rs[NT32];
   const int64_t nwork = (int64_t)toff[G] * nsplit;
   unsigned long long resc = 0;
@@ -50,70 +60,91 @@ __global__ __launch_bounds__(NT32) void k_nearest32(const float4* __restrict__ Q
       if (toff[mid] <= tile) lo = mid; else hi = mid;
     }
     const int g = lo;
-    const int q = qgrp[g] + (tile - toff[g]) * NT32 + threadIdx.x;
-    const bool live = q < qgrp[g + 1];
+    const int q0 = qgrp[g] + (tile - toff[g]) * NT32 + threadIdx.x;
+    const int qe = qgrp[g + 1];
     const int h = g ^ xmask;
     const int rg0 = rgrp[h], rn = rgrp[h + 1] - rg0;
     const int r0 = rg0 + (int)(((int64_t)rn * sp) / nsplit);
     const int r1 = rg0 + (int)(((int64_t)rn * (sp + 1)) / nsplit);
     const float Rb = (float)frame[g >> 1].w;
     const float band = (1.5f * 2.0f * 64.0f * U32) * (Rb * Rb);
-    float qx = 0.f, qy = 0.f, qz = 0.f;   // -2 q
-    double qx64 = 0.0, qy64 = 0.0, qz64 = 0.0;
-    if (live) {
-      const float4 a = Q32[q];
-      qx = -2.0f * a.x; qy = -2.0f * a.y; qz = -2.0f * a.z;
-      const double4 b = Q64[q];
-      qx64 = b.x; qy64 = b.y; qz64 = b.z;
+    // queries q0 and q0 + TPB32; a query past the group takes nothing
+    // (threshold -inf)
+    Near32 st[2];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int q = q0 + v * TPB32;
+      Near32& z = st[v];
+      z.qx = z.qy = z.qz = 0.f;
+      z.x = z.y = z.z = 0.0;
+      z.m = INFINITY;
+      z.thr = q < qe ? INFINITY : -INFINITY;
+      z.best = INFINITY;
+      z.bi = -1;
+      if (q < qe) {
+        const float4 a = Q32[q];
+        z.qx = -2.0f * a.x; z.qy = -2.0f * a.y; z.qz = -2.0f * a.z;
+        const double4 b = Q64[q];
+        z.x = b.x; z.y = b.y; z.z = b.z;
+      }
     }
-    float m = INFINITY, thr = INFINITY;
-    double best = INFINITY;
-    int bi = -1;
+    const bool live = q0 < qe;
     for (int rb = r0; rb < r1; rb += NT32) {
       __syncthreads();
-      const int rr = rb + threadIdx.x;
-      if (rr < r1) rs[threadIdx.x] = R32[rr];
+      for (int k = threadIdx.x; k < NT32; k += TPB32)
+        if (rb + k < r1) rs[k] = R32[rb + k];
       __syncthreads();
       const int n = min(NT32, r1 - rb);
       if (live) {
-        auto take = [&](float e, int t) {
-          if (e <= thr) {
+        auto take = [&](Near32& z, float e, int t) {
+          if (e <= z.thr) {
             ++resc;
-            if (e < m) {
-              m = e;
-              thr = m + band;
+            if (e < z.m) {
+              z.m = e;
+              z.thr = z.m + band;
             }
             const double4 a = R64[rb + t];
-            const double ex = a.x - qx64, ey = a.y - qy64, ez = a.z - qz64;
+            const double ex = a.x - z.x, ey = a.y - z.y, ez = a.z - z.z;
             const double d2 = (ex * ex + ey * ey) + ez * ez;
-            if (d2 < best || (d2 == best && rb - rg0 + t < bi)) { best = d2; bi = rb - rg0 + t; }
+            if (d2 < z.best || (d2 == z.best && rb - rg0 + t < z.bi)) { z.best = d2; z.bi = rb - rg0 + t; }
           }
         };
         int t = 0;
-        // four references per test: the threshold only falls, so a group
-        // whose smallest e is above it holds no candidate
+        // four references per test: a threshold only falls, so a group
+        // whose smallest e is above it holds no candidate for that query
         for (; t + 4 <= n; t += 4) {
-          float e[4];
+          float e0[4], e1[4];
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
             const float4 r = rs[t + k];
-            e[k] = fmaf(r.x, qx, fmaf(r.y, qy, fmaf(r.z, qz, r.w)));
+            e0[k] = fmaf(r.x, st[0].qx, fmaf(r.y, st[0].qy, fmaf(r.z, st[0].qz, r.w)));
+            e1[k] = fmaf(r.x, st[1].qx, fmaf(r.y, st[1].qy, fmaf(r.z, st[1].qz, r.w)));
           }
-          if (fminf(fminf(e[0], e[1]), fminf(e[2], e[3])) <= thr) {
+          const bool h0 = fminf(fminf(e0[0], e0[1]), fminf(e0[2], e0[3])) <= st[0].thr;
+          const bool h1 = fminf(fminf(e1[0], e1[1]), fminf(e1[2], e1[3])) <= st[1].thr;
+          if (h0 || h1) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) take(e[k], t + k);
+            for (int k = 0; k < 4; ++k) {
+              take(st[0], e0[k], t + k);
+              take(st[1], e1[k], t + k);
+            }
           }
         }
         for (; t < n; ++t) {
           const float4 r = rs[t];
-          take(fmaf(r.x, qx, fmaf(r.y, qy, fmaf(r.z, qz, r.w))), t);
+          take(st[0], fmaf(r.x, st[0].qx, fmaf(r.y, st[0].qy, fmaf(r.z, st[0].qz, r.w))), t);
+          take(st[1], fmaf(r.x, st[1].qx, fmaf(r.y, st[1].qy, fmaf(r.z, st[1].qz, r.w))), t);
         }
       }
     }
-    if (live) {
-      const int64_t o = (int64_t)sp * pstride + q;
-      out_d2[o] = best;
-      if (IDX) out_idx[o] = bi;
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int q = q0 + v * TPB32;
+      if (q < qe) {
+        const int64_t o = (int64_t)sp * pstride + q;
+        out_d2[o] = st[v].best;
+        if (IDX) out_idx[o] = st[v].bi;
+      }
     }
     __syncthreads();
   }
