@@ -34,6 +34,19 @@ __device__ __forceinline__ uint32_t cells_within(const GridInfo& gi, double x, d
   return mask;
 }
 
+// The same test for the one cell k of the block (one cell per lane of a
+// wavefront, the mask then taken by ballot).
+__device__ __forceinline__ bool cell_within(const GridInfo& gi, double x, double y, double z, int ix, int iy, int iz,
+                                            double lim, int k) {
+  const double fx = x - (gi.x0 + ix * gi.h), fy = y - (gi.y0 + iy * gi.h), fz = z - (gi.z0 + iz * gi.h);
+  const int kx = k / 9, ky = (k / 3) % 3, kz = k % 3;
+  const double a = kx == 0 ? fmax(fx, 0.0) : (kx == 1 ? 0.0 : fmax(gi.h - fx, 0.0));
+  const double b = ky == 0 ? fmax(fy, 0.0) : (ky == 1 ? 0.0 : fmax(gi.h - fy, 0.0));
+  const double c = kz == 0 ? fmax(fz, 0.0) : (kz == 1 ? 0.0 : fmax(gi.h - fz, 0.0));
+  const double l2 = (lim + 1e-6) * (lim + 1e-6);
+  return (a * a + b * b) + c * c <= l2;
+}
+
 __global__ void k_grid_setup(int B, const int* aoff, const double4* atom, Settings s, GridInfo* grid,
                              double4* frame, int* ncell);
 __global__ void k_grid_offsets(int B, const int* cscan, GridInfo* grid);

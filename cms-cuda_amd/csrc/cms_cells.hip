@@ -557,11 +557,16 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     int cnt = 0, cntb = 0, nbmax = -1;
+    // both rows take atoms within the bridge distance r_i + r_j + 2 rp, at
+    // most r_i + r_max + 2 rp: only the cells whose box comes that close
+    // can hold them (one cell tested per lane)
+    const uint32_t near = (uint32_t)ballot(
+        lane() < 27 && cell_within(gi, ai.x, ai.y, ai.z, ix, iy, iz, (ai.w + gi.rmax) + 2.0 * s.rp, lane()));
     // two sweeps: the atom's own molecule (neighbour row), then the other
     // molecule (buried row); the cell slot gives the molecule
     for (int sweep = 0; sweep < 2; ++sweep) {
       const bool own = sweep == 0;
-      cells_around(gi, ix, iy, iz, cstart, own ? m : 1 - m, 0x7ffffffu, cbeg, cpre);
+      cells_around(gi, ix, iy, iz, cstart, own ? m : 1 - m, near, cbeg, cpre);
       const int T = cpre[27];
       for (int i0 = 0; i0 < T; i0 += WAVE) {
         const int idx = i0 + lane();
