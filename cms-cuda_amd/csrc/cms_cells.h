@@ -34,6 +34,26 @@ __device__ __forceinline__ uint32_t cells_within(const GridInfo& gi, double x, d
   return mask;
 }
 
+// The same test column by column: the run of cells z0 .. z1 (grid-clipped)
+// of column (kx, ky) (0..2 each) of the block whose boxes come within lim
+// (l2 = (lim + 1e-6)^2) of the point at offsets (fx, fy, fz) from the low
+// faces of its cell (ix, iy, iz), with the arithmetic of cells_within, so
+// the same cells; false when none does or the column is outside the grid.
+// The column's middle cell is its nearest, so the run holds it.
+__device__ __forceinline__ bool column_within(const GridInfo& gi, double fx, double fy, double fz, int ix, int iy,
+                                              int iz, int kx, int ky, double l2, int* z0, int* z1) {
+  const int cx = ix + kx - 1, cy = iy + ky - 1;
+  if (cx < 0 || cx >= gi.nx || cy < 0 || cy >= gi.ny) return false;
+  const double a = kx == 0 ? fmax(fx, 0.0) : (kx == 1 ? 0.0 : fmax(gi.h - fx, 0.0));
+  const double b = ky == 0 ? fmax(fy, 0.0) : (ky == 1 ? 0.0 : fmax(gi.h - fy, 0.0));
+  const double gxy = a * a + b * b;
+  if (!(gxy + 0.0 * 0.0 <= l2)) return false;
+  const double cl = fmax(fz, 0.0), ch = fmax(gi.h - fz, 0.0);
+  *z0 = max(iz - ((gxy + cl * cl <= l2) ? 1 : 0), 0);
+  *z1 = min(iz + ((gxy + ch * ch <= l2) ? 1 : 0), gi.nz - 1);
+  return true;
+}
+
 // The same test for the one cell k of the block (one cell per lane of a
 // wavefront, the mask then taken by ballot).
 __device__ __forceinline__ bool cell_within(const GridInfo& gi, double x, double y, double z, int ix, int iy, int iz,

@@ -510,15 +510,14 @@ __global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s,
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     // the bridge distance is at most r_i + r_max + 2 rp; the cells within
-    // it form one z-run per (x, y) column (the column's middle cell is the
-    // nearest)
-    const uint32_t within = cells_within(gi, ai.x, ai.y, ai.z, ix, iy, iz, (ai.w + gi.rmax) + 2.0 * s.rp);
+    // it form one z-run per (x, y) column
+    const double lim = (ai.w + gi.rmax) + 2.0 * s.rp;
+    const double l2 = (lim + 1e-6) * (lim + 1e-6);
+    const double fx = ai.x - (gi.x0 + ix * gi.h), fy = ai.y - (gi.y0 + iy * gi.h), fz = ai.z - (gi.z0 + iz * gi.h);
     for (int col = 0; col < 9; ++col) {
-      const uint32_t zb = (within >> (3 * col)) & 7u;
+      int z0, z1;
+      if (!column_within(gi, fx, fy, fz, ix, iy, iz, col / 3, col % 3, l2, &z0, &z1)) continue;
       const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
-      if (!zb || cx < 0 || cx >= gi.nx || cy < 0 || cy >= gi.ny) continue;
-      const int z0 = max(iz + __builtin_ctz(zb) - 1, 0), z1 = min(iz + 31 - __builtin_clz(zb) - 1, gi.nz - 1);
-      if (z0 > z1) continue;
       const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, m);
 #pragma unroll 4
       for (int t = 0; t < cr.total; ++t) {
