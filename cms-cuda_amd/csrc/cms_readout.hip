@@ -760,38 +760,34 @@ __global__ __launch_bounds__(256) void k_trim_grid(Count nb, const double4* __re
     int ix, iy, iz;
     dot_cell(gi, q.x, q.y, q.z, &ix, &iy, &iz);
     bool hit = false;
-    // only cells whose box comes within the band of the dot
-    const uint32_t within = cells_within(gi, q.x, q.y, q.z, ix, iy, iz, sqrt(band2));
-    for (int dx = -1; dx <= 1 && !hit; ++dx) {
-      const int cx = ix + dx;
-      if (cx < 0 || cx >= gi.nx) continue;
-      for (int dy = -1; dy <= 1 && !hit; ++dy) {
-        const int cy = iy + dy;
-        if (cy < 0 || cy >= gi.ny) continue;
-        // the cells along z are contiguous in the cell array
-        const uint32_t col = (within >> (9 * (dx + 1) + 3 * (dy + 1))) & 7u;
-        if (!col) continue;
-        const int zlo = max(iz - ((col & 1u) ? 1 : 0), 0), zhi = min(iz + ((col & 4u) ? 1 : 0), gi.nz - 1);
-        const int c0 = gi.cell0 + (cx * gi.ny + cy) * gi.nz;
-        const int lo = cstart[c0 + zlo], hi = cstart[c0 + zhi + 1];
-        if constexpr (L == 1) {
-          for (int u = lo; u < hi; ++u) {
+    // only cells whose box comes within the band of the dot, one z-run per
+    // (x, y) column
+    const double band = sqrt(band2);
+    const double l2 = (band + 1e-6) * (band + 1e-6);
+    const double fx = q.x - (gi.x0 + ix * gi.h), fy = q.y - (gi.y0 + iy * gi.h), fz = q.z - (gi.z0 + iz * gi.h);
+    for (int col = 0; col < 9 && !hit; ++col) {
+      int zlo, zhi;
+      if (!column_within(gi, fx, fy, fz, ix, iy, iz, col / 3, col % 3, l2, &zlo, &zhi)) continue;
+      const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
+      const int c0 = gi.cell0 + (cx * gi.ny + cy) * gi.nz;
+      const int lo = cstart[c0 + zlo], hi = cstart[c0 + zhi + 1];
+      if constexpr (L == 1) {
+        for (int u = lo; u < hi; ++u) {
+          const double4 a = sorted[u];
+          const double ddx = q.x - a.x, ddy = q.y - a.y, ddz = q.z - a.z;
+          if ((ddx * ddx + ddy * ddy) + ddz * ddz <= band2) { hit = true; break; }
+        }
+      } else {
+        // uniform trip count over the L lanes of the dot
+        for (int u0 = lo; u0 < hi; u0 += L) {
+          const int u = u0 + sub;
+          bool h = false;
+          if (u < hi) {
             const double4 a = sorted[u];
             const double ddx = q.x - a.x, ddy = q.y - a.y, ddz = q.z - a.z;
-            if ((ddx * ddx + ddy * ddy) + ddz * ddz <= band2) { hit = true; break; }
+            h = (ddx * ddx + ddy * ddy) + ddz * ddz <= band2;
           }
-        } else {
-          // uniform trip count over the L lanes of the dot
-          for (int u0 = lo; u0 < hi; u0 += L) {
-            const int u = u0 + sub;
-            bool h = false;
-            if (u < hi) {
-              const double4 a = sorted[u];
-              const double ddx = q.x - a.x, ddy = q.y - a.y, ddz = q.z - a.z;
-              h = (ddx * ddx + ddy * ddy) + ddz * ddz <= band2;
-            }
-            if (ballot(h) & submask) { hit = true; break; }
-          }
+          if (ballot(h) & submask) { hit = true; break; }
         }
       }
     }
