@@ -236,3 +236,21 @@ def test_group_totals_numpy_order(golden_complex):
         sizes += [nb, nt]
         off += nb + nt
     assert min(sizes) < 128 and max(sizes) > 4000
+
+
+def test_large_batch_kernels_match_small(golden_complex):
+    """Batches of at least 200,000 atoms take the per-complex cell-list
+    kernels and the thread-per-atom attention and region sweeps (column
+    runs, early stop of the attention sweep); smaller batches take the
+    wavefront-per-atom forms.  The same complexes as one large batch and in
+    chunks of 20: identical results."""
+    cs = _batch(golden_complex, [(1800 + 40 * k, 200 + 53 * (k % 9)) for k in range(80)], 1200)
+    n = sum(len(c[1]) + len(c[3]) for c in cs)
+    assert n >= 200000
+    say("large batch (%d complexes, %d atoms) vs chunks" % (len(cs), n))
+    big = _reference(cs)
+    parts = [_reference(cs[k:k + 20]) for k in range(0, len(cs), 20)]
+    assert max(sum(len(c[1]) + len(c[3]) for c in cs[k:k + 20]) for k in range(0, len(cs), 20)) < 200000
+    merged = {key: np.concatenate([p[key] for p in parts]) for key in ("target", "binder", "per_atom", "sc")}
+    assert _same(big, merged)
+    assert (big["target"] > 0).sum() > 60
