@@ -987,10 +987,15 @@ __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s,
 // collision with every neighbour but the nearest one.
 template <bool EMIT>
 __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) {
+  // LDS is what limits the waves per CU here (at most 13,312 B for twelve
+  // 64-lane workgroups, the count pass's VGPR limit): the point spacings
+  // exist in the emit pass only, and the emit pass's list of kept points
+  // shares the neighbours' storage (listed and placed before they load)
   __shared__ double4 lx[KCAP];
-  __shared__ uint16_t klist[CMASK_WORDS * WAVE];   // emit: kept points under the count pass's mask
+  uint16_t* const klist = reinterpret_cast<uint16_t*>(lx);   // emit: kept points under the count pass's mask
+  static_assert(CMASK_WORDS * WAVE * sizeof(uint16_t) <= KCAP * sizeof(double4), "kept-point list fits");
   __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
-  __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
+  __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[EMIT ? MAX_SUBDIV : 1];
   __shared__ int lat_start[MAX_SUBDIV + 1];
   const int nR = *w.nR;
   // count pass: accessible non-blocker atoms with neighbours (the counts of
@@ -1058,7 +1063,8 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
             ps = P > 0 ? (rad * TWO_PI) / (double)P : 0.0;
           }
           lat_cx[t] = cen.x; lat_cy[t] = cen.y; lat_cz[t] = cen.z;
-          lat_rad[t] = rad; lat_delta[t] = cdelta; lat_ps[t] = ps;
+          lat_rad[t] = rad; lat_delta[t] = cdelta;
+          if (EMIT) lat_ps[t] = ps;
         }
         const int inc = wave_incl_scan(P);
         if (t < L) lat_start[t] = run + inc - P;
@@ -1073,12 +1079,6 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
         obase = w.dbase[grp * 4 + 0] + w.cscan[r];
       }
       const double scale = eri / ri;
-      // the neighbours, for the collision tests the emit pass does not take
-      // from the count pass
-      if (!EMIT || C > CMASK_WORDS * WAVE) {
-        for (int sl = lane(); sl < nn; sl += WAVE) lx[sl] = b.atom[row[sl]];
-        __syncthreads();
-      }
       // the emit pass takes the collision outcome of the first
       // CMASK_WORDS * 64 points from the count pass: their kept points are
       // listed (in point order) and placed one per lane; the points past
@@ -1112,6 +1112,14 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
         }
         total = nk;
         q_from = cm;
+      }
+      // the neighbours, for the collision tests the emit pass does not take
+      // from the count pass (in the emit pass over the kept-point list,
+      // which every lane has finished reading)
+      if (!EMIT || C > CMASK_WORDS * WAVE) {
+        if (EMIT) __syncthreads();
+        for (int sl = lane(); sl < nn; sl += WAVE) lx[sl] = b.atom[row[sl]];
+        __syncthreads();
       }
       for (int q0 = q_from; q0 < C; q0 += WAVE) {
         const int q = q0 + lane();
