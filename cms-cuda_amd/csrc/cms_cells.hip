@@ -438,8 +438,10 @@ __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, co
 
 // Thread-per-atom variants for large batches: consecutive threads take
 // consecutive cell-sorted atoms, so a wavefront reads a handful of cells
-// that stay in L1/L2 instead of every lane's own neighbourhood.
-__global__ __launch_bounds__(256) void k_attention_cells_t(BatchView b, Settings s, int all_atoms, int full,
+// that stay in L1/L2 instead of every lane's own neighbourhood.  The
+// attention sweep waits on its loads: it is held to 64 VGPRs for eight
+// wavefronts per SIMD.
+__global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Settings s, int all_atoms, int full,
                                                            const GridInfo* __restrict__ grid,
                                                            const int* __restrict__ cstart,
                                                            const int* __restrict__ catom,
