@@ -1422,10 +1422,11 @@ __global__ __launch_bounds__(64) void k_toroid_emit(BatchView b, Settings s, Wor
 // rejection, and the closest of the three atoms as owner.
 constexpr int CC_BLOCK = 512;   // concave candidate points per prefilter block
 
-// Held to 168 VGPRs for three wavefronts per SIMD: the few spilled values
-// cost less than the lost occupancy.
+// The count pass is held to 168 VGPRs for three wavefronts per SIMD: the
+// few spilled values cost less than the lost occupancy (not so in the emit
+// pass).
 template <bool EMIT>
-__global__ __launch_bounds__(64, 3) void k_concave(BatchView b, Settings s, Work w) {
+__global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Settings s, Work w) {
   __shared__ uint16_t clist[CC_BLOCK];
   __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
   __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[MAX_SUBDIV];
