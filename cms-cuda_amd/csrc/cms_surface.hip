@@ -986,15 +986,18 @@ __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s,
 // the pole to the torus-derived south point, circles at each latitude,
 // collision with every neighbour but the nearest one.
 template <bool EMIT>
-__global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) {
-  // LDS is what limits the waves per CU here (at most 13,312 B for twelve
-  // 64-lane workgroups, the count pass's VGPR limit): the point spacings
+__global__ __launch_bounds__(64, EMIT ? 3 : 4) void k_convex(BatchView b, Settings s, Work w) {
+  // Occupancy: the count pass is held to 128 VGPRs (four wavefronts per
+  // SIMD) and its LDS to 11,264 B (fourteen 64-lane workgroups per CU), the
+  // emit pass (three per SIMD) to 13,312 B (twelve): the point spacings
   // exist in the emit pass only, and the emit pass's list of kept points
   // shares the neighbours' storage (listed and placed before they load)
   __shared__ double4 lx[KCAP];
   uint16_t* const klist = reinterpret_cast<uint16_t*>(lx);   // emit: kept points under the count pass's mask
   static_assert(CMASK_WORDS * WAVE * sizeof(uint16_t) <= KCAP * sizeof(double4), "kept-point list fits");
-  __shared__ double lat_cx[MAX_SUBDIV], lat_cy[MAX_SUBDIV], lat_cz[MAX_SUBDIV];
+  // latitude circle t: centre xi + north * lat_dt[t] (recomputed where used,
+  // the same operations, so the same bits)
+  __shared__ double lat_dt[MAX_SUBDIV];
   __shared__ double lat_rad[MAX_SUBDIV], lat_delta[MAX_SUBDIV], lat_ps[EMIT ? MAX_SUBDIV : 1];
   __shared__ int lat_start[MAX_SUBDIV + 1];
   const int nR = *w.nR;
@@ -1053,7 +1056,6 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
         if (t < L) {
           const V3 lat = subdiv_point(v3(0.0, 0.0, 0.0), north, y, ri, delta, t);
           const double dt = dot_seq(lat, north);
-          const V3 cen = xi + north * dt;
           const double rsq = ri * ri - dt * dt;
           double rad = 0.0, cdelta = 0.0, ps = 0.0;
           if (rsq > 0.0) {
@@ -1062,7 +1064,7 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
             P = subdiv_count(cdelta, TWO_PI);
             ps = P > 0 ? (rad * TWO_PI) / (double)P : 0.0;
           }
-          lat_cx[t] = cen.x; lat_cy[t] = cen.y; lat_cz[t] = cen.z;
+          lat_dt[t] = dt;
           lat_rad[t] = rad; lat_delta[t] = cdelta;
           if (EMIT) lat_ps[t] = ps;
         }
@@ -1103,7 +1105,7 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
               if (lat_start[mid] <= q) lo = mid; else hi = mid;
             }
             const int t = lo;
-            const V3 pt = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], lat_delta[t],
+            const V3 pt = subdiv_point(xi + north * lat_dt[t], cx, cy, lat_rad[t], lat_delta[t],
                                        q - lat_start[t]);
             const V3 pc = xi + (pt - xi) * scale;
             store_dot(b, s, w, obase + kq, pt, pc, lat_ps[t] * cs, g, r, 1, make_int4(DK_CONVEX, g, row[0], -1),
@@ -1134,7 +1136,7 @@ __global__ __launch_bounds__(64) void k_convex(BatchView b, Settings s, Work w) 
           }
           t = lo;
           const int kk = q - lat_start[t];
-          pt = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t], lat_delta[t], kk);
+          pt = subdiv_point(xi + north * lat_dt[t], cx, cy, lat_rad[t], lat_delta[t], kk);
           pc = xi + (pt - xi) * scale;
           {
             keep = true;
