@@ -497,40 +497,60 @@ __global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Setti
   }
 }
 
-__global__ __launch_bounds__(256) void k_region_cells_t(BatchView b, Settings s, const GridInfo* __restrict__ grid,
-                                                        const int* __restrict__ cstart,
-                                                        const int* __restrict__ catom,
-                                                        const double4* __restrict__ cxyz,
-                                                        const int8_t* __restrict__ atten, int* __restrict__ need) {
-  for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
-    const int g = catom[sidx];
-    if (atten[g] <= ATTEN_BLOCKER) continue;
-    const int m = b.amol[g];
-    const GridInfo gi = grid[b.acplx[g]];
-    const double4 ai = cxyz[sidx];
-    const V3 xi = ld3(ai);
-    int ix, iy, iz;
-    cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-    // the bridge distance is at most r_i + r_max + 2 rp; the cells within
-    // it form one z-run per (x, y) column
-    const double lim = (ai.w + gi.rmax) + 2.0 * s.rp;
-    const double l2 = (lim + 1e-6) * (lim + 1e-6);
-    const double fx = ai.x - (gi.x0 + ix * gi.h), fy = ai.y - (gi.y0 + iy * gi.h), fz = ai.z - (gi.z0 + iz * gi.h);
-    for (int col = 0; col < 9; ++col) {
-      int z0, z1;
-      if (!column_within(gi, fx, fy, fz, ix, iy, iz, col / 3, col % 3, l2, &z0, &z1)) continue;
-      const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
-      const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, m);
-#pragma unroll 4
-      for (int t = 0; t < cr.total; ++t) {
-        const int slot = col_index(cr, t);
-        const int j = catom[slot];
-        const double4 aj = cxyz[slot];
-        const V3 d = xi - ld3(aj);
-        const double br = (ai.w + aj.w) + 2.0 * s.rp;
-        if (dot_seq(d, d) < br * br) need[j] = 1;
+// Only non-blockers sweep: one wavefront takes a chunk of RG_CHUNK
+// consecutive cell slots, lists its non-blockers in LDS (still in cell
+// order, so neighbouring lanes share cells) and sweeps them 64 at a time,
+// instead of idling the lanes of the blockers.
+constexpr int RG_CHUNK = 256;
+
+__global__ __launch_bounds__(64) void k_region_cells_t(BatchView b, Settings s, const GridInfo* __restrict__ grid,
+                                                       const int* __restrict__ cstart,
+                                                       const int* __restrict__ catom,
+                                                       const double4* __restrict__ cxyz,
+                                                       const int8_t* __restrict__ atten, int* __restrict__ need) {
+  __shared__ int sel[RG_CHUNK];
+  for (int64_t base = (int64_t)blockIdx.x * RG_CHUNK; base < b.ntot; base += (int64_t)gridDim.x * RG_CHUNK) {
+    int nsel = 0;
+    for (int k = 0; k < RG_CHUNK; k += WAVE) {
+      const int64_t sx = base + k + lane();
+      const bool nb = sx < b.ntot && atten[catom[sx]] > ATTEN_BLOCKER;
+      const uint64_t mk = ballot(nb);
+      if (nb) sel[nsel + lanes_below(mk)] = (int)sx;
+      nsel += __popcll(mk);
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < nsel; k0 += WAVE) {
+      if (k0 + lane() >= nsel) continue;
+      const int sidx = sel[k0 + lane()];
+      const int g = catom[sidx];
+      const int m = b.amol[g];
+      const GridInfo gi = grid[b.acplx[g]];
+      const double4 ai = cxyz[sidx];
+      const V3 xi = ld3(ai);
+      int ix, iy, iz;
+      cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
+      // the bridge distance is at most r_i + r_max + 2 rp; the cells within
+      // it form one z-run per (x, y) column
+      const double lim = (ai.w + gi.rmax) + 2.0 * s.rp;
+      const double l2 = (lim + 1e-6) * (lim + 1e-6);
+      const double fx = ai.x - (gi.x0 + ix * gi.h), fy = ai.y - (gi.y0 + iy * gi.h), fz = ai.z - (gi.z0 + iz * gi.h);
+      for (int col = 0; col < 9; ++col) {
+        int z0, z1;
+        if (!column_within(gi, fx, fy, fz, ix, iy, iz, col / 3, col % 3, l2, &z0, &z1)) continue;
+        const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
+        const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, m);
+  #pragma unroll 4
+        for (int t = 0; t < cr.total; ++t) {
+          const int slot = col_index(cr, t);
+          const int j = catom[slot];
+          const double4 aj = cxyz[slot];
+          const V3 d = xi - ld3(aj);
+          const double br = (ai.w + aj.w) + 2.0 * s.rp;
+          if (dot_seq(d, d) < br * br) need[j] = 1;
+        }
       }
     }
+    __syncthreads();
   }
 }
 

@@ -727,7 +727,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
           b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
           ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
       if (!full)
-        TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
+        TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 64, 0, st>>>(
             b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
     } else {
       TIMED("attention", k_attention_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
