@@ -972,9 +972,11 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
   hipStream_t st = ctx->stream;
   CK(ctx->tcnt.ensure(sizeof(int) * (G + 1)));
   CK(ctx->toff.ensure(sizeof(int) * (G + 1)));
-  k_tile_counts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, qgrp, 256, ctx->tcnt.as<int>());
+  // queries per tile: 128 for the FP32 search (two per thread), 256 for FP64
+  const int tq = precision == CMS_PREC_FP32_RESCORE ? 128 : 256;
+  k_tile_counts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, qgrp, tq, ctx->tcnt.as<int>());
   scan_excl(ctx->tcnt.as<int>(), ctx->toff.as<int>(), G, ctx->part.as<int>(), st);
-  const int64_t tiles_est = nq_hint / 256 + G;
+  const int64_t tiles_est = nq_hint / tq + G;
   const int64_t target = 2048;
   int nsplit = 1;
   if (tiles_est < target) {
@@ -997,11 +999,11 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
   if (precision == CMS_PREC_FP32_RESCORE) {
     unsigned long long* nr = ctx->nresc.as<unsigned long long>();
     if (ido)
-      // 128 threads per block, two queries per thread (256-query tiles)
-      TIMED("nearest32", k_nearest32<true><<<grid, 128, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+      // 64 threads per block, two queries per thread (128-query tiles)
+      TIMED("nearest32", k_nearest32<true><<<grid, 64, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
                                                                  frame, d2o, ido, nr));
     else
-      TIMED("nearest32", k_nearest32<false><<<grid, 128, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+      TIMED("nearest32", k_nearest32<false><<<grid, 64, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
                                                                   frame, d2o, nullptr, nr));
   } else {
     if (ido)

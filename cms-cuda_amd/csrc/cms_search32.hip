@@ -27,7 +27,7 @@
 
 namespace cms {
 
-constexpr int NT32 = 256;          // queries per tile, references staged per round
+constexpr int NT32 = 128;          // queries per tile, references staged per round
 constexpr int TPB32 = NT32 / 2;    // threads per block: two queries per thread
 constexpr float U32 = 5.9604645e-08f;  // 2^-24
 
