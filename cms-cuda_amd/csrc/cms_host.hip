@@ -1156,11 +1156,11 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
                                          ctx->dpos.as<double4>(), ctx->tsorted.as<double4>());
   {
     // lanes per buried dot: enough threads to fill the GPU for small
-    // batches; eight read a dot's candidates together (measured on
-    // 2,112 complexes: 424 us against 451 us with one lane per dot and 596 us
-    // with sixteen)
+    // batches; four read a dot's candidates together (measured on
+    // 2,112 complexes: 374 us against 451, 406, 424 and 596 us with 1, 2, 8
+    // and 16 lanes per dot)
     const int64_t nb = std::max<int64_t>(estB, 1);
-    const int L = nb >= (1 << 23) ? 1 : nb >= (1 << 14) ? 8 : 32;
+    const int L = nb >= (1 << 23) ? 1 : nb >= (1 << 14) ? 4 : 32;
     const int gL = grid_for(nb * L, 256, 8192);
     const double band2 = ctx->s.band * ctx->s.band;
     const int* cs = ctx->tstart.as<int>();
@@ -1169,8 +1169,8 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
     if (L == 1)
       TIMED("trim", (k_trim_grid<1><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs, srt,
                                                         band2, fl)));
-    else if (L == 8)
-      TIMED("trim", (k_trim_grid<8><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs, srt,
+    else if (L == 4)
+      TIMED("trim", (k_trim_grid<4><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs, srt,
                                                         band2, fl)));
     else
       TIMED("trim", (k_trim_grid<32><<<gL, 256, 0, st>>>(cB, ctx->bxyz.as<double4>(), ctx->bgrp.as<int>(), G, tg, cs,

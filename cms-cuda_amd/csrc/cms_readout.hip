@@ -796,7 +796,7 @@ __global__ __launch_bounds__(256) void k_trim_grid(Count nb, const double4* __re
 }
 template __global__ void k_trim_grid<1>(Count, const double4*, const int*, int, const GridInfo*, const int*,
                                         const double4*, double, int*);
-template __global__ void k_trim_grid<8>(Count, const double4*, const int*, int, const GridInfo*, const int*,
+template __global__ void k_trim_grid<4>(Count, const double4*, const int*, int, const GridInfo*, const int*,
                                         const double4*, double, int*);
 template __global__ void k_trim_grid<32>(Count, const double4*, const int*, int, const GridInfo*, const int*,
                                          const double4*, double, int*);
