@@ -973,7 +973,7 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
   CK(ctx->tcnt.ensure(sizeof(int) * (G + 1)));
   CK(ctx->toff.ensure(sizeof(int) * (G + 1)));
   // queries per tile: 128 for the FP32 search (two per thread), 256 for FP64
-  const int tq = precision == CMS_PREC_FP32_RESCORE ? 128 : 256;
+  const int tq = precision == CMS_PREC_FP32_RESCORE ? NT32 : NT64;
   k_tile_counts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, qgrp, tq, ctx->tcnt.as<int>());
   scan_excl(ctx->tcnt.as<int>(), ctx->toff.as<int>(), G, ctx->part.as<int>(), st);
   const int64_t tiles_est = nq_hint / tq + G;
@@ -1000,16 +1000,16 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
     unsigned long long* nr = ctx->nresc.as<unsigned long long>();
     if (ido)
       // 64 threads per block, two queries per thread (128-query tiles)
-      TIMED("nearest32", k_nearest32<true><<<grid, 64, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+      TIMED("nearest32", k_nearest32<true><<<grid, TPB32, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
                                                                  frame, d2o, ido, nr));
     else
-      TIMED("nearest32", k_nearest32<false><<<grid, 64, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
+      TIMED("nearest32", k_nearest32<false><<<grid, TPB32, 0, st>>>(Q32, Q, qgrp, R32, R, rgrp, xmask, toff, G, nsplit, ps,
                                                                   frame, d2o, nullptr, nr));
   } else {
     if (ido)
-      TIMED("nearest", k_nearest64<true><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, ido));
+      TIMED("nearest", k_nearest64<true><<<grid, NT64, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, ido));
     else
-      TIMED("nearest", k_nearest64<false><<<grid, 256, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, nullptr));
+      TIMED("nearest", k_nearest64<false><<<grid, NT64, 0, st>>>(Q, qgrp, R, rgrp, xmask, toff, G, nsplit, ps, d2o, nullptr));
   }
   if (nsplit > 1)
     k_nearest_combine<<<grid_for(std::max<int64_t>(nq_hint, 1), 256, 4096), 256, 0, st>>>(nq, nsplit, ps, d2o, ido,
@@ -1578,6 +1578,10 @@ int cms_counts(cms_ctx* ctx, int64_t* out) {
   return CMS_OK;
 }
 
+}  // extern "C"
+
+namespace cms {
+
 __global__ void k_pack3(int n, const double* xyz, double4* out) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     out[i] = make_double4(xyz[3 * (int64_t)i], xyz[3 * (int64_t)i + 1], xyz[3 * (int64_t)i + 2], 0.0);
@@ -1607,6 +1611,10 @@ __global__ void k_points_frame(int nq, const double4* q, int nr, const double4* 
     frame[0] = make_double4(cx, cy, cz, half * 1.0000001 + 1e-6);
   }
 }
+
+}  // namespace cms
+
+extern "C" {
 
 int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r, int64_t nr,
                 double* out_d2, int32_t* out_idx) {

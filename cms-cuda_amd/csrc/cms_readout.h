@@ -7,6 +7,14 @@
 
 namespace cms {
 
+// FP32 search tiles: NT32 queries per tile (references staged NT32 per
+// round), TPB32 threads per block, two queries per thread.  The host sizes
+// tiles and launches from these; k_nearest32 assumes blockDim.x == TPB32.
+constexpr int NT32 = 128;
+constexpr int TPB32 = NT32 / 2;
+// FP64 search: queries per tile = threads per block
+constexpr int NT64 = 256;
+
 __global__ void k_dot_flag(Count n, const int8_t* bur, int want, int* flag);
 __global__ void k_dot_split(Count cnt, const int8_t* dbur, const int* bscan, const double4* dpos, int* blist,
                             double4* bxyz);

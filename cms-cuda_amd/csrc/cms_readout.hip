@@ -47,7 +47,7 @@ __global__ void k_tile_counts(int G, const int* __restrict__ qgrp, int tq, int* 
 // references, staged in LDS 256 at a time.  With nsplit > 1 every split
 // writes a partial (d^2, index) and k_nearest_combine takes the minimum in
 // split order, so the first index still wins ties.
-constexpr int NT = 256;
+constexpr int NT = NT64;
 template <bool IDX>
 __global__ __launch_bounds__(NT) void k_nearest64(const double4* __restrict__ Q, const int* __restrict__ qgrp,
                                                   const double4* __restrict__ R, const int* __restrict__ rgrp,

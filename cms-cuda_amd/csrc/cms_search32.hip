@@ -27,8 +27,6 @@
 
 namespace cms {
 
-constexpr int NT32 = 128;          // queries per tile, references staged per round
-constexpr int TPB32 = NT32 / 2;    // threads per block: two queries per thread
 constexpr float U32 = 5.9604645e-08f;  // 2^-24
 
 // One query's running state: -2 q (float), q (FP64), the float minimum of

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "cms_internal.h"
 
 namespace cms {
@@ -27,6 +29,8 @@ __global__ void k_pack_atoms(int ntot, const double* xyz, const double* rad, con
 __global__ void k_group_starts_from_atoms(Count n, const int* items, const int* acplx,
                                           const int8_t* amol, int G, int* start);
 __global__ void k_gather_scan(int G, const int* idx, int mul, const int* scan, int* out);
+__global__ void k_pack3(int n, const double* xyz, double4* out);
+__global__ void k_points_frame(int nq, const double4* q, int nr, const double4* r, double4* frame);
 __global__ void k_dot_bases(int G, const int* rgrp, const int* pgrp, const int* prgrp,
                             const int* cscan, const int* t1scan, const int* t2scan,
                             const int* kscan, int64_t* dbase, int* dgrp);

@@ -21,6 +21,16 @@ Fixtures (every array float64 unless noted):
   ref_maxcms.npz  maximum possible CMS of the known-answer binder alone
   ref_c1.npz      a 400 + 1,000-atom complex (gap 4 A): CMS both sides,
                   per-atom arrays, SC record, dot counts
+  ref_examples.npz the reference's five example complexes
+                  (example_*.pdb, selections as the reference's README
+                  runs them), inputs as cms_cuda.structure.load_atoms reads
+                  them: CMS of both sides with per-atom arrays, SC record,
+                  counts, every dot's owner / burial / type in order, dot
+                  checksums and 400 sampled dots per molecule; and the
+                  maximum possible CMS of the 4,167-atom target (chain A)
+                  alone with every dot's owner in order
+
+    python tests/golden/make_golden.py [--only examples]
 """
 
 import contextlib
@@ -92,6 +102,59 @@ def counts(calc):
     return np.array(out, np.int64)
 
 
+EXAMPLES = [
+    # file, binder selection, target selection
+    ("linearpeptide", dict(chains="B"), dict(chains="A")),
+    ("Nanobody", dict(chains="B"), dict(chains="A")),
+    ("binder", dict(chains="B"), dict(chains="A")),
+    ("scFv", dict(chains="B"), dict(chains="A")),
+    ("7jkv-smallmol", dict(chains="A", exclude_resnames={"V7G"}), dict(resnames={"V7G"})),
+]
+
+
+def example_inputs(name, bsel, tsel):
+    from cms_cuda.structure import load_atoms
+    path = os.path.join(HERE, "example_%s.pdb" % name)
+    bx, br, _ = load_atoms(path, **bsel)
+    tx, tr, _ = load_atoms(path, **tsel)
+    return bx, br, tx, tr
+
+
+def examples(core, info):
+    out = {}
+    rng = np.random.default_rng(9)
+    for name, bsel, tsel in EXAMPLES:
+        t0 = time.time()
+        key = name.replace("-", "_")
+        bx, br, tx, tr = example_inputs(name, bsel, tsel)
+        calc, r = run_complex(core, bx, br, tx, tr)
+        out.update({"%s_%s" % (key, k): v for k, v in dict(binder_xyz=bx, binder_radii=br, target_xyz=tx,
+                                                          target_radii=tr, counts=counts(calc), **r).items()})
+        for m in (0, 1):
+            d = dots_of(calc, m)
+            for k in ("atom", "buried", "type"):
+                out["%s_dots%d_%s" % (key, m, k)] = d[k]
+            idx = np.sort(rng.choice(len(d["area"]), size=min(400, len(d["area"])), replace=False))
+            out["%s_sample%d_idx" % (key, m)] = idx.astype(np.int64)
+            for k in ("coor", "nml", "area"):
+                out["%s_sample%d_%s" % (key, m, k)] = d[k][idx]
+            out["%s_sum%d" % (key, m)] = np.array([d["coor"].sum(), d["nml"].sum(), d["area"].sum()])
+        print("%s: %d + %d atoms, cms %.16g / %.16g, sc %s, %d + %d dots (%.1f s)"
+              % (name, len(br), len(tr), r["cms_target"], r["cms_binder"], r["sc"],
+                 len(calc.run.dots[0]), len(calc.run.dots[1]), time.time() - t0), flush=True)
+    # whole-molecule surface of the 4,167-atom target
+    t0 = time.time()
+    _, _, tx, tr = example_inputs("scFv", dict(chains="B"), dict(chains="A"))
+    mx, per, calc = core.calculate_maximum_possible_contact_ms(tx, tr)
+    d = dots_of(calc, 0)
+    out.update(maxcms_xyz=tx, maxcms_radii=tr, maxcms_total=np.float64(mx),
+               maxcms_per_atom=np.asarray(per, np.float64), maxcms_atom=d["atom"],
+               maxcms_sum=np.array([d["coor"].sum(), d["area"].sum()]),
+               maxcms_counts=np.array([len(d["area"])] + [int((d["type"] == t).sum()) for t in (1, 2, 3)]))
+    print("max-cms of the target: %.16g from %d dots (%.1f s)" % (mx, len(d["area"]), time.time() - t0))
+    np.savez_compressed(os.path.join(HERE, "ref_examples.npz"), **out, runtime=np.array(info))
+
+
 def main():
     from cms_cuda import synth
     from cms_cuda import get_radii_from_names
@@ -99,6 +162,10 @@ def main():
     core = load_reference()
     info = runtime_info()
     t0 = time.time()
+    if "--only" in sys.argv:
+        if sys.argv[sys.argv.index("--only") + 1] == "examples":
+            examples(core, info)
+        return
 
     # small synthetic complex, everything
     c = synth.make_complex(260, 80, 21, gap=0.5)

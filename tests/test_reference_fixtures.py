@@ -127,3 +127,56 @@ def test_c1_complex():
     assert np.array_equal(counts(o), f["counts"])
     check_scalars(f, *out)
     assert rel(side_stats(o), f["side"]) <= TOL
+
+
+EXAMPLES = [
+    ("linearpeptide", dict(chains="B"), dict(chains="A")),
+    ("Nanobody", dict(chains="B"), dict(chains="A")),
+    ("binder", dict(chains="B"), dict(chains="A")),
+    ("scFv", dict(chains="B"), dict(chains="A")),
+    ("7jkv-smallmol", dict(chains="A", exclude_resnames={"V7G"}), dict(resnames={"V7G"})),
+]
+
+
+@pytest.mark.parametrize("name,bsel,tsel", EXAMPLES)
+def test_example_inputs_from_structure_files(name, bsel, tsel):
+    """cms_cuda.structure.load_atoms reads the example files into exactly the
+    inputs the reference vectors were computed on (float32 coordinates, as
+    Bio.PDB stores them; radii from the table, none unmatched)."""
+    from cms_cuda.structure import load_atoms
+    z = load("ref_examples.npz")
+    key = name.replace("-", "_")
+    path = os.path.join(GOLD, "example_%s.pdb" % name)
+    bx, br, binfo = load_atoms(path, **bsel)
+    tx, tr, tinfo = load_atoms(path, **tsel)
+    assert np.array_equal(bx, z[key + "_binder_xyz"]) and np.array_equal(br, z[key + "_binder_radii"])
+    assert np.array_equal(tx, z[key + "_target_xyz"]) and np.array_equal(tr, z[key + "_target_radii"])
+    assert np.array_equal(bx, bx.astype(np.float32).astype(np.float64))
+    assert (br > 0).all() and (tr > 0).all()
+    assert len(binfo) == len(bx) and len(tinfo) == len(tx)
+
+
+@pytest.mark.slow
+def test_example_ligand_complex_oracle():
+    """The oracle on the example protein-ligand complex (2,329 + 41 atoms)
+    against the reference's vectors: every dot's owner, burial and type in
+    order, sampled dots, CMS of both sides with per-atom arrays, SC and the
+    per-side statistics."""
+    z = load("ref_examples.npz")
+    k = "7jkv_smallmol_"
+    o, cms_t, per_t, cms_b, per_b, sc = oracle_run(z[k + "binder_xyz"], z[k + "binder_radii"],
+                                                   z[k + "target_xyz"], z[k + "target_radii"])
+    assert np.array_equal(counts(o), z[k + "counts"])
+    for m in (0, 1):
+        d = o.dots[m]
+        assert np.array_equal(d.atom, z[k + "dots%d_atom" % m])
+        assert np.array_equal(d.buried, z[k + "dots%d_buried" % m])
+        assert np.array_equal(d.type, z[k + "dots%d_type" % m])
+        idx = z[k + "sample%d_idx" % m]
+        assert rel(d.coor[idx], z[k + "sample%d_coor" % m]) <= TOL
+        assert rel(d.nml[idx], z[k + "sample%d_nml" % m]) <= TOL
+        assert rel(d.area[idx], z[k + "sample%d_area" % m]) <= TOL
+    assert rel(cms_t, z[k + "cms_target"]) <= TOL and rel(per_t, z[k + "per_target"]) <= TOL
+    assert rel(cms_b, z[k + "cms_binder"]) <= TOL and rel(per_b, z[k + "per_binder"]) <= TOL
+    assert rel(sc, z[k + "sc"]) <= TOL
+    assert rel(side_stats(o), z[k + "side"]) <= TOL
