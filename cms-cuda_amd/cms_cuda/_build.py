@@ -14,7 +14,7 @@ CSRC = os.path.normpath(os.path.join(HERE, "..", "csrc"))
 INCLUDE = os.path.normpath(os.path.join(HERE, "..", "..", "include"))
 LIB_PATH = os.path.join(HERE, "libcms_hip.so")
 SOURCES = ["cms_cells.hip", "cms_surface.hip", "cms_readout.hip", "cms_util.hip", "cms_search32.hip",
-           "cms_backward.hip", "cms_host.hip"]
+           "cms_backward.hip", "cms_noalloc.hip", "cms_host.hip"]
 ARCH = os.environ.get("CMS_HIP_ARCH", "gfx950")
 
 COMMON = [

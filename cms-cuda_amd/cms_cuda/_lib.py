@@ -21,6 +21,8 @@ CMS_ERR_STATE = 5
 PREC_FP64 = 0
 PREC_FP32_RESCORE = 1
 
+NOALLOC_MAX_POINTS = 131072
+
 NCOUNTS = 15
 SC_NRES = 24
 
@@ -59,6 +61,7 @@ _PROTOS = {
     "cms_probe_count": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_i64)]),
     "cms_copy_probes": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp]),
     "cms_nearest": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "cms_nearest_noalloc": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),
     "cms_profile": (_c.c_int, [_vp, _c.c_char_p]),
     "cms_backward_cms": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "cms_frozen_cms": (_c.c_int, [_vp, _vp, _vp]),

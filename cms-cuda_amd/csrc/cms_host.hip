@@ -348,6 +348,17 @@ int cms_create(int device, void* stream, cms_ctx** out) {
     delete ctx;
     return CMS_ERR_HIP;
   }
+  {
+    // device address of this device's scratch for cms_nearest_noalloc
+    std::string serr;
+    if (noalloc_setup(device, &serr) != CMS_OK) {
+      g_create_err = serr;
+      ctx->nresc.release();
+      ctx->meta.release();
+      delete ctx;
+      return CMS_ERR_HIP;
+    }
+  }
   fill_u64(ctx->nresc.as<unsigned long long>(), 2, 0ull, ctx->stream);
   fill_int(ctx->meta.as<int>(), M_WORDS, 0, ctx->stream);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -1651,6 +1662,26 @@ int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const 
                           ctx->grp1.as<int>() + 2, 0, 1, ctx->frame.as<double4>(), out_d2, out_idx);
   if (rc) return rc;
   CK(hipStreamSynchronize(st));
+  return CMS_OK;
+}
+
+int cms_nearest_noalloc(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r, int64_t nr,
+                        double* out_d2, int32_t* out_idx) {
+  if (!ctx || nq < 0 || nr < 0 || (nq > 0 && (!q || !out_d2)) || (nr > 0 && !r)) return CMS_ERR_INVALID;
+  if (nq > NOALLOC_MAX_POINTS || nr > NOALLOC_MAX_POINTS)
+    return fail(ctx, CMS_ERR_INVALID, "cms_nearest_noalloc: %lld queries / %lld references exceed the scratch limit "
+                "of %lld points per side", (long long)nq, (long long)nr, (long long)NOALLOC_MAX_POINTS);
+  if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
+    return fail(ctx, CMS_ERR_INVALID, "cms_nearest_noalloc: unsupported precision");
+  CK(hipSetDevice(ctx->device));
+  if (nq == 0) return CMS_OK;
+  std::string err;
+  const int rc = nearest_noalloc(ctx->device, ctx->stream, precision, q, nq, r, nr, out_d2, out_idx, &err);
+  if (rc != CMS_OK) {
+    ctx->err = err;
+    return rc;
+  }
+  CK(hipStreamSynchronize(ctx->stream));
   return CMS_OK;
 }
 

@@ -35,4 +35,10 @@ __global__ void k_dot_bases(int G, const int* rgrp, const int* pgrp, const int* 
                             const int* cscan, const int* t1scan, const int* t2scan,
                             const int* kscan, int64_t* dbase, int* dgrp);
 
+// allocation-free nearest search (cms_noalloc.hip)
+constexpr int64_t NOALLOC_MAX_POINTS = 131072;
+int noalloc_setup(int device, std::string* err);
+int nearest_noalloc(int device, hipStream_t st, int precision, const double* q, int64_t nq, const double* r,
+                    int64_t nr, double* out_d2, int32_t* out_idx, std::string* err);
+
 }  // namespace cms

@@ -203,6 +203,18 @@ int cms_copy_probes(cms_ctx* ctx, int complex_index, int32_t* atoms, double* hei
 int cms_nearest(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r,
                 int64_t nr, double* out_d2, int32_t* out_idx);
 
+/* cms_nearest without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The working buffers live in one fixed device array of
+   21 MiB per GPU (held until the process ends), so nq and nr are each at
+   most 131,072 (CMS_ERR_INVALID beyond).  Results equal cms_nearest's bit
+   for bit.  All calls on one GPU share that array: calls that overlap in
+   time on one GPU (other streams or host threads) race.  Synchronises the
+   context's stream before returning. */
+int cms_nearest_noalloc(cms_ctx* ctx, int precision, const double* q, int64_t nq, const double* r, int64_t nr,
+                        double* out_d2, int32_t* out_idx);
+
 /* Backward pass of cms_contact_ms at frozen topology (dot existence,
    burial, nearest partners and every subdivision count kept from the
    forward pass; dot positions and areas differentiated through the one to
