@@ -46,6 +46,7 @@ _PROTOS = {
     "cms_load": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp, _vp, _c.POINTER(cms_settings)]),
     "cms_generate": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "cms_status": (_c.c_int, [_vp, _vp]),
+    "cms_failed_count": (_c.c_int, [_vp, _c.POINTER(_i32)]),
     "cms_coincident": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_i32), _c.POINTER(_i32)]),
     "cms_counts": (_c.c_int, [_vp, _vp]),
     "cms_contact_ms": (_c.c_int, [_vp, _c.c_int, _vp, _vp]),

@@ -9,6 +9,11 @@ Results per complex are bit-identical to running it alone.
     calc.load([(binder_xyz, binder_radii, target_xyz, target_radii), ...])
     cms = calc.contact_ms()          # dict: target, binder totals + per-atom
     sc = calc.shape_complementarity() # (B, 24) records (see include/cms_hip.h)
+
+A complex with coincident atoms or a neighbour row over capacity fails
+alone: ``calc.status()`` flags it, its results are zero, and the other
+complexes are unaffected.  Only a batch in which every complex fails raises
+(for one complex, the reference's ``RuntimeError``).
 """
 
 import ctypes
@@ -69,9 +74,36 @@ class BatchCalculator:
         self.mode = None
 
     # ------------------------------------------------------------ stages
+    def _surface_call(self, name, *args):
+        """A library call that generates the surface; a one-complex batch
+        with coincident atoms raises the reference's RuntimeError
+        (cms-cuda/_core.py:1285-1293)."""
+        try:
+            self.ctx(name, *args)
+        except _lib.CmsError as e:
+            if e.code == _lib.CMS_ERR_COINCIDENT and self.B == 1:
+                i, j = ctypes.c_int32(), ctypes.c_int32()
+                self.ctx("cms_coincident", 0, ctypes.byref(i), ctypes.byref(j))
+                raise RuntimeError("Coincident atoms: %d:: == %d::" % (i.value, j.value)) from None
+            raise
+
     def generate(self, all_atoms=False, full_state=False):
-        self.ctx("cms_generate", 1 if all_atoms else 0, 1 if full_state else 0)
+        self._surface_call("cms_generate", 1 if all_atoms else 0, 1 if full_state else 0)
         self.mode = bool(all_atoms)
+
+    def status(self):
+        """Per-complex status words of the last generation (0: ok; bit 1
+        coincident atoms, 2 / 4 a neighbour / buried row over capacity, 8 a
+        near-probe list over capacity).  Complexes with a non-zero word have
+        zero results."""
+        out = np.zeros(max(self.B, 1), dtype=np.int32)
+        self.ctx("cms_status", ctypes.c_void_p(out.ctypes.data))
+        return out[:self.B]
+
+    def failed_count(self):
+        n = ctypes.c_int32()
+        self.ctx("cms_failed_count", ctypes.byref(n))
+        return n.value
 
     def contact_ms(self, precision=_lib.PREC_FP64, out=None):
         """CMS of every complex, both sides.  Returns dict with numpy arrays
@@ -106,7 +138,7 @@ class BatchCalculator:
             per_atom = torch.empty(max(self.ntot, 1), dtype=torch.float64, device=self.dev)
             totals = torch.empty(2 * self.B, dtype=torch.float64, device=self.dev)
             res = torch.empty(self.B * _lib.SC_NRES, dtype=torch.float64, device=self.dev) if sc else None
-        self.ctx("cms_run", int(precision), _ptr(per_atom), _ptr(totals), _ptr(res))
+        self._surface_call("cms_run", int(precision), _ptr(per_atom), _ptr(totals), _ptr(res))
         self.mode = False
         if out is not None:
             return None

@@ -122,6 +122,7 @@ struct cms_ctx {
   hipGraphExec_t gexec = nullptr;
   std::vector<uint64_t> gkey, ekey;  // key of the captured graph / of the last eager run
   int64_t n_captures = 0, n_replays = 0, n_exact = 0, n_overflow = 0;
+  int n_failed = 0;                 // complexes of the last run with a non-zero status word
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
   struct Ev { std::string name; hipEvent_t a, b; };
@@ -621,15 +622,24 @@ static int finish_run(cms_ctx* ctx, hipStream_t st, int mode) {
   ctx->nP = hm[M_P];
   ctx->nQ = hm[M_Q];
   ctx->nD = hm[M_D];
-  int worst = 0;
+  // a failed complex (coincident atoms, a row over capacity) has empty rows
+  // and zeroed results, the rest of the batch is unaffected; the run fails
+  // only when every complex did
+  int worst = 0, first = -1, coincident = -1;
+  ctx->n_failed = 0;
   for (int c = 0; c < ctx->B; ++c) {
     const int stc = hm[M_WORDS + c];
-    if (stc & ST_COINCIDENT) return fail(ctx, CMS_ERR_COINCIDENT, "coincident atoms in complex %d", c);
-    if (stc) worst = stc;
+    if (!stc) continue;
+    ++ctx->n_failed;
+    if (first < 0) first = c;
+    if ((stc & ST_COINCIDENT) && coincident < 0) coincident = c;
+    worst |= stc;
   }
-  if (worst)
+  if (ctx->B > 0 && ctx->n_failed == ctx->B) {
+    if (coincident >= 0) return fail(ctx, CMS_ERR_COINCIDENT, "coincident atoms in complex %d", coincident);
     return fail(ctx, CMS_ERR_CAPACITY, "row capacity exceeded (status %d): more than %d neighbours, "
                 "%d buried partners or %d near low probes", worst, KCAP, BCAP, NEARCAP);
+  }
   return CMS_OK;
 }
 
@@ -773,6 +783,9 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
                                                                                 ctx->catom.as<int>(),
                                                                                 ctx->cxyz.as<double4>()));
     // (the neighbour pass counted each row's pairs)
+    k_drop_failed_rows<<<grid_for(est[M_R], 256, 4096), 256, 0, st>>>(cR, ctx->ratom.as<int>(), ctx->acplx.as<int>(),
+                                                                      ctx->status.as<int>(), ctx->nn.as<int>(),
+                                                                      ctx->nb.as<int>(), ctx->pcnt.as<int>());
     CKL();
   }
   scan_excl(ctx->pcnt.as<int>(), ctx->pscan.as<int>(), cR, ctx->cap[M_R], part, st);
@@ -924,6 +937,12 @@ int cms_status(cms_ctx* ctx, int32_t* status_host) {
   if (!ctx || !status_host) return CMS_ERR_INVALID;
   CK(hipMemcpyAsync(status_host, ctx->status.p, sizeof(int) * ctx->B, hipMemcpyDeviceToHost, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
+  return CMS_OK;
+}
+
+int cms_failed_count(cms_ctx* ctx, int32_t* n_failed) {
+  if (!ctx || !n_failed) return CMS_ERR_INVALID;
+  *n_failed = ctx->n_failed;
   return CMS_OK;
 }
 
@@ -1086,6 +1105,14 @@ static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* 
   return CMS_OK;
 }
 
+// Outputs of the complexes with a non-zero status word set to zero (a
+// kernel, so the step stays capturable).
+static void zero_failed(cms_ctx* ctx, double* per_atom, int width, double* totals, int tstride, double* sc) {
+  k_zero_failed<<<grid_for(ctx->B, 1, 4096), 256, 0, ctx->stream>>>(ctx->B, ctx->status.as<int>(),
+                                                                     ctx->aoff.as<int>(), per_atom, width, totals,
+                                                                     tstride, sc, CMS_SC_NRES);
+}
+
 static int contact_ms_pass(cms_ctx* ctx, int precision, double* per_atom, double* totals) {
   int r = prepare_lists(ctx);
   if (r) return r;
@@ -1105,6 +1132,7 @@ static int contact_ms_pass(cms_ctx* ctx, int precision, double* per_atom, double
   CKL();
   r = owner_sums(ctx, cB, estB, ctx->blist.as<int>(), ctx->wv.as<double>(), 1, per_atom, totals, G, 1);
   if (r) return r;
+  zero_failed(ctx, per_atom, 1, totals, 2, nullptr);
   ctx->cms_ready = true;
   return CMS_OK;
 }
@@ -1126,8 +1154,12 @@ int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals) {
   CK(hipSetDevice(ctx->device));
   // items: all dots; value: dpos[i].w (area)
   const double* val = reinterpret_cast<const double*>(ctx->dpos.as<double4>()) + 3;
-  return owner_sums(ctx, dev_count(ctx->meta.as<int>() + M_D), ctx->est[M_D], nullptr, val, 4, per_atom, totals,
-                    2 * ctx->B, 2);
+  int r = owner_sums(ctx, dev_count(ctx->meta.as<int>() + M_D), ctx->est[M_D], nullptr, val, 4, per_atom, totals,
+                     2 * ctx->B, 2);
+  if (r) return r;
+  zero_failed(ctx, per_atom, 1, totals, 1, nullptr);
+  CKL();
+  return CMS_OK;
 }
 
 static int sc_pass(cms_ctx* ctx, int precision, double* results) {
@@ -1212,6 +1244,7 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
                                                     ctx->tdist.as<double>(), ctx->trr.as<double>(),
                                                     ctx->dgrp.as<int>(), ctx->s, ctx->side.as<double>()));
   k_sc_combine<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->side.as<double>(), results);
+  zero_failed(ctx, nullptr, 0, nullptr, 0, results);
   CKL();
   return CMS_OK;
 }
@@ -1758,6 +1791,7 @@ int cms_backward_cms(cms_ctx* ctx, const double* g_totals, const double* g_per_a
   TIMED("dot_grad", k_dot_grad<<<grid_for(std::max<int64_t>(estD, 1), 64, 16384), 64, 0, st>>>(
       view(ctx), ctx->s, dev_count(meta + M_D), ctx->ddesc.as<int4>(), ctx->ddesc2.as<int4>(),
       ctx->adj.as<double>(), grad_xyz));
+  zero_failed(ctx, grad_xyz, 3, nullptr, 0, nullptr);
   CKL();
   return CMS_OK;
 }
@@ -1791,6 +1825,7 @@ int cms_frozen_cms(cms_ctx* ctx, const double* xyz_eval, double* totals) {
   // per-group sums in buried-dot order: deterministic, so finite differences
   // see no summation-order noise
   k_group_np_sum<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->bgrp.as<int>(), ctx->wv.as<double>(), 1, totals);
+  zero_failed(ctx, nullptr, 0, totals, 2, nullptr);
   CKL();
   return CMS_OK;
 }

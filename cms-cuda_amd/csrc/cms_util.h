@@ -35,6 +35,11 @@ __global__ void k_dot_bases(int G, const int* rgrp, const int* pgrp, const int* 
                             const int* cscan, const int* t1scan, const int* t2scan,
                             const int* kscan, int64_t* dbase, int* dgrp);
 
+__global__ void k_drop_failed_rows(Count cnt, const int* ratom, const int* acplx, const int* status, int* nn,
+                                   int* nb, int* pcnt);
+__global__ void k_zero_failed(int B, const int* status, const int* aoff, double* per_atom, int width,
+                              double* totals, int tstride, double* sc, int sc_width);
+
 // allocation-free nearest search (cms_noalloc.hip)
 constexpr int64_t NOALLOC_MAX_POINTS = 131072;
 int noalloc_setup(int device, std::string* err);

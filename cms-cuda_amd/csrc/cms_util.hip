@@ -346,4 +346,37 @@ __global__ __launch_bounds__(256) void k_dot_bases(int G, const int* __restrict_
   if (threadIdx.x == 0) dgrp[G] = carry;
 }
 
+// ------------------------------------------------------------ failed complexes
+// A complex whose status word flags coincident atoms or a neighbour / buried
+// row over capacity gets empty rows right after the neighbour pass: no
+// pairs, probes or dots, so it cannot disturb the rest of the batch.
+__global__ void k_drop_failed_rows(Count cnt, const int* __restrict__ ratom, const int* __restrict__ acplx,
+                                   const int* __restrict__ status, int* __restrict__ nn, int* __restrict__ nb,
+                                   int* __restrict__ pcnt) {
+  const int n = cnt.get();
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    if (status[acplx[ratom[r]]] & (ST_COINCIDENT | ST_NBR_CAP | ST_BUR_CAP)) {
+      nn[r] = 0;
+      nb[r] = 0;
+      pcnt[r] = 0;
+    }
+  }
+}
+
+// Outputs of every complex with a non-zero status word set to 0: per-atom
+// values (width doubles per atom), totals (tstride per complex), SC records
+// (sc_width per complex).  One workgroup per complex.
+__global__ void k_zero_failed(int B, const int* __restrict__ status, const int* __restrict__ aoff, double* per_atom,
+                              int width, double* totals, int tstride, double* sc, int sc_width) {
+  for (int c = blockIdx.x; c < B; c += gridDim.x) {
+    if (status[c] == 0) continue;
+    if (per_atom)
+      for (int64_t i = (int64_t)width * aoff[c] + threadIdx.x; i < (int64_t)width * aoff[c + 1]; i += blockDim.x)
+        per_atom[i] = 0.0;
+    if (totals && (int)threadIdx.x < tstride) totals[(int64_t)c * tstride + threadIdx.x] = 0.0;
+    if (sc)
+      for (int k = threadIdx.x; k < sc_width; k += blockDim.x) sc[(int64_t)c * sc_width + k] = 0.0;
+  }
+}
+
 }  // namespace cms

@@ -102,13 +102,22 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
    inspection).  Cached: a second call with the same all_atoms is a no-op,
    one with the other mode returns CMS_ERR_STATE.  Replaces
    _ensure_surfaces_generated (cms-cuda/_core.py:865-909, 1197-2377;
-   _gpu.py:421-461, 691-1765).  Returns CMS_ERR_COINCIDENT when a complex
-   holds coincident atoms (see cms_coincident). */
+   _gpu.py:421-461, 691-1765).  A complex holding coincident atoms (see
+   cms_coincident) or a row over capacity fails alone: its status word is
+   set, it gets no dots and zero results in every readout, and the rest of
+   the batch is unaffected (cms_failed_count).  The call returns
+   CMS_ERR_COINCIDENT / CMS_ERR_CAPACITY only when every complex of the
+   batch failed -- for one complex, the reference's RuntimeError
+   (cms-cuda/_core.py:1285-1293). */
 int cms_generate(cms_ctx* ctx, int all_atoms, int full_state);
 
 /* Per-complex status word after cms_generate (bit 1 coincident atoms, 2 / 4
    neighbour / buried row over capacity, 8 near-probe list over capacity). */
 int cms_status(cms_ctx* ctx, int32_t* status_host);
+
+/* Number of complexes of the last cms_generate / cms_run whose status word
+   is non-zero (their results are zero).  No host wait. */
+int cms_failed_count(cms_ctx* ctx, int32_t* n_failed);
 
 /* First coincident pair (row-major, as np.argwhere) of complex c, local
    atom indices; -1/-1 if none.  Mirrors the RuntimeError of
