@@ -134,6 +134,29 @@ class ProbeArray:
         return len(self.height) > 0
 
 
+class NeighborArray:
+    """Host copy of neighbour or buried rows (the fields of the reference's
+    SimpleNeighborArray, _core.py:596-611): natom (N, K) int32 padded with
+    -1, radius (N, K) and xyz (N, K, 3) padded with NaN, nneighbors (N);
+    K = the longest row (at least 1)."""
+
+    def __init__(self, nneighbors, natom, xyz, radius):
+        n, k = natom.shape
+        self.nneighbors = nneighbors.astype(np.int32)
+        self.natom = natom.astype(np.int32)
+        self.xyz = np.full((n, k, 3), np.nan)
+        self.radius = np.full((n, k), np.nan)
+        m = self.natom >= 0
+        self.xyz[m] = xyz[self.natom[m]]
+        self.radius[m] = radius[self.natom[m]]
+
+    def __len__(self):
+        return len(self.nneighbors)
+
+    def __bool__(self):
+        return len(self.nneighbors) > 0
+
+
 class AtomArray:
     def __init__(self, xyz, radius, density, molecule):
         n = len(radius)
@@ -166,6 +189,9 @@ class RunState:
         self._dots = None
         self._probes = None
         self._atoms = None
+        self._trimmed = None
+        self._rows = None
+        self._toroid = None
 
     @property
     def radmax(self):
@@ -190,6 +216,34 @@ class RunState:
             self._atoms = self._calc._copy_atoms()
         return self._atoms
 
+    @property
+    def trimmed_dots(self):
+        """Buried dots kept by the SC trim, per molecule (empty until
+        CalcLoadedSC; _core.py:828-838)."""
+        if self._trimmed is None:
+            self._trimmed = self._calc._copy_trimmed()
+        return self._trimmed
+
+    @property
+    def neighbor_array(self):
+        if self._rows is None:
+            self._rows = self._calc._copy_rows()
+        return self._rows[0]
+
+    @property
+    def buried_array(self):
+        if self._rows is None:
+            self._rows = self._calc._copy_rows()
+        return self._rows[1]
+
+    @property
+    def toroid_queue(self):
+        """List of (atom 1, atom 2, uij, tij, rij, between) in the reference's
+        order (_core.py:1419-1431)."""
+        if self._toroid is None:
+            self._toroid = self._calc._copy_toroid_queue()
+        return self._toroid
+
 
 # ------------------------------------------------------------ calculator
 
@@ -201,6 +255,7 @@ class MolecularSurfaceCalculator:
         self.settings = Settings()
         self._ctx = None
         self._fin = None
+        self._full = None
         self.reset()
 
     # ---------------------------------------------------------- state
@@ -228,6 +283,10 @@ class MolecularSurfaceCalculator:
             fin()
         self._fin = None
         self._ctx = None
+        full = getattr(self, "_full", None)
+        if full is not None:
+            full.close()
+        self._full = None
 
     def _radii_all(self):
         return np.concatenate(self._rad)
@@ -343,7 +402,8 @@ class MolecularSurfaceCalculator:
         (molecule 1) when target_side, else the binder.  _core.py:946-1004."""
         n0, n1 = len(self._rad[0]), len(self._rad[1])
         nq = n1 if target_side else n0
-        if self._mode is not False:
+        # like the reference, read whatever surface exists (none: zeros)
+        if self._mode is None:
             return 0.0, np.zeros(nq, dtype=np.float64)
         if self._cms is None:
             self._cms = self._compute_cms()
@@ -355,7 +415,7 @@ class MolecularSurfaceCalculator:
     def calc_max_possible_contact_molecular_surface(self, target_side=True):
         """(total area, per-atom area) of molecule 0.  _core.py:1007-1034."""
         n0 = len(self._rad[0])
-        if self._mode is not True:
+        if self._mode is None:
             return 0.0, np.zeros(n0, dtype=np.float64)
         if self._maxcms is None:
             torch = _torch()
@@ -443,31 +503,106 @@ class MolecularSurfaceCalculator:
         return ProbeArray(atoms[:n].cpu().numpy(), h[:n].cpu().numpy(), p[:n].cpu().numpy(),
                           a[:n].cpu().numpy())
 
-    def _copy_atoms(self):
-        xyz = np.concatenate(self._xyz)
-        rad = self._radii_all()
-        mol = np.concatenate([np.zeros(len(self._rad[0]), np.int8), np.ones(len(self._rad[1]), np.int8)])
-        arr = AtomArray(xyz, rad, np.concatenate(self._dens), mol)
-        if self._mode is not None and len(rad):
-            # the cached surface covers the interface region only; the
-            # reference's per-atom state covers every atom, so it comes from a
-            # full-state generation of the same inputs (identical dots)
+    def _full_state(self):
+        """A context holding a full-state generation of the same inputs and
+        mode (rows for every atom, identical dots): the reference's per-atom
+        state and neighbour tables cover every atom, the cached surface only
+        the interface region."""
+        if self._full is None:
             torch = _torch()
-            att = torch.empty(len(rad), dtype=torch.int8, device=self._dev)
-            acc = torch.empty(len(rad), dtype=torch.int8, device=self._dev)
+            xyz = np.concatenate(self._xyz)
             full = _lib.Context(self._dev.index, torch.cuda.current_stream(self._dev).cuda_stream)
             try:
                 d_xyz = torch.from_numpy(np.ascontiguousarray(xyz, dtype=np.float64)).to(self._dev)
-                d_rad = torch.from_numpy(np.ascontiguousarray(rad, dtype=np.float64)).to(self._dev)
+                d_rad = torch.from_numpy(np.ascontiguousarray(self._radii_all(), dtype=np.float64)).to(self._dev)
                 d_den = torch.from_numpy(np.ascontiguousarray(np.concatenate(self._dens), dtype=np.float64)).to(self._dev)
                 nb = (ctypes.c_int32 * 1)(len(self._rad[0]))
                 nt = (ctypes.c_int32 * 1)(len(self._rad[1]))
                 st = self.settings.to_c()
                 full("cms_load", 1, nb, nt, _ptr(d_xyz), _ptr(d_rad), _ptr(d_den), ctypes.byref(st))
                 full("cms_generate", 1 if self._mode else 0, 1)
-                full("cms_copy_atom_state", _ptr(att), _ptr(acc))
-                arr.atten = att.cpu().numpy()
-                arr.access = acc.cpu().numpy()
-            finally:
+                torch.cuda.current_stream(self._dev).synchronize()
+            except Exception:
                 full.close()
+                raise
+            self._full = full
+        return self._full
+
+    def _copy_atoms(self):
+        xyz = np.concatenate(self._xyz)
+        rad = self._radii_all()
+        mol = np.concatenate([np.zeros(len(self._rad[0]), np.int8), np.ones(len(self._rad[1]), np.int8)])
+        arr = AtomArray(xyz, rad, np.concatenate(self._dens), mol)
+        if self._mode is not None and len(rad):
+            torch = _torch()
+            att = torch.empty(len(rad), dtype=torch.int8, device=self._dev)
+            acc = torch.empty(len(rad), dtype=torch.int8, device=self._dev)
+            self._full_state()("cms_copy_atom_state", _ptr(att), _ptr(acc))
+            arr.atten = att.cpu().numpy()
+            arr.access = acc.cpu().numpy()
         return arr
+
+    def _copy_rows(self):
+        """(neighbor_array, buried_array) of every atom, or (None, None)
+        before a surface exists."""
+        if self._mode is None:
+            return None, None
+        torch = _torch()
+        n = len(self._radii_all())
+        xyz = np.concatenate(self._xyz)
+        rad = self._radii_all()
+        out = []
+        full = self._full_state()
+        for which in (0, 1):
+            cnt = torch.zeros(max(n, 1), dtype=torch.int32, device=self._dev)
+            full("cms_copy_rows", 0, which, _ptr(cnt), None, 0)
+            counts = cnt[:n].cpu().numpy()
+            k = max(int(counts.max()) if n else 0, 1)
+            nat = torch.empty((max(n, 1), k), dtype=torch.int32, device=self._dev)
+            full("cms_copy_rows", 0, which, None, _ptr(nat), k)
+            out.append(NeighborArray(counts, nat[:n].cpu().numpy(), xyz, rad))
+        return tuple(out)
+
+    def _copy_trimmed(self):
+        if self._mode is None or self._sc is None:
+            return [DotArray.empty(), DotArray.empty()]
+        torch = _torch()
+        out = []
+        for m in (0, 1):
+            n = ctypes.c_int64()
+            self._ctx("cms_trimmed_count", 0, m, ctypes.byref(n))
+            n = n.value
+            if n == 0:
+                out.append(DotArray.empty())
+                continue
+            dev = self._dev
+            coor = torch.empty((n, 3), dtype=torch.float64, device=dev)
+            nml = torch.empty((n, 3), dtype=torch.float64, device=dev)
+            area = torch.empty(n, dtype=torch.float64, device=dev)
+            bur = torch.empty(n, dtype=torch.int8, device=dev)
+            typ = torch.empty(n, dtype=torch.int8, device=dev)
+            atom = torch.empty(n, dtype=torch.int32, device=dev)
+            self._ctx("cms_copy_trimmed_dots", 0, m, _ptr(coor), _ptr(nml), _ptr(area), _ptr(bur), _ptr(typ),
+                      _ptr(atom))
+            out.append(DotArray(coor.cpu().numpy(), nml.cpu().numpy(), area.cpu().numpy(),
+                                bur.cpu().numpy(), typ.cpu().numpy(), atom.cpu().numpy()))
+        return out
+
+    def _copy_toroid_queue(self):
+        if self._mode is None:
+            return []
+        torch = _torch()
+        n = ctypes.c_int64()
+        self._ctx("cms_toroid_count", 0, ctypes.byref(n))
+        n = n.value
+        if n == 0:
+            return []
+        dev = self._dev
+        at = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        u = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        t = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        r = torch.empty(n, dtype=torch.float64, device=dev)
+        b = torch.empty(n, dtype=torch.int8, device=dev)
+        self._ctx("cms_copy_toroid_queue", 0, _ptr(at), _ptr(u), _ptr(t), _ptr(r), _ptr(b))
+        at, u, t, r, b = (x.cpu().numpy() for x in (at, u, t, r, b))
+        return [(int(at[i, 0]), int(at[i, 1]), u[i], t[i], float(r[i]), bool(b[i])) for i in range(n)]

@@ -61,6 +61,11 @@ _PROTOS = {
     "cms_copy_dot_desc": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp]),
     "cms_probe_count": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_i64)]),
     "cms_copy_probes": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp]),
+    "cms_copy_rows": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _c.c_int]),
+    "cms_trimmed_count": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.POINTER(_i64)]),
+    "cms_copy_trimmed_dots": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cms_toroid_count": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_i64)]),
+    "cms_copy_toroid_queue": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp, _vp]),
     "cms_nearest": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),
     "cms_nearest_noalloc": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _vp, _i64, _vp, _vp]),
     "cms_profile": (_c.c_int, [_vp, _c.c_char_p]),

@@ -64,6 +64,12 @@ class _ContactMS:
             self.ctx.close()
             self.ctx = None
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def _function():
     torch = _torch()
@@ -78,10 +84,12 @@ def _function():
 
         @staticmethod
         def backward(fctx, g_totals, g_per_atom):
+            # the state (and its library context) lives as long as the graph
+            # node, so a graph kept with retain_graph=True can be
+            # differentiated again
             state = fctx.state
             grad = state.backward(g_totals.reshape(-1) if g_totals is not None else None,
                                   g_per_atom)
-            state.close()
             return grad, None, None, None, None, None
 
     return ContactMSFunction

@@ -89,9 +89,13 @@ struct cms_ctx {
   DBuf trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
   DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;   // frame: cms_nearest's point frame
-  DBuf bidx, bgof, adj, eatom, evpos, chk, qlast;
+  DBuf bidx, bgof, adj, eatom, evpos, chk, qlast, tqidx;
   DBuf tgrid, tncell, tncscan, tacell, tcount, tstart, tcur, tsorted;   // trim grid
   bool cms_ready = false;
+  bool sc_ready = false;            // trimmed lists of the current surface exist
+  bool host_tgroups = false;        // h_tgrp holds the trimmed-list offsets
+  std::vector<int> h_tgrp, h_pgrp, h_tq;
+  int h_tq_c = -1;                  // complex whose toroid-queue pairs h_tq lists
   bool float_lists = false;
   // per-run count block and capacities
   DBuf meta;
@@ -123,6 +127,7 @@ struct cms_ctx {
   std::vector<uint64_t> gkey, ekey;  // key of the captured graph / of the last eager run
   int64_t n_captures = 0, n_replays = 0, n_exact = 0, n_overflow = 0;
   int n_failed = 0;                 // complexes of the last run with a non-zero status word
+  int trim_lanes = 0;               // lanes per buried dot in the SC trim (0: by batch size)
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
   struct Ev { std::string name; hipEvent_t a, b; };
@@ -284,7 +289,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
                   &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
                   &ctx->bxyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
-                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->tgrid, &ctx->tncell, &ctx->tncscan, &ctx->tacell,
+                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->tqidx, &ctx->tgrid, &ctx->tncell, &ctx->tncscan, &ctx->tacell,
           &ctx->tcount, &ctx->tstart, &ctx->tcur, &ctx->tsorted, &ctx->meta};
 }
 
@@ -336,6 +341,10 @@ int cms_create(int device, void* stream, cms_ctx** out) {
   if (const char* pm = std::getenv("CMS_PROBE_COUNT")) {
     if (!std::strcmp(pm, "pairs")) ctx->probe_count_mode = 1;
     if (!std::strcmp(pm, "rows")) ctx->probe_count_mode = 2;
+  }
+  if (const char* tl = std::getenv("CMS_TRIM_LANES")) {
+    const int v = std::atoi(tl);
+    if (v == 1 || v == 4 || v == 32) ctx->trim_lanes = v;
   }
   if (const char* ps = std::getenv("CMS_PROBE_SLOTS")) {
     ctx->probe_slots = std::min(std::max(std::atoi(ps), 0), PSLOTS);
@@ -474,6 +483,9 @@ int cms_load(cms_ctx* ctx, int n_complex, const int32_t* n_binder, const int32_t
   ctx->lists_ready = false;
   ctx->float_lists = false;
   ctx->cms_ready = false;
+  ctx->sc_ready = false;
+  ctx->host_tgroups = false;
+  ctx->h_tq_c = -1;
   ctx->host_groups = false;
   return CMS_OK;
 }
@@ -1139,8 +1151,9 @@ static int contact_ms_pass(cms_ctx* ctx, int precision, double* per_atom, double
 
 int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals) {
   if (!ctx || !per_atom || !totals) return CMS_ERR_INVALID;
-  if (!ctx->generated || ctx->gen_all_atoms != 0)
-    return fail(ctx, CMS_ERR_STATE, "cms_contact_ms: surface not generated in CMS mode");
+  // any generated surface, like the reference's readout of whatever dots
+  // exist (cms-cuda/_core.py:946-1004)
+  if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_contact_ms: no surface");
   if (precision != CMS_PREC_FP64 && precision != CMS_PREC_FP32_RESCORE)
     return fail(ctx, CMS_ERR_INVALID, "cms_contact_ms: unsupported precision");
   CK(hipSetDevice(ctx->device));
@@ -1149,8 +1162,8 @@ int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals
 
 int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals) {
   if (!ctx || !per_atom || !totals) return CMS_ERR_INVALID;
-  if (!ctx->generated || ctx->gen_all_atoms != 1)
-    return fail(ctx, CMS_ERR_STATE, "cms_max_contact_ms: surface not generated in all_atoms mode");
+  // any generated surface (cms-cuda/_core.py:1007-1034)
+  if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_max_contact_ms: no surface");
   CK(hipSetDevice(ctx->device));
   // items: all dots; value: dpos[i].w (area)
   const double* val = reinterpret_cast<const double*>(ctx->dpos.as<double4>()) + 3;
@@ -1203,7 +1216,7 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
     // 2,112 complexes: 374 us against 451, 406, 424 and 596 us with 1, 2, 8
     // and 16 lanes per dot)
     const int64_t nb = std::max<int64_t>(estB, 1);
-    const int L = nb >= (1 << 23) ? 1 : nb >= (1 << 14) ? 4 : 32;
+    const int L = ctx->trim_lanes ? ctx->trim_lanes : nb >= (1 << 23) ? 1 : nb >= (1 << 14) ? 4 : 32;
     const int gL = grid_for(nb * L, 256, 8192);
     const double band2 = ctx->s.band * ctx->s.band;
     const int* cs = ctx->tstart.as<int>();
@@ -1246,6 +1259,8 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   k_sc_combine<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->side.as<double>(), results);
   zero_failed(ctx, nullptr, 0, nullptr, 0, results);
   CKL();
+  ctx->sc_ready = true;
+  ctx->host_tgroups = false;
   return CMS_OK;
 }
 
@@ -1361,6 +1376,9 @@ int cms_run(cms_ctx* ctx, int precision, double* per_atom, double* totals, doubl
       ctx->lists_ready = true;
       ctx->float_lists = precision == CMS_PREC_FP32_RESCORE;
       ctx->cms_ready = true;
+      ctx->sc_ready = sc_results != nullptr;
+      ctx->host_tgroups = false;
+      ctx->h_tq_c = -1;
       ctx->host_groups = false;
     } else {
       r = generate_pass(ctx, 0, 0, false);
@@ -1561,6 +1579,151 @@ int cms_copy_probes(cms_ctx* ctx, int c, int32_t* atoms, double* height, double*
       (int)n, ctx->h_prgrp[2 * c], ctx->h_aoff[c], ctx->patom.as<int4>(), ctx->ppos.as<double4>(),
       ctx->palt.as<double4>(), atoms, height, point, alt);
   CKL();
+  return CMS_OK;
+}
+
+// ------------------------------------------------------------ run-state copies
+// Neighbour (which = 0) or buried (which = 1) rows of complex c's atoms: row
+// length and entries as local atom indices, -1 past the row's end (the
+// reference's neighbor_array / buried_array natom tables).
+__global__ void k_copy_rows(int n, int a0, const int* rpos, const int* cnt, const int* rows, int rowcap, int width,
+                            int32_t* counts, int32_t* natom) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int a = a0 + i;
+    const int r = rpos[a];
+    const int k = (rpos[a + 1] > r) ? min(cnt[r], rowcap) : 0;
+    if (counts) counts[i] = k;
+    if (natom)
+      for (int t = 0; t < width; ++t) natom[(int64_t)i * width + t] = t < k ? rows[(int64_t)r * rowcap + t] - a0 : -1;
+  }
+}
+
+int cms_copy_rows(cms_ctx* ctx, int c, int which, int32_t* counts, int32_t* natom, int width) {
+  if (!ctx || c < 0 || c >= ctx->B || which < 0 || which > 1 || (natom && width < 0)) return CMS_ERR_INVALID;
+  if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_copy_rows: no surface");
+  const int a0 = ctx->h_aoff[c], n = ctx->h_aoff[c + 1] - a0;
+  if (n == 0 || (!counts && !natom)) return CMS_OK;
+  k_copy_rows<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(
+      n, a0, ctx->rpos.as<int>(), which ? ctx->nb.as<int>() : ctx->nn.as<int>(),
+      which ? ctx->bur.as<int>() : ctx->nbr.as<int>(), which ? BCAP : KCAP, width, counts, natom);
+  CKL();
+  return CMS_OK;
+}
+
+static int ensure_host_tgroups(cms_ctx* ctx) {
+  if (ctx->host_tgroups) return CMS_OK;
+  const int G = 2 * ctx->B;
+  ctx->h_tgrp.assign(G + 1, 0);
+  CK(hipMemcpyAsync(ctx->h_tgrp.data(), ctx->tgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->host_tgroups = true;
+  return CMS_OK;
+}
+
+int cms_trimmed_count(cms_ctx* ctx, int c, int mol, int64_t* n) {
+  if (!ctx || !n || c < 0 || c >= ctx->B || mol < 0 || mol > 1) return CMS_ERR_INVALID;
+  if (!ctx->generated || !ctx->sc_ready)
+    return fail(ctx, CMS_ERR_STATE, "cms_trimmed_count: run cms_shape_complementarity on the current surface first");
+  int r = ensure_host_tgroups(ctx);
+  if (r) return r;
+  const int g = 2 * c + mol;
+  *n = ctx->h_tgrp[g + 1] - ctx->h_tgrp[g];
+  return CMS_OK;
+}
+
+__global__ void k_unpack_dot_list(int n, const int* list, int lo, int aoff, const double4* dpos, const double4* dnml,
+                                  const int8_t* dbur, const int8_t* dtype, const int* datom, double* coor, double* nml,
+                                  double* area, int8_t* buried, int8_t* type, int32_t* atom) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int d = list[lo + i];
+    const double4 p = dpos[d], q = dnml[d];
+    if (coor) { coor[3 * i] = p.x; coor[3 * i + 1] = p.y; coor[3 * i + 2] = p.z; }
+    if (nml) { nml[3 * i] = q.x; nml[3 * i + 1] = q.y; nml[3 * i + 2] = q.z; }
+    if (area) area[i] = p.w;
+    if (buried) buried[i] = dbur[d];
+    if (type) type[i] = dtype[d];
+    if (atom) atom[i] = datom[d] - aoff;
+  }
+}
+
+int cms_copy_trimmed_dots(cms_ctx* ctx, int c, int mol, double* coor, double* nml, double* area, int8_t* buried,
+                          int8_t* type, int32_t* atom) {
+  int64_t n = 0;
+  int r = cms_trimmed_count(ctx, c, mol, &n);
+  if (r) return r;
+  if (n == 0) return CMS_OK;
+  const int g = 2 * c + mol;
+  k_unpack_dot_list<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(
+      (int)n, ctx->tlist.as<int>(), ctx->h_tgrp[g], ctx->h_aoff[c], ctx->dpos.as<double4>(), ctx->dnml.as<double4>(),
+      ctx->dbur.as<int8_t>(), ctx->dtype.as<int8_t>(), ctx->datom.as<int>(), coor, nml, area, buried, type, atom);
+  CKL();
+  return CMS_OK;
+}
+
+// Toroid-queue pairs of complex c (pair flag bit 2), in pair order: the
+// reference's queue order (atoms ascending, each atom's forward neighbours
+// in row order, cms-cuda/_core.py:1419-1431).
+static int toroid_pairs(cms_ctx* ctx, int c) {
+  if (ctx->h_tq_c == c) return CMS_OK;
+  const int G = 2 * ctx->B;
+  ctx->h_pgrp.assign(G + 1, 0);
+  CK(hipMemcpyAsync(ctx->h_pgrp.data(), ctx->pgrp.p, sizeof(int) * (G + 1), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  const int p0 = ctx->h_pgrp[2 * c], p1 = ctx->h_pgrp[2 * c + 2];
+  std::vector<int8_t> fl(std::max(p1 - p0, 0));
+  if (p1 > p0) {
+    CK(hipMemcpyAsync(fl.data(), ctx->pflag.as<int8_t>() + p0, p1 - p0, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->h_tq.clear();
+  for (int p = p0; p < p1; ++p)
+    if (fl[p - p0] & 2) ctx->h_tq.push_back(p);
+  ctx->h_tq_c = c;
+  return CMS_OK;
+}
+
+int cms_toroid_count(cms_ctx* ctx, int c, int64_t* n) {
+  if (!ctx || !n || c < 0 || c >= ctx->B) return CMS_ERR_INVALID;
+  if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_toroid_count: no surface");
+  int r = toroid_pairs(ctx, c);
+  if (r) return r;
+  *n = (int64_t)ctx->h_tq.size();
+  return CMS_OK;
+}
+
+// Queue entries from the pairs' atoms with the pair geometry of the
+// surface kernels (pair_torus), so they equal what those kernels used.
+__global__ void k_toroid_entries(int n, const int* idx, const int* pi, const int* pj, const double4* atom, double rp,
+                                 int aoff, int32_t* atoms, double* uij, double* tij, double* rij, int8_t* between) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int p = idx[i];
+    const int a = pi[p], b = pj[p];
+    const double4 xa = atom[a], xb = atom[b];
+    V3 u, t;
+    double rr = 0.0;
+    bool bt = false;
+    pair_torus(ld3(xa), xa.w, ld3(xb), xb.w, rp, &u, &t, &rr, &bt);
+    if (atoms) { atoms[2 * i] = a - aoff; atoms[2 * i + 1] = b - aoff; }
+    if (uij) { uij[3 * i] = u.x; uij[3 * i + 1] = u.y; uij[3 * i + 2] = u.z; }
+    if (tij) { tij[3 * i] = t.x; tij[3 * i + 1] = t.y; tij[3 * i + 2] = t.z; }
+    if (rij) rij[i] = rr;
+    if (between) between[i] = bt ? 1 : 0;
+  }
+}
+
+int cms_copy_toroid_queue(cms_ctx* ctx, int c, int32_t* atoms, double* uij, double* tij, double* rij,
+                          int8_t* between) {
+  int64_t n = 0;
+  int r = cms_toroid_count(ctx, c, &n);
+  if (r) return r;
+  if (n == 0) return CMS_OK;
+  CK(ctx->tqidx.ensure(sizeof(int) * n));
+  CK(hipMemcpyAsync(ctx->tqidx.p, ctx->h_tq.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  k_toroid_entries<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(
+      (int)n, ctx->tqidx.as<int>(), ctx->pi.as<int>(), ctx->pj.as<int>(), ctx->atom.as<double4>(), ctx->s.rp,
+      ctx->h_aoff[c], atoms, uij, tij, rij, between);
+  CKL();
+  CK(hipStreamSynchronize(ctx->stream));
   return CMS_OK;
 }
 

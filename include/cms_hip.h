@@ -137,8 +137,9 @@ int cms_counts(cms_ctx* ctx, int64_t* out_host);
    (cms-cuda/_core.py:946-1004; _gpu.py:515-571). */
 int cms_contact_ms(cms_ctx* ctx, int precision, double* per_atom, double* totals);
 
-/* Maximum possible CMS (surface area of molecule 0).  Requires a surface
-   generated with all_atoms = 1.  per_atom: device, ntot; totals: device, B.
+/* Maximum possible CMS (surface area of molecule 0) from the current
+   surface's dots -- the max-CMS value on a surface generated with
+   all_atoms = 1; like the reference, any generated surface is read.  per_atom: device, ntot; totals: device, B.
    Replaces calc_max_possible_contact_molecular_surface
    (cms-cuda/_core.py:1007-1034; _gpu.py:573-600). */
 int cms_max_contact_ms(cms_ctx* ctx, double* per_atom, double* totals);
@@ -203,6 +204,33 @@ int cms_copy_atom_state(cms_ctx* ctx, int8_t* atten, int8_t* access);
 int cms_probe_count(cms_ctx* ctx, int complex_index, int64_t* n);
 int cms_copy_probes(cms_ctx* ctx, int complex_index, int32_t* atoms, double* height,
                     double* point, double* alt);
+
+/* Neighbour rows (which = 0) or buried rows (which = 1) of complex c's
+   atoms: counts (device, n_atoms(c) int32, or NULL) and natom (device,
+   n_atoms(c) * width int32, or NULL) with each row's atoms as local
+   indices, -1 past its end; rows longer than width are cut.  Neighbour
+   rows are ordered by (d^2, index), buried rows by index.  Atoms outside
+   the interface region have rows only in a full_state generation.
+   Counterpart of calc.run.neighbor_array / .buried_array .nneighbors and
+   .natom (cms-cuda/_core.py:1318-1348; _gpu.py:270-296). */
+int cms_copy_rows(cms_ctx* ctx, int complex_index, int which, int32_t* counts, int32_t* natom, int width);
+
+/* Trimmed dots of (complex, molecule) -- the buried dots farther than the
+   band from every accessible dot of their molecule, in dot order -- after
+   cms_shape_complementarity (or cms_run with SC) on the current surface;
+   fields as cms_copy_dots.  Counterpart of calc.run.trimmed_dots
+   (cms-cuda/_core.py:1037-1077, 828-838). */
+int cms_trimmed_count(cms_ctx* ctx, int complex_index, int mol, int64_t* n);
+int cms_copy_trimmed_dots(cms_ctx* ctx, int complex_index, int mol, double* coor, double* nml, double* area,
+                          int8_t* buried, int8_t* type, int32_t* atom);
+
+/* Toroid queue of complex c in the reference's order: atoms (device, 2n
+   int32 local indices: atom 1, atom 2), uij (3n), tij (3n), rij (n) float64,
+   between (n int8); any pointer may be NULL.  Synchronous.  Counterpart of
+   calc.run.toroid_queue (cms-cuda/_core.py:1419-1431; _gpu.py:262-296). */
+int cms_toroid_count(cms_ctx* ctx, int complex_index, int64_t* n);
+int cms_copy_toroid_queue(cms_ctx* ctx, int complex_index, int32_t* atoms, double* uij, double* tij, double* rij,
+                          int8_t* between);
 
 /* Standalone nearest-point search: for each of nq query points (device,
    3*nq float64) the squared distance to the nearest of nr reference

@@ -788,9 +788,14 @@ class OracleCalculator:
         return int((self.mol == m).sum())
 
     def contact_ms(self, target_side=True):
-        """Distance-weighted buried area of the query molecule.
-        _core.py:946-1004."""
+        """Distance-weighted buried area of the query molecule on the CMS-mode
+        surface.  _core.py:791-797, 946-1004."""
         self.generate(False)
+        return self.contact_ms_of_surface(target_side)
+
+    def contact_ms_of_surface(self, target_side=True):
+        """The CMS readout on whatever surface was generated (no mode check,
+        as the reference's calc_contact_molecular_surface).  _core.py:946-1004."""
         n1 = self._natoms(1)
         n0 = len(self.mol) - n1
         if target_side:
@@ -813,6 +818,11 @@ class OracleCalculator:
         """Total and per-atom area of molecule 0's whole surface.
         _core.py:799-805, 1007-1034."""
         self.generate(True)
+        return self.max_contact_ms_of_surface()
+
+    def max_contact_ms_of_surface(self):
+        """Area readout of molecule 0's dots on whatever surface was
+        generated.  _core.py:1007-1034."""
         n0 = self._natoms(0)
         d = self.dots[0]
         if len(d) == 0:

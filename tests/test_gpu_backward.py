@@ -176,3 +176,18 @@ def test_gradient_matches_torch_fp64_reference(complexes, which):
     say("  max |grad| %.3e, max abs diff / max |grad| %.2e" % (scale, err))
     assert scale > 0 and err < 1e-9
     st.close()
+
+
+def test_autograd_retain_graph(complexes):
+    """A graph kept with retain_graph=True is differentiated twice with the
+    same result (to rounding: per-atom gradients are accumulated with FP64
+    atomics, whose order varies between calls)."""
+    import torch
+    from cms_cuda.autograd import contact_ms_pair
+    bx, br, tx, tr = complexes[1]
+    b = torch.from_numpy(bx).cuda().requires_grad_(True)
+    t = torch.from_numpy(tx).cuda().requires_grad_(True)
+    cms = contact_ms_pair(b, torch.from_numpy(br).cuda(), t, torch.from_numpy(tr).cuda())
+    (g1,) = torch.autograd.grad(cms, b, retain_graph=True)
+    (g2,) = torch.autograd.grad(cms, b)
+    assert torch.allclose(g1, g2, rtol=1e-12, atol=1e-15 * float(g1.abs().max())) and g1.abs().sum() > 0

@@ -254,3 +254,23 @@ def test_large_batch_kernels_match_small(golden_complex):
     merged = {key: np.concatenate([p[key] for p in parts]) for key in ("target", "binder", "per_atom", "sc")}
     assert _same(big, merged)
     assert (big["target"] > 0).sum() > 60
+
+
+def test_trim_lanes_agree(golden_complex, monkeypatch):
+    """The SC trim reads each buried dot's candidate cells with 1, 4 or 32
+    lanes (chosen by batch size, overridable with CMS_TRIM_LANES): the same
+    trimmed dots and SC records on a mid-size batch."""
+    from cms_cuda.batch import BatchCalculator
+    cs = _batch(golden_complex, [(1800, 400), (900, 120), (3000, 900), (600, 60)], 1400)
+    outs = []
+    for lanes in ("1", "4", "32"):
+        monkeypatch.setenv("CMS_TRIM_LANES", lanes)
+        say("SC trim with %s lanes per buried dot" % lanes)
+        bc = BatchCalculator()
+        bc.load(cs)
+        outs.append(bc.run(0))
+        bc.close()
+    a = outs[0]
+    assert (a["sc"][:, 13] + a["sc"][:, 21]).sum() > 500
+    for b in outs[1:]:
+        assert _same(a, b)
