@@ -29,8 +29,9 @@ def gpu_available():
 def _check_device(device):
     d = "auto" if device is None else str(device).lower()
     if d == "cpu":
-        raise RuntimeError("device='cpu' is not provided by this MI355X build; use the reference's "
-                           "NumPy calculator (cms_cuda._core) for CPU runs")
+        raise RuntimeError("device='cpu' is not provided by this MI355X build (it has no CPU calculator): "
+                           "use device='auto' or 'cuda' here, or the NumPy calculator of the original "
+                           "cms-cuda package for CPU runs")
     if d not in ("auto", "cuda", "gpu"):
         raise ValueError("device must be 'auto', 'cpu', 'cuda' or 'gpu', got %r" % (device,))
     if not gpu_available():
