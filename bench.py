@@ -105,10 +105,11 @@ def make_workload(name, batch, seed):
 class Leg:
     """One workload resident on the GPU behind one context."""
 
-    def __init__(self, cs, precision, graph, grad=False):
+    def __init__(self, cs, precision, graph, grad=False, tuned=True):
         import torch
-        from cms_cuda import _lib
+        from cms_cuda import _lib, tuning
         from cms_cuda._calc import Settings
+        self.tuning = tuning
         self.torch = torch
         self._lib = _lib
         dev = torch.device("cuda", 0)
@@ -138,6 +139,22 @@ class Leg:
         self.prec = precision
         self.graph = 1 if graph else 0
         self.ctx("cms_set_graph", self.graph)
+        self.bucket = tuning.bucket(self.ntot)
+        self.choice = tuning.choice(self.ntot) if tuned else tuning.default()
+        tuning.apply(self.ctx, self.choice)
+
+    def timed_default(self, steps, warmup):
+        """The same timing with the library's built-in launch-shape rules."""
+        self.tuning.apply(self.ctx, self.tuning.default())
+        try:
+            return self.timed(steps, warmup)
+        finally:
+            self.tuning.apply(self.ctx, self.choice)
+
+    def tuning_record(self, value, steps, warmup):
+        dv, dms = self.timed_default(steps, warmup)
+        return {"bucket": self.bucket, "choice": self.choice, "value_default_rules": dv,
+                "ms_per_step_default_rules": dms, "tuned_over_default": value / dv}
 
     @staticmethod
     def P(t):
@@ -573,6 +590,8 @@ def main():
     log("timing %d steps (graph %s)" % (args.steps, "on" if graph else "off"))
     value, ms_per_step = main_leg.timed(args.steps, 2)
     log("%.1f complexes/s, %.3f ms/step" % (value, ms_per_step))
+    main_tuning = main_leg.tuning_record(value, args.steps, 2)
+    log("tuning %s: %s" % (main_leg.bucket, json.dumps(main_tuning)))
     gstats = main_leg.stats()
     tot, per, sc = main_leg.results()
     counts = main_leg.counts()
@@ -665,7 +684,7 @@ def main():
         leg = Leg(cs[:2048], _lib.PREC_FP64, graph)
         v3, ms3 = leg.timed(args.steps, args.warmup)
         c3 = {"workload": WORKLOAD_TEXT["c3"], "value": v3, "unit": "complexes/s", "ms_per_step": ms3,
-              "precision": "fp64", "graph": graph}
+              "precision": "fp64", "graph": graph, "tuning": leg.tuning_record(v3, args.steps, 2)}
         leg.close()
         torch.cuda.empty_cache()
         log("c3: %.1f complexes/s" % v3)
@@ -692,6 +711,7 @@ def main():
         gcs = make_workload("c5", 0, args.seed)
         gl = Leg(gcs, _lib.PREC_FP32_RESCORE, graph, grad=True)
         gv, gms = gl.timed(args.steps, args.warmup)
+        g_tuning = gl.tuning_record(gv, args.steps, 2)
         bd = gl.breakdown()
         gl.close()
         log("with grad: %.1f complexes/s; finite-difference check" % gv)
@@ -699,7 +719,7 @@ def main():
         with_grad = {"workload": WORKLOAD_TEXT["c5"], "value": gv, "unit": "complexes/s", "ms_per_step": gms,
                      "precision": "fp32-search+f64",
                      "backward_us": round(bd.get("cms_adjoint", 0) + bd.get("dot_grad", 0), 1),
-                     "fd_check": fd}
+                     "fd_check": fd, "tuning": g_tuning}
         log("fd check: %s" % json.dumps(fd))
 
     cpu = None
@@ -757,6 +777,7 @@ def main():
         "kernel_us_per_step": breakdown,
         "roofline_by_kernel": by_kernel,
         "graph": gstats,
+        "tuning": main_tuning,
         "counts_total": counts.sum(axis=0).tolist(),
     }
     print(json.dumps(out), flush=True)

@@ -16,7 +16,7 @@ import weakref
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tuning
 
 ATTEN_BLOCKER = 1
 ATTEN_2 = 2
@@ -340,6 +340,7 @@ class MolecularSurfaceCalculator:
         nb = (ctypes.c_int32 * 1)(n0)
         nt = (ctypes.c_int32 * 1)(n1)
         s = self.settings.to_c()
+        tuning.apply(self._ctx, tuning.choice(n0 + n1))
         self._ctx("cms_load", 1, nb, nt, _ptr(xyz), _ptr(rad), _ptr(dens), ctypes.byref(s))
         try:
             self._ctx("cms_generate", 1 if all_atoms else 0, 0)

@@ -27,6 +27,11 @@ NCOUNTS = 15
 SC_NRES = 24
 
 
+class cms_tuning(ctypes.Structure):
+    _fields_ = [("trim_lanes", ctypes.c_int), ("probe_count", ctypes.c_int), ("probe_slots", ctypes.c_int),
+                ("cell_sweep", ctypes.c_int), ("search_split", ctypes.c_int)]
+
+
 class cms_settings(ctypes.Structure):
     _fields_ = [("rp", ctypes.c_double), ("density", ctypes.c_double), ("band", ctypes.c_double),
                 ("sep", ctypes.c_double), ("weight", ctypes.c_double),
@@ -47,6 +52,8 @@ _PROTOS = {
     "cms_generate": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "cms_status": (_c.c_int, [_vp, _vp]),
     "cms_failed_count": (_c.c_int, [_vp, _c.POINTER(_i32)]),
+    "cms_set_tuning": (_c.c_int, [_vp, _vp]),
+    "cms_get_tuning": (_c.c_int, [_vp, _vp]),
     "cms_coincident": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_i32), _c.POINTER(_i32)]),
     "cms_counts": (_c.c_int, [_vp, _vp]),
     "cms_contact_ms": (_c.c_int, [_vp, _c.c_int, _vp, _vp]),

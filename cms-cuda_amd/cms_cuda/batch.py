@@ -20,12 +20,14 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tuning
 from ._calc import Settings, _device, _ptr
 
 
 class BatchCalculator:
-    def __init__(self, settings=None, device=None, stream=None):
+    def __init__(self, settings=None, device=None, stream=None, tuned=True):
+        """tuned: launch shapes from the tuning table for the loaded batch's
+        size (cms_cuda.tuning); False keeps the library's built-in rules."""
         import torch
         self.settings = settings or Settings()
         self.dev = device or _device()
@@ -33,6 +35,7 @@ class BatchCalculator:
         self.ctx = _lib.Context(self.dev.index, self.stream.cuda_stream)
         self.B = 0
         self.mode = None
+        self.tuned = tuned
 
     def close(self):
         if self.ctx is not None:
@@ -69,6 +72,7 @@ class BatchCalculator:
         self._nb_c = (ctypes.c_int32 * self.B)(*self.nb.tolist())
         self._nt_c = (ctypes.c_int32 * self.B)(*self.nt.tolist())
         self._st = self.settings.to_c()
+        tuning.apply(self.ctx, tuning.choice(self.ntot) if self.tuned else tuning.default())
         self.ctx("cms_load", self.B, self._nb_c, self._nt_c, _ptr(d_xyz), _ptr(d_rad), _ptr(d_dens),
                  ctypes.byref(self._st))
         self.mode = None

@@ -128,6 +128,8 @@ struct cms_ctx {
   int64_t n_captures = 0, n_replays = 0, n_exact = 0, n_overflow = 0;
   int n_failed = 0;                 // complexes of the last run with a non-zero status word
   int trim_lanes = 0;               // lanes per buried dot in the SC trim (0: by batch size)
+  int cell_sweep = 0;               // cell sweeps: 0 by batch size, 1 wavefront per atom, 2 thread per atom
+  int search_split = 0;             // nearest search: work items the reference splits aim for (0: 2048)
   // kernel timing with HIP events on the context's stream
   std::string prof_filter;  // "" = off, "*" = every instrumented kernel
   struct Ev { std::string name; hipEvent_t a, b; };
@@ -731,7 +733,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     // in the sweeps (parallelism); large batches: cell lists workgroup per
     // complex (LDS counters) and one thread per cell-sorted atom in the
     // sweeps (cell data shared within a wavefront)
-    const bool per_thread = n >= 200000;
+    const bool per_thread = ctx->cell_sweep ? ctx->cell_sweep == 2 : n >= 200000;
     if (per_thread)
       k_cell_count<<<grid_for(B, 1, 65536), 512, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
     else
@@ -952,6 +954,30 @@ int cms_status(cms_ctx* ctx, int32_t* status_host) {
   return CMS_OK;
 }
 
+int cms_set_tuning(cms_ctx* ctx, const cms_tuning* t) {
+  if (!ctx || !t) return CMS_ERR_INVALID;
+  if (!(t->trim_lanes == 0 || t->trim_lanes == 1 || t->trim_lanes == 4 || t->trim_lanes == 32) ||
+      t->probe_count < 0 || t->probe_count > 2 || t->probe_slots < 0 || t->probe_slots > PSLOTS ||
+      t->cell_sweep < 0 || t->cell_sweep > 2 || t->search_split < 0 || t->search_split > 65536)
+    return fail(ctx, CMS_ERR_INVALID, "cms_set_tuning: value out of range");
+  ctx->trim_lanes = t->trim_lanes;
+  ctx->probe_count_mode = t->probe_count;
+  ctx->probe_slots = t->probe_slots ? t->probe_slots : PSLOTS;
+  ctx->cell_sweep = t->cell_sweep;
+  ctx->search_split = t->search_split;
+  return CMS_OK;
+}
+
+int cms_get_tuning(cms_ctx* ctx, cms_tuning* t) {
+  if (!ctx || !t) return CMS_ERR_INVALID;
+  t->trim_lanes = ctx->trim_lanes;
+  t->probe_count = ctx->probe_count_mode;
+  t->probe_slots = ctx->probe_slots;
+  t->cell_sweep = ctx->cell_sweep;
+  t->search_split = ctx->search_split;
+  return CMS_OK;
+}
+
 int cms_failed_count(cms_ctx* ctx, int32_t* n_failed) {
   if (!ctx || !n_failed) return CMS_ERR_INVALID;
   *n_failed = ctx->n_failed;
@@ -1019,7 +1045,7 @@ static int launch_nearest(cms_ctx* ctx, int precision, const double4* Q, const f
   k_tile_counts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, qgrp, tq, ctx->tcnt.as<int>());
   scan_excl(ctx->tcnt.as<int>(), ctx->toff.as<int>(), G, ctx->part.as<int>(), st);
   const int64_t tiles_est = nq_hint / tq + G;
-  const int64_t target = 2048;
+  const int64_t target = ctx->search_split ? ctx->search_split : 2048;
   int nsplit = 1;
   if (tiles_est < target) {
     nsplit = (int)std::min<int64_t>((target + tiles_est - 1) / tiles_est, 64);
@@ -1288,6 +1314,8 @@ static std::vector<uint64_t> run_key(cms_ctx* ctx, int precision, double* per_at
                              (uint64_t)(uintptr_t)per_atom, (uint64_t)(uintptr_t)totals,
                              (uint64_t)(uintptr_t)sc,  ctx->layout_epoch,       workspace_signature(ctx)};
   for (int i = M_R; i <= M_T; ++i) k.push_back((uint64_t)ctx->est[i]);
+  for (int v : {ctx->trim_lanes, ctx->probe_count_mode, ctx->probe_slots, ctx->cell_sweep, ctx->search_split})
+    k.push_back((uint64_t)(int64_t)v);
   const double sv[6] = {ctx->s.rp, ctx->s.band, ctx->s.sep, ctx->s.weight, ctx->s.binwidth_dist,
                         ctx->s.binwidth_norm};
   for (double v : sv) {

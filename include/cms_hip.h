@@ -70,6 +70,24 @@ typedef struct cms_settings {
 
 typedef struct cms_ctx cms_ctx;
 
+/* Launch-shape choices of a context (0 in a field = the built-in rule by
+   batch size).  Every choice gives bit-identical results; they only change
+   speed.  trim_lanes: lanes per buried dot in the SC trim (1, 4 or 32);
+   probe_count: probe / torus-point count pass per pair (1) or per
+   neighbour row (2); probe_slots: surviving probes staged per row by the
+   per-row pass (1..8); cell_sweep: attention and region sweeps with a
+   wavefront per atom (1) or a thread per cell-sorted atom (2);
+   search_split: work items the nearest-dot searches split their reference
+   ranges to fill (1..65536).  No reference counterpart (the reference
+   fixes its launch shapes, cms-cuda/_gpu_kernels.py:34, 504-505). */
+typedef struct cms_tuning {
+  int trim_lanes;
+  int probe_count;
+  int probe_slots;
+  int cell_sweep;
+  int search_split;
+} cms_tuning;
+
 /* Library version (major*10000 + minor*100 + patch). */
 int cms_version(void);
 
@@ -114,6 +132,11 @@ int cms_generate(cms_ctx* ctx, int all_atoms, int full_state);
 /* Per-complex status word after cms_generate (bit 1 coincident atoms, 2 / 4
    neighbour / buried row over capacity, 8 near-probe list over capacity). */
 int cms_status(cms_ctx* ctx, int32_t* status_host);
+
+/* Set / read the context's launch-shape choices (a captured graph is
+   re-captured after a change). */
+int cms_set_tuning(cms_ctx* ctx, const cms_tuning* tuning);
+int cms_get_tuning(cms_ctx* ctx, cms_tuning* tuning);
 
 /* Number of complexes of the last cms_generate / cms_run whose status word
    is non-zero (their results are zero).  No host wait. */
