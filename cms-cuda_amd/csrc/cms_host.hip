@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
+  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -254,6 +254,9 @@ static Work work(cms_ctx* ctx) {
   w.t2scan = ctx->t2scan.as<int>();
   w.tflag = ctx->tflag.as<int8_t>();
   w.tmask = ctx->tmask.as<uint64_t>();
+  w.trel = ctx->trel.as<uint64_t>();
+  w.rrel = ctx->rrel.as<uint64_t>();
+  w.rtor = ctx->rtor.as<int8_t>();
   w.kcnt = ctx->kcnt.as<int>();
   w.kpos = ctx->kpos.as<double4>();
   w.kinfo = ctx->kinfo.as<uint64_t>();
@@ -282,7 +285,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
                   &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -521,6 +524,8 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->cmask.ensure(sizeof(uint64_t) * (c * CMASK_WORDS + 1)));
   CK(ctx->pstage.ensure(sizeof(ProbeRec) * PSLOTS * (c + 1)));
   CK(ctx->nps.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->rrel.ensure(sizeof(uint64_t) * (KCAP / WAVE) * (c + 1)));
+  CK(ctx->rtor.ensure(c + 1));
   ctx->cap[M_R] = std::max(ctx->cap[M_R], c);
   return CMS_OK;
 }
@@ -539,6 +544,7 @@ static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->t2scan.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tflag.ensure(c + 1));
   CK(ctx->tmask.ensure(sizeof(uint64_t) * 2 * (c + 1)));
+  CK(ctx->trel.ensure(sizeof(uint64_t) * (KCAP / WAVE) * (c + 1)));
   CK(ctx->tef.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tes.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tel.ensure(sizeof(int) * (c + 1)));
@@ -825,7 +831,19 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   if (est[M_P] > 0) {
     if (by_row) {
       fill_int(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0, st);
-      TIMED("probes_count", k_probe_rows<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+      fill_int8(ctx->rtor.as<int8_t>(), ctx->cap[M_R], 0, st);
+      // the torus pass writes the toroidal counts of the pairs of rows with
+      // a live toroid-queue pair only
+      fill_int(ctx->t1cnt.as<int>(), cP, ctx->cap[M_P], 0, st);
+      fill_int(ctx->t2cnt.as<int>(), cP, ctx->cap[M_P], 0, st);
+      fill_int8(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0, st);
+      // rows of at most RC_SMALL neighbours, then the longer ones (LDS staging
+      // sized for each); then the torus points of the live toroid-queue
+      // pairs with the reach masks of the pass before
+      TIMED("probes_count", (k_probe_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
+      TIMED("probes_count_long", (k_probe_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
+      TIMED("torus_count", (k_torus_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
+      TIMED("torus_count_long", (k_torus_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
     }
     else
       TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));

@@ -134,6 +134,9 @@ struct Work {
   int* t2scan;
   int8_t* tflag;
   uint64_t* tmask;        // count pass: free torus points of each pair (two words)
+  uint64_t* trel;         // per-row pass: row slots within reach of each toroid pair's circle (4 words)
+  uint64_t* rrel;         // per-row pass: per row, the union of its live toroid pairs' reach masks (4 words)
+  int8_t* rtor;           // per-row pass: the row has a live toroid pair
   int* kcnt;
   double4* kpos;          // KSLOTS staged concave dots per probe
   uint64_t* kinfo;
@@ -156,8 +159,11 @@ struct Work {
 
 __global__ void k_pairs(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
-__global__ void k_probe_rows(BatchView, Settings, Work);
+// LDS row capacity of the common instantiation of the per-row kernels
+constexpr int RC_SMALL = 128;
+template <int RC> __global__ void k_probe_rows(BatchView, Settings, Work);
 __global__ void k_probe_place(BatchView, Work);
+template <int RC> __global__ void k_torus_rows(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
 __global__ void k_toroid_count(BatchView, Settings, Work);
 __global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);

@@ -433,8 +433,15 @@ __device__ __forceinline__ void torus_point_arcs(V3 pm, V3 xi, V3 xj, double eri
 // atoms, whether any torus point is free and the free points among the
 // first 64 (_core.py:1847-2044).
 constexpr int PR_PAIRS = 32;            // pairs per batch (one per lane in the per-pair steps)
-constexpr int PR_QCAP = 896;            // triangle-test items per batch, bounds the triple queue
-constexpr int PR_RELW = KCAP / WAVE;    // reach mask words per pair
+// Row capacity of the LDS staging (RC_SMALL, cms_internal.h): rows of at
+// most RC_SMALL neighbours run in the RC_SMALL instantiation of the per-row
+// kernels (half the LDS, so more workgroups per CU), longer rows (up to
+// KCAP) in the KCAP one; each launch visits only its own rows.
+constexpr int TRELW = KCAP / WAVE;      // reach mask words per pair in HBM (trel)
+template <int RC>
+__device__ __forceinline__ bool row_in_class(int nn) {
+  return RC < KCAP ? nn <= RC : nn > RC_SMALL;
+}
 // Reach margin: a computed probe of a non-near-degenerate triple lies within
 // 1e-4 A of its pair's torus circle (see circle_in_one_sphere), and the
 // squared-distance forms below round far below 1e-10 A.
@@ -447,28 +454,28 @@ constexpr int LATT_ATT = 0x3f, LATT_IK = 0x40;
 constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32,
               PB_BETWEEN = 64, PB_ARC2 = 128, PB_EMIT = 256;
 
-__global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work w) {
-  __shared__ int lj[KCAP];
-  __shared__ double4 lx[KCAP];
-  __shared__ int8_t latt[KCAP];
-  __shared__ uint8_t fws[KCAP];   // forward slots (atom index > i) by ascending atom index
-  __shared__ uint8_t frk[KCAP];   // forward rank of a forward slot
-  __shared__ uint8_t pslots[KCAP];   // row slots of the row's pairs, in pair order
+// Three wavefronts per SIMD (at most 168 VGPRs): the kernel waits on LDS
+// and global loads between dependent steps, so occupancy is what hides them.
+template <int RC>
+__global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, Work w) {
+  constexpr int PR_RELW = RC / WAVE;      // reach mask words per pair
+  constexpr int PR_QCAP = RC * 7 / 2;     // triangle-test items per batch, bounds the triple queue
+  __shared__ int lj[RC];
+  __shared__ double4 lx[RC];
+  __shared__ int8_t latt[RC];
+  __shared__ uint8_t fws[RC];   // forward slots (atom index > i) by ascending atom index
+  __shared__ uint8_t frk[RC];   // forward rank of a forward slot
+  __shared__ uint8_t pslots[RC];   // row slots of the row's pairs, in pair order
   __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
-  // cpair (step 3) and tpre (step 5) share their words
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
-  int* const tpre = cpair;
   __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
+  __shared__ uint64_t rrel_s[PR_RELW];   // union of the reach masks of the row's live toroid-queue pairs
+  __shared__ int rtor_s;                 // the row has a live toroid-queue pair
   __shared__ uint16_t queue[PR_QCAP];
-  // toroid-queue pairs: torus circle frame, spacings, point prefix, results
-  __shared__ double4 tcx[PR_PAIRS], tcy[PR_PAIRS];
-  __shared__ double bdens[PR_PAIRS], ttd[PR_PAIRS], tad[PR_PAIRS];
-  __shared__ int tn1[PR_PAIRS], tn2[PR_PAIRS];
-  __shared__ unsigned long long tmk0[PR_PAIRS], tmk1[PR_PAIRS];
-  static_assert(KCAP <= 256 && PR_PAIRS <= 32, "queue entries pack (pair << 8) | slot in 16 bits");
+  static_assert(RC <= 256 && PR_PAIRS <= 32 && RC <= PR_QCAP, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
-  auto live = [&](int64_t q) { return w.pcnt[q] != 0; };
+  auto live = [&](int64_t q) { return w.pcnt[q] != 0 && row_in_class<RC>(w.nn[q]); };
   CMS_SPARSE_LOOP(r, nR, live) {
     const int i = w.ratom[r];
     const int nn = w.nn[r];
@@ -510,8 +517,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
     const int grp = group_of_atom(b, i);
     const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
     const int8_t iatt = w.atten[i];
-    const bool iblock = iatt <= ATTEN_BLOCKER, iarc = iatt >= ATTEN_2;
-    const double idens = b.dens[i];
+    const bool iblock = iatt <= ATTEN_BLOCKER;
     const int nw = (nn + WAVE - 1) / WAVE;
     // the row's pairs as the pair pass numbers them: relevant forward
     // neighbours in row order (the pass's count for the row is p_hi - p_lo)
@@ -534,6 +540,8 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         np += __popcll(m);
       }
     }
+    if (lane() < PR_RELW) rrel_s[lane()] = 0ull;
+    if (lane() == 0) rtor_s = 0;
     __syncthreads();
     int prow = 0;   // surviving candidates offered to the row's probe slots
     for (int pb = p_lo; pb < p_hi;) {
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         cnt = F - first;
       }
       const int inc = wave_incl_scan(cnt);
-      const int nb = __popcll(ballot(lane() < avail && inc <= PR_QCAP));   // >= 1: cnt < KCAP <= PR_QCAP
+      const int nb = __popcll(ballot(lane() < avail && inc <= PR_QCAP));   // >= 1: cnt < RC <= PR_QCAP
       const int items = __shfl(inc, nb - 1, WAVE);
       if (lane() < nb) {
         const double4 aj = lx[slot];
@@ -561,7 +569,6 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
         bu[lane()] = make_double4(u.x, u.y, u.z, rij);
         bt[lane()] = make_double4(t.x, t.y, t.z, 0.0);
         bx[lane()] = aj;
-        bdens[lane()] = b.dens[j];
         const int8_t jatt = (int8_t)(latt[slot] & LATT_ATT);
         const bool bij = iblock && jatt <= ATTEN_BLOCKER;
         bflag[lane()] = (pair_toroid(iatt, jatt, s.rp) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0) |
@@ -740,72 +747,6 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
       }
       __syncthreads();
 
-      // 5. torus points (_core.py:1863-1940) and saddle arcs of the free ones
-      {
-        int M = 0;
-        if (lane() < nb) {
-          const int f = bflag[lane()];
-          if ((f & PB_TOR) && !(f & PB_COVERED)) {
-            const double4 pu = bu[lane()];
-            const TorusFrame tf = torus_frame(ai.w, bx[lane()].w, s.rp, pu.w, ld3(pu),
-                                              (idens + bdens[lane()]) * 0.5);
-            tcx[lane()] = make_double4(tf.cx.x, tf.cx.y, tf.cx.z, 0.0);
-            tcy[lane()] = make_double4(tf.cy.x, tf.cy.y, tf.cy.z, 0.0);
-            ttd[lane()] = tf.tdelta;
-            tad[lane()] = tf.adelta;
-            M = tf.M;
-          }
-          tn1[lane()] = 0;
-          tn2[lane()] = 0;
-          tmk0[lane()] = 0ull;
-          tmk1[lane()] = 0ull;
-        }
-        const int incM = wave_incl_scan(M);
-        if (lane() < nb) tpre[lane()] = incM - M;
-        const int T = __shfl(incM, WAVE - 1, WAVE);
-        __syncthreads();
-        for (int it0 = 0; it0 < T; it0 += WAVE) {
-          const int it = it0 + lane();
-          if (it < T) {
-            int a = 0;
-#pragma unroll
-            for (int step = PR_PAIRS / 2; step > 0; step >>= 1)
-              if (a + step < nb && tpre[a + step] <= it) a += step;
-            const int m = it - tpre[a];
-            const double4 pu = bu[a], pt = bt[a], aj = bx[a];
-            const V3 uij = ld3(pu), tij = ld3(pt);
-            const double rij = pu.w;
-            const V3 pm = subdiv_point(tij, ld3(tcx[a]), ld3(tcy[a]), rij, ttd[a], m);
-            // collisions with the row atoms (other than j) within reach of
-            // the circle (strict <, _core.py:1905-1917)
-            bool live = true;
-            for (int wd = 0; wd < nw && live; ++wd) {
-              uint64_t rm = brel[a][wd];
-              while (rm) {
-                const int t = wd * WAVE + __builtin_ctzll(rm);
-                rm &= rm - 1;
-                const double4 ax = lx[t];
-                const V3 d = pm - ld3(ax);
-                const double lim = ax.w + s.rp;
-                if (dot_alt(d, d) < lim * lim) { live = false; break; }
-              }
-            }
-            if (live) {
-              const int f = bflag[a];
-              TorusSub ts_{};
-              ts_.n1 = ts_.n2 = 0;
-              torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, tij, uij, rij, s.rp, (f & PB_BETWEEN) != 0, iarc,
-                               (f & PB_ARC2) != 0, tad[a], &ts_);
-              if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
-              if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
-              if (m < WAVE) atomicOr(&tmk0[a], 1ull << m);
-              else atomicOr(&tmk1[a], 1ull << (m - WAVE));
-            }
-          }
-        }
-      }
-      __syncthreads();
-
       // 6. per-pair results
       if (lane() < nb) {
         const int p = pb + lane();
@@ -822,27 +763,167 @@ __global__ __launch_bounds__(64) void k_probe_rows(BatchView b, Settings s, Work
           if (np + nm > 0 && (f & PB_EMIT)) pf |= 8;
         }
         // the pair record (what the pair pass's emit would write); the
-        // emit passes read the geometry only of pairs with probes or
-        // toroidal dots
+        // emit passes read the geometry only of pairs with probes, the
+        // torus pass that of the live toroid-queue pairs, with their reach
+        // masks
+        const bool tor_live = (f & PB_TOR) && !(f & PB_COVERED);
+        if (f & PB_ARC2) pf |= 16;
         w.pi[p] = i;
         w.pj[p] = bj[lane()];
         w.pflag[p] = (int8_t)pf;
-        if (np + nm + tn1[lane()] + tn2[lane()] > 0) {
+        if (np + nm > 0 || tor_live) {
           w.pu[p] = bu[lane()];
           w.pt[p] = bt[lane()];
         }
+        if (tor_live) {
+          for (int wd = 0; wd < TRELW; ++wd) {
+            const uint64_t m = wd < PR_RELW ? brel[lane()][wd] : 0ull;
+            w.trel[TRELW * (int64_t)p + wd] = m;
+            if (m) atomicOr((unsigned long long*)&rrel_s[wd], (unsigned long long)m);
+          }
+          rtor_s = 1;
+        }
         w.prcnt[vp] = np;
         w.prcnt[vm] = nm;
+      }
+      pb += nb;
+      __syncthreads();
+    }
+    if (lane() == 0) {
+      w.nps[r] = min(prow, w.pslots);
+      w.rtor[r] = (int8_t)rtor_s;
+    }
+    if (lane() < TRELW) w.rrel[TRELW * (int64_t)r + lane()] = lane() < PR_RELW ? rrel_s[lane()] : 0ull;
+    __syncthreads();
+  }
+}
+
+// Torus points of the toroid-queue pairs (_core.py:1863-1940) after
+// k_probe_rows, one wavefront per neighbour row: the row's coordinates are
+// staged once, its pairs are taken in batches of TR_PAIRS, and the batch's
+// torus points are spread over the lanes across pair boundaries.  Each
+// point is tested against the row atoms that k_probe_rows found within
+// reach of its pair's circle (strict <, _core.py:1905-1917); the free ones
+// get their saddle-arc dot counts (_core.py:1947-2044).  k_probe_rows left
+// each live pair's reach mask (trel), per row their union (rrel) and
+// whether the row has a live pair (rtor).  Writes, for every pair of such a row, the arc dot counts of
+// both atoms, whether any torus point is free and the free points among the
+// first 128; the host zero-fills them for the other pairs.
+constexpr int TR_PAIRS = 32;
+
+template <int RC>
+__global__ __launch_bounds__(64, 3) void k_torus_rows(BatchView b, Settings s, Work w) {
+  constexpr int PR_RELW = RC / WAVE;
+  __shared__ double4 lx[RC];
+  __shared__ double4 bu[TR_PAIRS], bt[TR_PAIRS], bx[TR_PAIRS], tcx[TR_PAIRS], tcy[TR_PAIRS];
+  __shared__ double ttd[TR_PAIRS], tad[TR_PAIRS];
+  __shared__ int bflag[TR_PAIRS], tpre[TR_PAIRS], tn1[TR_PAIRS], tn2[TR_PAIRS];
+  __shared__ unsigned long long tmk0[TR_PAIRS], tmk1[TR_PAIRS];
+  __shared__ uint64_t brel[TR_PAIRS][PR_RELW];
+  static_assert(MAX_SUBDIV <= 2 * WAVE, "two mask words per pair");
+  const int nR = *w.nR;
+  const int nP = *w.nP;
+  // rows with a live toroid-queue pair (toroid queue, circle not inside a
+  // neighbour sphere), as k_probe_rows marked them
+  auto live = [&](int64_t q) { return w.rtor[q] != 0 && row_in_class<RC>(w.nn[q]); };
+  CMS_SPARSE_LOOP(r, nR, live) {
+    const int i = w.ratom[r];
+    const int nn = w.nn[r];
+    const int p_lo = w.pscan[r];
+    const int p_hi = min(p_lo + w.pcnt[r], nP);
+    if (p_lo >= p_hi || nn <= 0) continue;
+    const double4 ai = b.atom[i];
+    const V3 xi = ld3(ai);
+    const double eri = ai.w + s.rp;
+    const bool iarc = w.atten[i] >= ATTEN_2;
+    const double idens = b.dens[i];
+    const int nw = (nn + WAVE - 1) / WAVE;
+    {
+      // only the row atoms within reach of a live pair's circle
+      const int* row = w.nbr + (int64_t)r * KCAP;
+      for (int s0 = 0; s0 < nn; s0 += WAVE) {
+        const int sl = s0 + lane();
+        if (sl < nn && ((w.rrel[TRELW * (int64_t)r + s0 / WAVE] >> lane()) & 1ull)) lx[sl] = b.atom[row[sl]];
+      }
+    }
+    for (int pb = p_lo; pb < p_hi; pb += TR_PAIRS) {
+      const int nb = min(TR_PAIRS, p_hi - pb);
+      const int p = pb + lane();
+      const int pfl = lane() < nb ? w.pflag[p] : 0;
+      const bool tl = (pfl & 6) == 2;
+      int M = 0;
+      if (lane() < nb) {
+        tn1[lane()] = 0;
+        tn2[lane()] = 0;
+        tmk0[lane()] = 0ull;
+        tmk1[lane()] = 0ull;
+      }
+      if (tl) {
+        const int j = w.pj[p];
+        const double4 pu = w.pu[p], aj = b.atom[j];
+        bu[lane()] = pu;
+        bt[lane()] = w.pt[p];
+        bx[lane()] = aj;
+        bflag[lane()] = ((pfl & 1) ? PB_BETWEEN : 0) | ((pfl & 16) ? PB_ARC2 : 0);
+        for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = w.trel[TRELW * (int64_t)p + wd];
+        const TorusFrame tf = torus_frame(ai.w, aj.w, s.rp, pu.w, ld3(pu), (idens + b.dens[j]) * 0.5);
+        tcx[lane()] = make_double4(tf.cx.x, tf.cx.y, tf.cx.z, 0.0);
+        tcy[lane()] = make_double4(tf.cy.x, tf.cy.y, tf.cy.z, 0.0);
+        ttd[lane()] = tf.tdelta;
+        tad[lane()] = tf.adelta;
+        M = tf.M;
+      }
+      const int incM = wave_incl_scan(M);
+      if (lane() < nb) tpre[lane()] = incM - M;
+      const int T = __shfl(incM, WAVE - 1, WAVE);
+      __syncthreads();
+      for (int it0 = 0; it0 < T; it0 += WAVE) {
+        const int it = it0 + lane();
+        if (it < T) {
+          int a = 0;
+#pragma unroll
+          for (int step = TR_PAIRS / 2; step > 0; step >>= 1)
+            if (a + step < nb && tpre[a + step] <= it) a += step;
+          const int m = it - tpre[a];
+          const double4 pu = bu[a], pt = bt[a], aj = bx[a];
+          const V3 uij = ld3(pu), tij = ld3(pt);
+          const double rij = pu.w;
+          const V3 pm = subdiv_point(tij, ld3(tcx[a]), ld3(tcy[a]), rij, ttd[a], m);
+          bool free_pt = true;
+          for (int wd = 0; wd < nw && free_pt; ++wd) {
+            uint64_t rm = brel[a][wd];
+            while (rm) {
+              const int t = wd * WAVE + __builtin_ctzll(rm);
+              rm &= rm - 1;
+              const double4 ax = lx[t];
+              const V3 d = pm - ld3(ax);
+              const double lim = ax.w + s.rp;
+              if (dot_alt(d, d) < lim * lim) { free_pt = false; break; }
+            }
+          }
+          if (free_pt) {
+            const int f = bflag[a];
+            TorusSub ts_{};
+            ts_.n1 = ts_.n2 = 0;
+            torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, tij, uij, rij, s.rp, (f & PB_BETWEEN) != 0, iarc,
+                             (f & PB_ARC2) != 0, tad[a], &ts_);
+            if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
+            if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
+            if (m < WAVE) atomicOr(&tmk0[a], 1ull << m);
+            else atomicOr(&tmk1[a], 1ull << (m - WAVE));
+          }
+        }
+      }
+      __syncthreads();
+      if (lane() < nb) {
         w.t1cnt[p] = tn1[lane()];
         w.t2cnt[p] = tn2[lane()];
         w.tflag[p] = (tmk0[lane()] | tmk1[lane()]) ? 1 : 0;
         w.tmask[2 * (int64_t)p] = tmk0[lane()];
         w.tmask[2 * (int64_t)p + 1] = tmk1[lane()];
       }
-      pb += nb;
       __syncthreads();
     }
-    if (lane() == 0) w.nps[r] = min(prow, w.pslots);
   }
 }
 
@@ -1604,6 +1685,10 @@ template __global__ void k_convex<false>(BatchView, Settings, Work);
 template __global__ void k_convex<true>(BatchView, Settings, Work);
 template __global__ void k_concave<false>(BatchView, Settings, Work);
 template __global__ void k_concave<true>(BatchView, Settings, Work);
+template __global__ void k_probe_rows<RC_SMALL>(BatchView, Settings, Work);
+template __global__ void k_probe_rows<KCAP>(BatchView, Settings, Work);
+template __global__ void k_torus_rows<RC_SMALL>(BatchView, Settings, Work);
+template __global__ void k_torus_rows<KCAP>(BatchView, Settings, Work);
 
 // Staged concave dots to their place in the dot arrays: one thread per
 // staging slot of probes with at most KSLOTS dots.
