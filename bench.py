@@ -253,11 +253,16 @@ def stage_models(counts, sc, nb, nt, fp32):
                       "34 B per row atom + 12 B per neighbour entry + 4 B per buried entry", False),
         "probes_count": ("hbm", float(np.sum(36.0 * En + 108.0 * Np)), "GB/s", PEAK_HBM_GBS,
                          "36 B per neighbour entry (index + coordinates, rows read once) + 108 B per torus "
-                         "pair (74 B record in, 34 B counts, flags and masks out)", False),
+                         "pair (74 B record in, 34 B counts, flags and reach masks out)", False),
         "trim": ("valu", float(np.sum(trim)), "TFLOP/s", PEAK_FP64_TFLOPS,
                  "8 flops per (buried, accessible) dot pair of a side (the dense search the cell-grid trim "
                  "replaces)", True),
     }
+    # torus points of the toroid-queue pairs: no per-item model; the executed
+    # flops come from the SQ counters
+    m["torus_count"] = ("valu", None, "TFLOP/s", PEAK_FP64_TFLOPS,
+                        "torus points, collisions and saddle-arc counts of the live toroid-queue pairs "
+                        "(executed flops from the SQ counters)", True)
     # two searches per step share the name; brute force, so the modelled
     # pairs are the pairs evaluated
     name = "nearest32" if fp32 else "nearest"
@@ -620,8 +625,9 @@ def main():
         if dense:
             # the cell sweeps do not execute the dense work; the executed
             # FP64 flops from the counters are the achieved figure
-            rec["dense_equivalent_per_launch"] = work
-            rec["dense_equivalent_tflops"] = work / avg_s / 1e12
+            if work is not None:
+                rec["dense_equivalent_per_launch"] = work
+                rec["dense_equivalent_tflops"] = work / avg_s / 1e12
             if vf:
                 rec["achieved"] = vf["executed_tflops"]
                 rec["frac"] = vf["executed_tflops"] / peak
