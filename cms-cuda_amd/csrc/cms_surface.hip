@@ -1527,23 +1527,23 @@ __global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Setti
     const V3 p = ld3(pq);
     const double h = pq.w;
     const V3 u = ld3(w.palt[q]);
-    const int aid[3] = {at.x, at.y, at.z};
+    // the three atoms in named registers: indexing arrays of them by the
+    // runtime owner / top choice would put them in scratch memory
     const double4 A0 = b.atom[at.x], A1 = b.atom[at.y], A2 = b.atom[at.z];
-    const V3 xa[3] = {ld3(A0), ld3(A1), ld3(A2)};
-    const double ra[3] = {A0.w, A1.w, A2.w};
     const double dens = ((b.dens[at.x] + b.dens[at.y]) + b.dens[at.z]) / 3.0;
-    V3 vp[3];
-    for (int a = 0; a < 3; ++a) vp[a] = unit(xa[a] - p);
-    const V3 vec0 = unit(cross(vp[0], vp[1]));
-    const V3 vec1 = unit(cross(vp[1], vp[2]));
-    const V3 vec2 = unit(cross(vp[2], vp[0]));
+    const V3 vp0 = unit(ld3(A0) - p), vp1 = unit(ld3(A1) - p), vp2 = unit(ld3(A2) - p);
+    const V3 vec0 = unit(cross(vp0, vp1));
+    const V3 vec1 = unit(cross(vp1, vp2));
+    const V3 vec2 = unit(cross(vp2, vp0));
     int mm = 0;
-    double best = dot_seq(u, vp[0]);
-    for (int a = 1; a < 3; ++a) {
-      const double d = dot_seq(u, vp[a]);
-      if (d > best) { best = d; mm = a; }
+    {
+      // the first of the largest (strict >), as in a loop over the atoms
+      double best = dot_seq(u, vp0);
+      const double d1 = dot_seq(u, vp1);
+      if (d1 > best) { best = d1; mm = 1; }
+      if (dot_seq(u, vp2) > best) mm = 2;
     }
-    const V3 top = vp[mm];
+    const V3 top = mm == 0 ? vp0 : (mm == 1 ? vp1 : vp2);
     const V3 south = neg(u);
     const V3 axis = unit(cross(top, south));
     V3 y;
@@ -1650,12 +1650,14 @@ __global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Setti
         const int k = total + lanes_below(mk);
         if (keep && (EMIT || k < KSLOTS)) {
           int mc = 0;
-          double dbest = norm_seq(pts - xa[0]) - ra[0];
-          for (int a = 1; a < 3; ++a) {
-            const double d = norm_seq(pts - xa[a]) - ra[a];
-            if (d < dbest) { dbest = d; mc = a; }
+          {
+            // the first of the closest (strict <)
+            double dbest = norm_seq(pts - ld3(A0)) - A0.w;
+            const double d1 = norm_seq(pts - ld3(A1)) - A1.w;
+            if (d1 < dbest) { dbest = d1; mc = 1; }
+            if (norm_seq(pts - ld3(A2)) - A2.w < dbest) mc = 2;
           }
-          const int owner = aid[mc];
+          const int owner = mc == 0 ? at.x : (mc == 1 ? at.y : at.z);
           const DotRec d = make_dot(b, s, w, pts, p, lat_ps[t] * cs, owner, w.rpos[owner], 3,
                                     make_int4(DK_CONCAVE | (mm << 4), at.x, at.y, at.z),
                                     make_int4(t, qq - lat_start[t], L, lat_start[t + 1] - lat_start[t]));
