@@ -382,9 +382,9 @@ __device__ void eval_dot(const BatchView& b, const Settings& s, const double4* _
 
 // Forward dots re-evaluated from their descriptors must equal the stored
 // ones bit for bit; counts mismatches and the largest deviation.
-__global__ void k_check_dots(BatchView b, Settings s, Count cnD, const double4* __restrict__ dpos,
-                             const int4* __restrict__ dd1, const int4* __restrict__ dd2,
-                             unsigned long long* __restrict__ nbad, double* __restrict__ maxdev) {
+__global__ __launch_bounds__(64) void k_check_dots(BatchView b, Settings s, Count cnD, const double4* __restrict__ dpos,
+                                                   const int4* __restrict__ dd1, const int4* __restrict__ dd2,
+                                                   unsigned long long* __restrict__ nbad, double* __restrict__ maxdev) {
   const int nD = cnD.get();
   double dev = 0.0;
   unsigned long long bad = 0;
@@ -465,9 +465,9 @@ __global__ __launch_bounds__(64) void k_dot_grad(BatchView b, Settings s, Count 
 
 // Frozen-topology CMS at other coordinates: positions and areas of every
 // buried dot re-evaluated, partners kept.
-__global__ void k_frozen_eval(BatchView b, Settings s, const double4* __restrict__ coords, Count cnB,
-                              const int* __restrict__ blist, const int4* __restrict__ dd1,
-                              const int4* __restrict__ dd2, double4* __restrict__ out) {
+__global__ __launch_bounds__(64) void k_frozen_eval(BatchView b, Settings s, const double4* __restrict__ coords, Count cnB,
+                                                    const int* __restrict__ blist, const int4* __restrict__ dd1,
+                                                    const int4* __restrict__ dd2, double4* __restrict__ out) {
   const int nB = cnB.get();
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nB; k += gridDim.x * blockDim.x) {
     const int d = blist[k];
