@@ -804,7 +804,8 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
 // torus points are spread over the lanes across pair boundaries.  Each
 // point is tested against the row atoms that k_probe_rows found within
 // reach of its pair's circle (strict <, _core.py:1905-1917); the free ones
-// get their saddle-arc dot counts (_core.py:1947-2044).  k_probe_rows left
+// are queued in LDS and get their saddle-arc dot counts (_core.py:1947-2044)
+// a wave of them at a time.  k_probe_rows left
 // each live pair's reach mask (trel), per row their union (rrel) and
 // whether the row has a live pair (rtor).  Writes, for every pair of such a row, the arc dot counts of
 // both atoms, whether any torus point is free and the free points among the
@@ -820,6 +821,7 @@ __global__ __launch_bounds__(64, 3) void k_torus_rows(BatchView b, Settings s, W
   __shared__ int bflag[TR_PAIRS], tpre[TR_PAIRS], tn1[TR_PAIRS], tn2[TR_PAIRS];
   __shared__ unsigned long long tmk0[TR_PAIRS], tmk1[TR_PAIRS];
   __shared__ uint64_t brel[TR_PAIRS][PR_RELW];
+  __shared__ uint16_t fq[2 * WAVE];   // queued free points: (pair << 8) | point
   static_assert(MAX_SUBDIV <= 2 * WAVE, "two mask words per pair");
   const int nR = *w.nR;
   const int nP = *w.nP;
@@ -877,8 +879,27 @@ __global__ __launch_bounds__(64, 3) void k_torus_rows(BatchView b, Settings s, W
       if (lane() < nb) tpre[lane()] = incM - M;
       const int T = __shfl(incM, WAVE - 1, WAVE);
       __syncthreads();
+      // the saddle arcs of the queued free points, one per lane (the queue
+      // keeps the arcs' heavier work off the lanes of the collision tests)
+      auto arcs = [&](int cnt) {
+        if (lane() < cnt) {
+          const int e = fq[lane()];
+          const int a = e >> 8;
+          const double4 pu = bu[a], aj = bx[a], pt = bt[a];
+          const int f = bflag[a];
+          const V3 pm = subdiv_point(ld3(pt), ld3(tcx[a]), ld3(tcy[a]), pu.w, ttd[a], e & 255);
+          TorusSub ts_{};
+          ts_.n1 = ts_.n2 = 0;
+          torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, ld3(pt), ld3(pu), pu.w, s.rp, (f & PB_BETWEEN) != 0,
+                           iarc, (f & PB_ARC2) != 0, tad[a], &ts_);
+          if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
+          if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
+        }
+      };
+      int nf = 0;   // queued free points
       for (int it0 = 0; it0 < T; it0 += WAVE) {
         const int it = it0 + lane();
+        int fe = -1;
         if (it < T) {
           int a = 0;
 #pragma unroll
@@ -902,18 +923,32 @@ __global__ __launch_bounds__(64, 3) void k_torus_rows(BatchView b, Settings s, W
             }
           }
           if (free_pt) {
-            const int f = bflag[a];
-            TorusSub ts_{};
-            ts_.n1 = ts_.n2 = 0;
-            torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, tij, uij, rij, s.rp, (f & PB_BETWEEN) != 0, iarc,
-                             (f & PB_ARC2) != 0, tad[a], &ts_);
-            if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
-            if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
             if (m < WAVE) atomicOr(&tmk0[a], 1ull << m);
             else atomicOr(&tmk1[a], 1ull << (m - WAVE));
+            fe = (a << 8) | m;
           }
         }
+        // free points join the queue; a full wave of them gets its arcs
+        const uint64_t fm = ballot(fe >= 0);
+        if (fe >= 0) {
+          const int o = nf + lanes_below(fm);
+          fq[o] = (uint16_t)fe;
+        }
+        nf += __popcll(fm);
+        if (nf >= WAVE) {
+          __syncthreads();
+          arcs(WAVE);
+          // move the rest (fewer than WAVE) to the front
+          const int rest = nf - WAVE;
+          uint16_t e = 0;
+          if (lane() < rest) e = fq[WAVE + lane()];
+          __syncthreads();
+          if (lane() < rest) fq[lane()] = e;
+          nf = rest;
+        }
       }
+      __syncthreads();
+      if (nf > 0) arcs(nf);
       __syncthreads();
       if (lane() < nb) {
         w.t1cnt[p] = tn1[lane()];
