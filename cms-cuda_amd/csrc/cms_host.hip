@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
+  DBuf ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -248,6 +248,7 @@ static Work work(cms_ctx* ctx) {
   w.ccnt = ctx->ccnt.as<int>();
   w.cscan = ctx->cscan.as<int>();
   w.cmask = ctx->cmask.as<uint64_t>();
+  w.cvf = ctx->cvf.as<double4>();
   w.t1cnt = ctx->t1cnt.as<int>();
   w.t1scan = ctx->t1scan.as<int>();
   w.t2cnt = ctx->t2cnt.as<int>();
@@ -258,6 +259,7 @@ static Work work(cms_ctx* ctx) {
   w.rrel = ctx->rrel.as<uint64_t>();
   w.rtor = ctx->rtor.as<int8_t>();
   w.kcnt = ctx->kcnt.as<int>();
+  w.ccf = ctx->ccf.as<double4>();
   w.kpos = ctx->kpos.as<double4>();
   w.kinfo = ctx->kinfo.as<uint64_t>();
   w.kscan = ctx->kscan.as<int>();
@@ -283,9 +285,9 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag, &ctx->pstage, &ctx->nps,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
-                  &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt,
+                  &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt, &ctx->cvf,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->ccf, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
                   &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -526,6 +528,7 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->nps.ensure(sizeof(int) * (c + 1)));
   CK(ctx->rrel.ensure(sizeof(uint64_t) * (KCAP / WAVE) * (c + 1)));
   CK(ctx->rtor.ensure(c + 1));
+  CK(ctx->cvf.ensure(sizeof(double4) * 4 * (c + 1)));
   ctx->cap[M_R] = std::max(ctx->cap[M_R], c);
   return CMS_OK;
 }
@@ -559,6 +562,7 @@ static int size_probes(cms_ctx* ctx, int64_t c) {
   CK(ctx->nearl.ensure(sizeof(int) * (c * NEARCAP + 1)));
   CK(ctx->nnear.ensure(sizeof(int) * (c + 1)));
   CK(ctx->kcnt.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->ccf.ensure(sizeof(double4) * 7 * (c + 1)));
   CK(ctx->kpos.ensure(sizeof(double4) * KSLOTS * (c + 1)));
   CK(ctx->kinfo.ensure(sizeof(uint64_t) * KSLOTS * (c + 1)));
   CK(ctx->kscan.ensure(sizeof(int) * (c + 1)));
@@ -873,6 +877,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   // ---- dot counts
   if (est[M_R] > 0) {
     fill_int(ctx->ccnt.as<int>(), cR, ctx->cap[M_R], 0, st);
+    TIMED("convex_frames", k_convex_frames<<<grid_for(est[M_R], 256, 8192), 256, 0, st>>>(b, s, w));
     TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
   }
   if (est[M_P] > 0 && !by_row) {
@@ -883,10 +888,12 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     TIMED("toroid_count", k_toroid_count<<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
   }
   if (est[M_Q] > 0) {
-    if (s.rp > 0.0)
+    if (s.rp > 0.0) {
+      TIMED("concave_frames", k_concave_frames<<<grid_for(est[M_Q], 256, 8192), 256, 0, st>>>(b, s, w));
       TIMED("concave_count", k_concave<false><<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
-    else
+    } else {
       fill_int(ctx->kcnt.as<int>(), ctx->cap[M_Q], 0, st);
+    }
   }
   CKL();
   scan_excl(ctx->ccnt.as<int>(), ctx->cscan.as<int>(), cR, ctx->cap[M_R], part, st);

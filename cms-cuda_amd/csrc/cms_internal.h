@@ -126,6 +126,7 @@ struct Work {
   int pslots;             // slots per row in use (<= PSLOTS)
   // dot counts
   int* ccnt;
+  double4* cvf;           // convex frames, 4 per row (k_convex_frames)
   int* cscan;
   uint64_t* cmask;        // count pass: kept convex points, CMASK_WORDS words per row
   int* t1cnt;
@@ -138,6 +139,7 @@ struct Work {
   uint64_t* rrel;         // per-row pass: per row, the union of its live toroid pairs' reach masks (4 words)
   int8_t* rtor;           // per-row pass: the row has a live toroid pair
   int* kcnt;
+  double4* ccf;           // concave frames, 7 per probe (k_concave_frames)
   double4* kpos;          // KSLOTS staged concave dots per probe
   uint64_t* kinfo;
   int* kscan;
@@ -165,11 +167,13 @@ template <int RC> __global__ void k_probe_rows(BatchView, Settings, Work);
 __global__ void k_probe_place(BatchView, Work);
 template <int RC> __global__ void k_torus_rows(BatchView, Settings, Work);
 template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
+__global__ void k_convex_frames(BatchView, Settings, Work);
 __global__ void k_toroid_count(BatchView, Settings, Work);
 __global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);
 __global__ void k_compact_count(Count, const int*, const int*, int*);
 __global__ void k_toroid_emit(BatchView, Settings, Work, const int*, const int*);
 template <bool EMIT> __global__ void k_concave(BatchView, Settings, Work);
+__global__ void k_concave_frames(BatchView, Settings, Work);
 __global__ void k_concave_place(BatchView, Settings, Work);
 __global__ void k_compact_index(int, const int*, const int*, int*);
 __global__ void k_near(BatchView, Settings, Work);

@@ -1101,6 +1101,68 @@ __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s,
 // _core.py:1646-1840.  Pole from the nearest neighbour, latitude arcs from
 // the pole to the torus-derived south point, circles at each latitude,
 // collision with every neighbour but the nearest one.
+
+// The atom's frame (_core.py:1660-1712): pole direction north (away from
+// the nearest neighbour x0), the latitude arc's second axis y, its spacing
+// delta and count L, the arc length cs of a latitude step, and the
+// latitude circles' frame (cx, cy).
+struct ConvexFrame {
+  V3 north, y, cx, cy;
+  double delta, cs;
+  int L;
+};
+__device__ __forceinline__ ConvexFrame convex_frame(V3 xi, double ri, double dens, double4 a0, double rp) {
+  ConvexFrame f;
+  const double eri = ri + rp;
+  const V3 x0 = ld3(a0);
+  const double r0 = a0.w;
+  const V3 north = unit(xi - x0);
+  V3 vt = v3(north.y * north.y + north.z * north.z, north.x * north.x + north.z * north.z,
+             north.x * north.x + north.y * north.y);
+  vt = unit(vt);
+  if (fabs(dot_seq(vt, north)) > 0.99) vt = v3(1.0, 0.0, 0.0);
+  const V3 eq = unit(cross(north, vt));
+  const V3 vql = cross(eq, north);
+  const double erj = r0 + rp;
+  const double dij = norm_seq(x0 - xi);
+  const V3 uij = (x0 - xi) / dij;
+  const double asymm = (eri * eri - erj * erj) / dij;
+  const V3 tij = ((xi + x0) * 0.5) + uij * (asymm * 0.5);
+  const double far_sq = (eri + erj) * (eri + erj) - dij * dij;
+  const double cont_sq = dij * dij - (ri - r0) * (ri - r0);
+  V3 south = v3(0.0, 0.0, -1.0);
+  if (far_sq > 0.0 && cont_sq > 0.0) {
+    const double rij = 0.5 * sqrt(far_sq) * sqrt(cont_sq) / dij;
+    const V3 pij = tij + vql * rij;
+    south = (pij - xi) / eri;
+  }
+  const double ang = arc_angle(north, south, eq, &f.y);
+  f.north = north;
+  f.delta = 1.0 / (sqrt(dens) * ri);
+  f.L = subdiv_count(f.delta, ang);
+  f.cs = f.L > 0 ? (ri * ang) / (double)f.L : 0.0;
+  circle_frame(north, &f.cx, &f.cy);
+  return f;
+}
+
+// Frames of the rows the convex count pass visits (accessible non-blocker
+// atoms with neighbours), one thread per row: the passes, one wavefront per
+// row, would otherwise compute the same frame on every lane.  Four double4
+// per row: (north, delta), (y, cs), (cx, L), (cy, 0).
+__global__ __launch_bounds__(256) void k_convex_frames(BatchView b, Settings s, Work w) {
+  const int nR = *w.nR;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nR; r += gridDim.x * blockDim.x) {
+    const int g = w.ratom[r];
+    if (!(w.nn[r] > 0 && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER)) continue;
+    const double4 ai = b.atom[g];
+    const ConvexFrame f = convex_frame(ld3(ai), ai.w, b.dens[g], b.atom[w.nbr[(int64_t)r * KCAP]], s.rp);
+    double4* o = w.cvf + 4 * (int64_t)r;
+    o[0] = make_double4(f.north.x, f.north.y, f.north.z, f.delta);
+    o[1] = make_double4(f.y.x, f.y.y, f.y.z, f.cs);
+    o[2] = make_double4(f.cx.x, f.cx.y, f.cx.z, (double)f.L);
+    o[3] = make_double4(f.cy.x, f.cy.y, f.cy.z, 0.0);
+  }
+}
 template <bool EMIT>
 __global__ __launch_bounds__(64, EMIT ? 3 : 4) void k_convex(BatchView b, Settings s, Work w) {
   // Occupancy: the count pass is held to 128 VGPRs (four wavefronts per
@@ -1134,36 +1196,12 @@ __global__ __launch_bounds__(64, EMIT ? 3 : 4) void k_convex(BatchView b, Settin
       const V3 xi = ld3(ai);
       const double ri = ai.w, eri = ri + s.rp;
       const double dens = b.dens[g];
-      const double4 a0 = b.atom[row[0]];
-      const V3 x0 = ld3(a0);
-      const double r0 = a0.w;
-      const V3 north = unit(xi - x0);
-      V3 vt = v3(north.y * north.y + north.z * north.z, north.x * north.x + north.z * north.z,
-                 north.x * north.x + north.y * north.y);
-      vt = unit(vt);
-      if (fabs(dot_seq(vt, north)) > 0.99) vt = v3(1.0, 0.0, 0.0);
-      const V3 eq = unit(cross(north, vt));
-      const V3 vql = cross(eq, north);
-      const double erj = r0 + s.rp;
-      const double dij = norm_seq(x0 - xi);
-      const V3 uij = (x0 - xi) / dij;
-      const double asymm = (eri * eri - erj * erj) / dij;
-      const V3 tij = ((xi + x0) * 0.5) + uij * (asymm * 0.5);
-      const double far_sq = (eri + erj) * (eri + erj) - dij * dij;
-      const double cont_sq = dij * dij - (ri - r0) * (ri - r0);
-      V3 south = v3(0.0, 0.0, -1.0);
-      if (far_sq > 0.0 && cont_sq > 0.0) {
-        const double rij = 0.5 * sqrt(far_sq) * sqrt(cont_sq) / dij;
-        const V3 pij = tij + vql * rij;
-        south = (pij - xi) / eri;
-      }
-      V3 y;
-      const double ang = arc_angle(north, south, eq, &y);
-      const double delta = 1.0 / (sqrt(dens) * ri);
-      const int L = subdiv_count(delta, ang);
-      const double cs = L > 0 ? (ri * ang) / (double)L : 0.0;
-      V3 cx, cy;
-      circle_frame(north, &cx, &cy);
+      // the row's frame from k_convex_frames
+      const double4 f0 = w.cvf[4 * (int64_t)r], f1 = w.cvf[4 * (int64_t)r + 1];
+      const double4 f2 = w.cvf[4 * (int64_t)r + 2], f3 = w.cvf[4 * (int64_t)r + 3];
+      const V3 north = ld3(f0), y = ld3(f1), cx = ld3(f2), cy = ld3(f3);
+      const double delta = f0.w, cs = f1.w;
+      const int L = (int)f2.w;
       // per-latitude circles
       int run = 0;
       for (int t0 = 0; t0 < L; t0 += WAVE) {
@@ -1540,6 +1578,66 @@ __global__ __launch_bounds__(64) void k_toroid_emit(BatchView b, Settings s, Wor
 // rejection, and the closest of the three atoms as owner.
 constexpr int CC_BLOCK = 512;   // concave candidate points per prefilter block
 
+// The probe's frame (_core.py:2140-2200): the three plane normals vec0-2 of
+// the spherical triangle, the highest vertex top (index mm) with the
+// latitude arc's second axis y, spacing delta, count L and step length cs,
+// the latitude circles' frame (cx, cy) about south = -alt, and the mean
+// density of the three atoms.
+struct ConcaveFrame {
+  V3 vec0, vec1, vec2, top, y, cx, cy;
+  double delta, cs, dens;
+  int L, mm;
+};
+__device__ __forceinline__ ConcaveFrame concave_frame(V3 p, V3 u, double4 A0, double4 A1, double4 A2, double dens,
+                                                      double rp) {
+  ConcaveFrame f;
+  const V3 vp0 = unit(ld3(A0) - p), vp1 = unit(ld3(A1) - p), vp2 = unit(ld3(A2) - p);
+  f.vec0 = unit(cross(vp0, vp1));
+  f.vec1 = unit(cross(vp1, vp2));
+  f.vec2 = unit(cross(vp2, vp0));
+  int mm = 0;
+  {
+    // the first of the largest (strict >), as in a loop over the atoms
+    double best = dot_seq(u, vp0);
+    const double d1 = dot_seq(u, vp1);
+    if (d1 > best) { best = d1; mm = 1; }
+    if (dot_seq(u, vp2) > best) mm = 2;
+  }
+  f.mm = mm;
+  f.top = mm == 0 ? vp0 : (mm == 1 ? vp1 : vp2);
+  const V3 south = neg(u);
+  const V3 axis = unit(cross(f.top, south));
+  const double ang = arc_angle(f.top, south, axis, &f.y);
+  f.delta = 1.0 / (sqrt(dens) * rp);
+  f.L = subdiv_count(f.delta, ang);
+  f.cs = f.L > 0 ? (rp * ang) / (double)f.L : 0.0;
+  f.dens = dens;
+  circle_frame(south, &f.cx, &f.cy);
+  return f;
+}
+
+// Frames of every probe, one thread per probe (the concave passes, one
+// wavefront per probe, would otherwise compute the same frame on every
+// lane).  Seven double4 per probe: (vec0, delta), (vec1, cs), (vec2, dens),
+// (top, L), (y, mm), (cx, 0), (cy, 0).
+__global__ __launch_bounds__(256) void k_concave_frames(BatchView b, Settings s, Work w) {
+  const int nQ = *w.nQ;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nQ; q += gridDim.x * blockDim.x) {
+    const int4 at = w.patom[q];
+    const double dens = ((b.dens[at.x] + b.dens[at.y]) + b.dens[at.z]) / 3.0;
+    const ConcaveFrame f = concave_frame(ld3(w.ppos[q]), ld3(w.palt[q]), b.atom[at.x], b.atom[at.y], b.atom[at.z],
+                                         dens, s.rp);
+    double4* o = w.ccf + 7 * (int64_t)q;
+    o[0] = make_double4(f.vec0.x, f.vec0.y, f.vec0.z, f.delta);
+    o[1] = make_double4(f.vec1.x, f.vec1.y, f.vec1.z, f.cs);
+    o[2] = make_double4(f.vec2.x, f.vec2.y, f.vec2.z, f.dens);
+    o[3] = make_double4(f.top.x, f.top.y, f.top.z, (double)f.L);
+    o[4] = make_double4(f.y.x, f.y.y, f.y.z, (double)f.mm);
+    o[5] = make_double4(f.cx.x, f.cx.y, f.cx.z, 0.0);
+    o[6] = make_double4(f.cy.x, f.cy.y, f.cy.z, 0.0);
+  }
+}
+
 // The count pass is held to 168 VGPRs for three wavefronts per SIMD: the
 // few spilled values cost less than the lost occupancy (not so in the emit
 // pass).
@@ -1563,31 +1661,16 @@ __global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Setti
     const double h = pq.w;
     const V3 u = ld3(w.palt[q]);
     // the three atoms in named registers: indexing arrays of them by the
-    // runtime owner / top choice would put them in scratch memory
+    // runtime owner choice would put them in scratch memory
     const double4 A0 = b.atom[at.x], A1 = b.atom[at.y], A2 = b.atom[at.z];
-    const double dens = ((b.dens[at.x] + b.dens[at.y]) + b.dens[at.z]) / 3.0;
-    const V3 vp0 = unit(ld3(A0) - p), vp1 = unit(ld3(A1) - p), vp2 = unit(ld3(A2) - p);
-    const V3 vec0 = unit(cross(vp0, vp1));
-    const V3 vec1 = unit(cross(vp1, vp2));
-    const V3 vec2 = unit(cross(vp2, vp0));
-    int mm = 0;
-    {
-      // the first of the largest (strict >), as in a loop over the atoms
-      double best = dot_seq(u, vp0);
-      const double d1 = dot_seq(u, vp1);
-      if (d1 > best) { best = d1; mm = 1; }
-      if (dot_seq(u, vp2) > best) mm = 2;
-    }
-    const V3 top = mm == 0 ? vp0 : (mm == 1 ? vp1 : vp2);
+    // the probe's frame from k_concave_frames
+    const double4* fr = w.ccf + 7 * (int64_t)q;
+    const double4 f0 = fr[0], f1 = fr[1], f2 = fr[2], f3 = fr[3], f4 = fr[4], f5 = fr[5], f6 = fr[6];
+    const V3 vec0 = ld3(f0), vec1 = ld3(f1), vec2 = ld3(f2), top = ld3(f3), y = ld3(f4), cx = ld3(f5),
+             cy = ld3(f6);
+    const double delta = f0.w, cs = f1.w, dens = f2.w;
+    const int L = (int)f3.w, mm = (int)f4.w;
     const V3 south = neg(u);
-    const V3 axis = unit(cross(top, south));
-    V3 y;
-    const double ang = arc_angle(top, south, axis, &y);
-    const double delta = 1.0 / (sqrt(dens) * rp);
-    const int L = subdiv_count(delta, ang);
-    const double cs = L > 0 ? (rp * ang) / (double)L : 0.0;
-    V3 cx, cy;
-    circle_frame(south, &cx, &cy);
     const int nnear = (h < rp) ? w.nnear[q] : 0;
     for (int t = lane(); t < nnear; t += WAVE) lnear[t] = w.ppos[w.near[(int64_t)q * NEARCAP + t]];
     int run = 0;
