@@ -331,7 +331,8 @@ __device__ __forceinline__ int cell_slot(const int* cbeg, const int* cpre, int i
 // neighbour cells within 0.01 A of it.  Also seeds the interface-region
 // flag (non-blockers; every atom when full != 0).  Atoms are visited in
 // cell order so that consecutive workgroups read the same cells.
-__device__ __forceinline__ void attention_store(int g, int c, double mn, unsigned long long ckey, int all_atoms,
+// Returns whether the atom is a non-blocker.
+__device__ __forceinline__ bool attention_store(int g, int c, double mn, unsigned long long ckey, int all_atoms,
                                                 int full, const Settings& s, int8_t* atten, int8_t* access,
                                                 int* need, unsigned long long* coinc, int* status) {
   // atten, access and need come pre-filled in atom order (atten: blocker,
@@ -339,7 +340,8 @@ __device__ __forceinline__ void attention_store(int g, int c, double mn, unsigne
   // only the non-blockers of a normal run are stored here: few scattered
   // stores instead of three per atom
   const double dmin = (mn == INFINITY) ? 99999.0 : sqrt(mn);
-  if (!all_atoms && !(dmin >= s.sep)) {
+  const bool nonblocker = !all_atoms && !(dmin >= s.sep);
+  if (nonblocker) {
     atten[g] = ATTEN_BURIED;
     if (!full) need[g] = 1;
   }
@@ -347,6 +349,7 @@ __device__ __forceinline__ void attention_store(int g, int c, double mn, unsigne
     atomicMin(&coinc[c], ckey);
     atomicOr(&status[c], ST_COINCIDENT);
   }
+  return nonblocker;
 }
 
 __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s, int all_atoms, int full,
@@ -404,13 +407,19 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
 // Interface region: every non-blocker marks the same-molecule atoms within
 // the bridge distance r_i + r_j + 2 rp of it (the relation is symmetric, so
 // this is the set of atoms within the bridge distance of a non-blocker).
+// One wavefront per cell slot, the candidates over the lanes; with a list
+// (the non-blockers' cell slots, *list_n of them, in any order) only its
+// slots.
 __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, const GridInfo* __restrict__ grid,
                                                      const int* __restrict__ cstart,
                                                      const int* __restrict__ catom,
                                                      const double4* __restrict__ cxyz,
-                                                     const int8_t* __restrict__ atten, int* __restrict__ need) {
+                                                     const int8_t* __restrict__ atten,
+                                                     const int* __restrict__ list,
+                                                     const int* __restrict__ list_n, int* __restrict__ need) {
   __shared__ int cbeg[27], cpre[28];
-  CMS_UNIT_LOOP(sidx, b.ntot) {
+  CMS_UNIT_LOOP(u, list ? *list_n : b.ntot) {
+    const int sidx = list ? list[u] : (int)u;
     const int g = catom[sidx];
     if (atten[g] <= ATTEN_BLOCKER) continue;
     const int c = b.acplx[g];
@@ -449,7 +458,8 @@ __global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Setti
                                                            int8_t* __restrict__ atten, int8_t* __restrict__ access,
                                                            int* __restrict__ need,
                                                            unsigned long long* __restrict__ coinc,
-                                                           int* __restrict__ status) {
+                                                           int* __restrict__ status, int* __restrict__ list,
+                                                           int* __restrict__ list_n) {
   for (int sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < b.ntot; sidx += gridDim.x * blockDim.x) {
     const int g = catom[sidx];
     const int c = b.acplx[g];
@@ -493,64 +503,19 @@ __global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Setti
         }
       }
     }
-    attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
-  }
-}
-
-// Only non-blockers sweep: one wavefront takes a chunk of RG_CHUNK
-// consecutive cell slots, lists its non-blockers in LDS (still in cell
-// order, so neighbouring lanes share cells) and sweeps them 64 at a time,
-// instead of idling the lanes of the blockers.
-constexpr int RG_CHUNK = 256;
-
-__global__ __launch_bounds__(64) void k_region_cells_t(BatchView b, Settings s, const GridInfo* __restrict__ grid,
-                                                       const int* __restrict__ cstart,
-                                                       const int* __restrict__ catom,
-                                                       const double4* __restrict__ cxyz,
-                                                       const int8_t* __restrict__ atten, int* __restrict__ need) {
-  __shared__ int sel[RG_CHUNK];
-  for (int64_t base = (int64_t)blockIdx.x * RG_CHUNK; base < b.ntot; base += (int64_t)gridDim.x * RG_CHUNK) {
-    int nsel = 0;
-    for (int k = 0; k < RG_CHUNK; k += WAVE) {
-      const int64_t sx = base + k + lane();
-      const bool nb = sx < b.ntot && atten[catom[sx]] > ATTEN_BLOCKER;
-      const uint64_t mk = ballot(nb);
-      if (nb) sel[nsel + lanes_below(mk)] = (int)sx;
-      nsel += __popcll(mk);
-    }
-    __syncthreads();
-    for (int k0 = 0; k0 < nsel; k0 += WAVE) {
-      if (k0 + lane() >= nsel) continue;
-      const int sidx = sel[k0 + lane()];
-      const int g = catom[sidx];
-      const int m = b.amol[g];
-      const GridInfo gi = grid[b.acplx[g]];
-      const double4 ai = cxyz[sidx];
-      const V3 xi = ld3(ai);
-      int ix, iy, iz;
-      cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
-      // the bridge distance is at most r_i + r_max + 2 rp; the cells within
-      // it form one z-run per (x, y) column
-      const double lim = (ai.w + gi.rmax) + 2.0 * s.rp;
-      const double l2 = (lim + 1e-6) * (lim + 1e-6);
-      const double fx = ai.x - (gi.x0 + ix * gi.h), fy = ai.y - (gi.y0 + iy * gi.h), fz = ai.z - (gi.z0 + iz * gi.h);
-      for (int col = 0; col < 9; ++col) {
-        int z0, z1;
-        if (!column_within(gi, fx, fy, fz, ix, iy, iz, col / 3, col % 3, l2, &z0, &z1)) continue;
-        const int cx = ix + col / 3 - 1, cy = iy + col % 3 - 1;
-        const ColRanges cr = column_ranges(gi, cstart, cx, cy, z0, z1, m);
-  #pragma unroll 4
-        for (int t = 0; t < cr.total; ++t) {
-          const int slot = col_index(cr, t);
-          const int j = catom[slot];
-          const double4 aj = cxyz[slot];
-          const V3 d = xi - ld3(aj);
-          const double br = (ai.w + aj.w) + 2.0 * s.rp;
-          if (dot_seq(d, d) < br * br) need[j] = 1;
-        }
+    const bool nonblocker = attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
+    if (list) {
+      // the region pass's list of non-blocker cell slots: one atomic per
+      // wavefront
+      const uint64_t mk = ballot(nonblocker);
+      if (mk) {
+        const int leader = __builtin_ctzll(mk);
+        int base = 0;
+        if (lane() == leader) base = atomicAdd(list_n, __popcll(mk));
+        base = __shfl(base, leader, WAVE);
+        if (nonblocker) list[base + lanes_below(mk)] = sidx;
       }
     }
-    __syncthreads();
   }
 }
 

@@ -74,7 +74,7 @@ struct cms_ctx {
   int gen_all_atoms = -1;
   int nR = 0, nP = 0, nQ = 0, nD = 0;
   std::vector<int> h_dgrp, h_prgrp;
-  DBuf atten, access, status, coinc, flag, nbscan, nbl, need, rpos, ratom, part;
+  DBuf atten, access, status, coinc, rsel, rseln, need, rpos, ratom, part;
   DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom, cxyz;
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
@@ -280,7 +280,7 @@ static Work work(cms_ctx* ctx) {
 static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
   return {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
                   &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
-                  &ctx->flag, &ctx->nbscan, &ctx->nbl, &ctx->grid, &ctx->gframe, &ctx->ncellc,
+                  &ctx->rsel, &ctx->rseln, &ctx->grid, &ctx->gframe, &ctx->ncellc,
                   &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag, &ctx->pstage, &ctx->nps,
@@ -697,6 +697,8 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CK(ctx->status.ensure(sizeof(int) * B));
   CK(ctx->coinc.ensure(sizeof(unsigned long long) * B));
   CK(ctx->need.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->rsel.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->rseln.ensure(sizeof(int)));
   CK(ctx->rpos.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ratom.ensure(sizeof(int) * (n + 1)));
   CK(ctx->part.ensure(sizeof(int) * 1040));
@@ -741,8 +743,9 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     fill_int(ctx->ccur.as<int>(), cCells, cell_cap, 0, st);
     // small batches: cell lists thread per atom and one wavefront per atom
     // in the sweeps (parallelism); large batches: cell lists workgroup per
-    // complex (LDS counters) and one thread per cell-sorted atom in the
-    // sweeps (cell data shared within a wavefront)
+    // complex (LDS counters), one thread per cell-sorted atom in the
+    // attention sweep (cell data shared within a wavefront) and one
+    // wavefront per listed non-blocker in the region sweep
     const bool per_thread = ctx->cell_sweep ? ctx->cell_sweep == 2 : n >= 200000;
     if (per_thread)
       k_cell_count<<<grid_for(B, 1, 65536), 512, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
@@ -768,19 +771,24 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     fill_int8(ctx->access.as<int8_t>(), n, 0, st);
     fill_int(ctx->need.as<int>(), (int64_t)n, full, st);
     if (per_thread) {
+      // the attention pass lists the non-blockers' cell slots (a few per
+      // cent of the atoms) for the region pass
+      int* list = full ? nullptr : ctx->rsel.as<int>();
+      if (list) fill_int(ctx->rseln.as<int>(), 1, 0, st);
       TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
           b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
-          ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
+          ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>(), list,
+          ctx->rseln.as<int>()));
       if (!full)
-        TIMED("region", k_region_cells_t<<<grid_for(n, 256, 65536), 64, 0, st>>>(
-            b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+        TIMED("region", k_region_cells<<<grid_for(n, 16, 65536), 64, 0, st>>>(
+            b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), list, ctx->rseln.as<int>(), ctx->need.as<int>()));
     } else {
       TIMED("attention", k_attention_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
           b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
           ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
       if (!full)
         TIMED("region", k_region_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
-            b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->need.as<int>()));
+            b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), nullptr, nullptr, ctx->need.as<int>()));
     }
     scan_excl(ctx->need.as<int>(), ctx->rpos.as<int>(), n, part, st);
     k_compact_index<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->need.as<int>(),
