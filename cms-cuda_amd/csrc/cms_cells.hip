@@ -491,17 +491,34 @@ __global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Setti
         }
       }
     }
+    // coincident partners, each pair once with key (lower, higher index):
+    // the minimum over pairs is np.argwhere's first entry.  The atom's own
+    // cell range is taken cyclically, the floor(L / 2) slots after the
+    // atom's (every pair of the range is within that distance one way
+    // round, and each lane does the same number of tests); the other cells
+    // within 0.01 A of the atom in full
     unsigned long long ckey = ~0ull;
-    const uint32_t cmask = coincident_cells(gi, ai.x, ai.y, ai.z, ix, iy, iz);
-    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cmask, cell) {
-      for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) {
+    auto coincident = [&](int t) {
+      const V3 d = xi - ld3(cxyz[t]);
+      if (dot_seq(d, d) <= 0.0001) {
         const int j = catom[t];
-        const V3 d = xi - ld3(cxyz[t]);
-        if (j != g && dot_seq(d, d) <= 0.0001) {
-          const unsigned long long key = ((unsigned long long)(g - a0) << 32) | (unsigned)(j - a0);
-          ckey = ckey < key ? ckey : key;
-        }
+        const int lo = min(g, j) - a0, hi = max(g, j) - a0;
+        const unsigned long long key = ((unsigned long long)lo << 32) | (unsigned)hi;
+        ckey = ckey < key ? ckey : key;
       }
+    };
+    {
+      const int own = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
+      const int lo = cstart[2 * own + m], hi = cstart[2 * own + m + 1];
+      const int L = hi - lo;
+      for (int k = 1; k <= L / 2; ++k) {
+        const int t = sidx + k < hi ? sidx + k : sidx + k - L;
+        coincident(t);
+      }
+    }
+    const uint32_t cmask = coincident_cells(gi, ai.x, ai.y, ai.z, ix, iy, iz) & ~(1u << 13);   // bit 13: own cell
+    FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cmask, cell) {
+      for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) coincident(t);
     }
     const bool nonblocker = attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
     if (list) {

@@ -141,3 +141,26 @@ def test_all_failed_and_single_complex_errors():
     with pytest.raises(RuntimeError, match=r"^Coincident atoms: 93:: == 97::$"):
         bc.run(_lib.PREC_FP64)
     bc.close()
+
+
+def test_coincident_pair_order_in_both_sweeps():
+    """Several coincident pairs: the message names np.argwhere's first pair
+    (cms-cuda/_core.py:1285-1293) in the wavefront-per-atom and the
+    thread-per-atom attention sweeps alike."""
+    from cms_cuda.batch import BatchCalculator
+    from cms_cuda import _lib, synth, tuning
+    b = synth.make_complex(500, 90, 4102, gap=3.6)
+    tx = b["target_xyz"].copy()
+    tx[40] = tx[17]
+    tx[9] = tx[30]
+    tx[11] = tx[30]
+    c = (b["binder_xyz"], b["binder_radii"], tx, b["target_radii"])
+    # target pairs (9, 11), (9, 30), (11, 30), (17, 40); binder atoms first
+    for sweep in (1, 2):
+        say("coincident pairs, cell sweep %d" % sweep)
+        bc = BatchCalculator()
+        bc.load([c])
+        tuning.apply(bc.ctx, dict(cell_sweep=sweep))
+        with pytest.raises(RuntimeError, match=r"^Coincident atoms: 99:: == 101::$"):
+            bc.run(_lib.PREC_FP64)
+        bc.close()
