@@ -134,6 +134,8 @@ class Leg:
             self.g_xyz = torch.empty(3 * self.ntot, dtype=torch.float64, device=dev)
         self.stream = torch.cuda.current_stream(dev)
         self.ctx = _lib.Context(0, self.stream.cuda_stream)
+        # dot descriptors only for the backward pass
+        self.ctx("cms_set_keep_descriptors", 1 if grad else 0)
         self.st = Settings().to_c()
         self.nb_c = (ctypes.c_int32 * self.B)(*self.nb.tolist())
         self.nt_c = (ctypes.c_int32 * self.B)(*self.nt.tolist())

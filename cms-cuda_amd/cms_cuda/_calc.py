@@ -276,6 +276,8 @@ class MolecularSurfaceCalculator:
         # calculator returns it to the pool instead of destroying it
         self._ctx = _lib.acquire_context(device, stream)
         self._fin = weakref.finalize(self, _lib.release_context, self._ctx)
+        # the calculator has no backward pass: no dot descriptors
+        self._ctx("cms_set_keep_descriptors", 0)
 
     def _release(self):
         fin = getattr(self, "_fin", None)

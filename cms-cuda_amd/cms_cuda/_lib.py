@@ -61,6 +61,7 @@ _PROTOS = {
     "cms_shape_complementarity": (_c.c_int, [_vp, _c.c_int, _vp]),
     "cms_run": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp]),
     "cms_set_graph": (_c.c_int, [_vp, _c.c_int]),
+    "cms_set_keep_descriptors": (_c.c_int, [_vp, _c.c_int]),
     "cms_run_stats": (_c.c_int, [_vp, _vp]),
     "cms_dot_count": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.POINTER(_i64)]),
     "cms_copy_dots": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -31,6 +31,7 @@ class _ContactMS:
         self.dev = xyz.device
         self.stream = torch.cuda.current_stream(self.dev)
         self.ctx = _lib.Context(self.dev.index, self.stream.cuda_stream)
+        self.ctx("cms_set_keep_descriptors", 1)      # what the backward pass re-evaluates dots from
         self.B = len(n_binder)
         self.ntot = int(xyz.shape[0])
         self.nb = (ctypes.c_int32 * self.B)(*[int(v) for v in n_binder])

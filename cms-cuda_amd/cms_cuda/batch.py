@@ -25,14 +25,18 @@ from ._calc import Settings, _device, _ptr
 
 
 class BatchCalculator:
-    def __init__(self, settings=None, device=None, stream=None, tuned=True):
+    def __init__(self, settings=None, device=None, stream=None, tuned=True, keep_descriptors=False):
         """tuned: launch shapes from the tuning table for the loaded batch's
-        size (cms_cuda.tuning); False keeps the library's built-in rules."""
+        size (cms_cuda.tuning); False keeps the library's built-in rules.
+        keep_descriptors: write each dot's descriptors during generation
+        (what cms_copy_dot_desc / the backward pass read; forward-only
+        batches skip them)."""
         import torch
         self.settings = settings or Settings()
         self.dev = device or _device()
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.dev)
         self.ctx = _lib.Context(self.dev.index, self.stream.cuda_stream)
+        self.ctx("cms_set_keep_descriptors", 1 if keep_descriptors else 0)
         self.B = 0
         self.mode = None
         self.tuned = tuned

@@ -71,6 +71,7 @@ class Runner:
         self.nt = (ctypes.c_int32 * self.B)(*nt)
         self.prec = precision
         self.ctx("cms_set_graph", 1 if graph else 0)
+        self.ctx("cms_set_keep_descriptors", 0)
 
     def step(self):
         P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731

@@ -109,6 +109,8 @@ struct cms_ctx {
   uint64_t layout_epoch = 0;        // bumped when the batch layout changes
   // one captured cms_run step
   bool use_graph = false;
+  bool keep_desc = true;            // generation writes dot descriptors
+  bool gen_desc = false;            // the current surface has them
   // the packed per-atom ids / densities came from default densities
   bool packed_default = false;
   double packed_density = 0.0;
@@ -163,6 +165,8 @@ struct ProfScope {
   } while (0)
 
 static thread_local std::string g_create_err;
+static const char* const NO_DESC =
+    "the surface was generated without dot descriptors (cms_set_keep_descriptors(ctx, 1) before generating)";
 
 #define CK(expr)                                                                      \
   do {                                                                                \
@@ -269,8 +273,9 @@ static Work work(cms_ctx* ctx) {
   w.dbur = ctx->dbur.as<int8_t>();
   w.dtype = ctx->dtype.as<int8_t>();
   w.datom = ctx->datom.as<int>();
-  w.ddesc = ctx->ddesc.as<int4>();
-  w.ddesc2 = ctx->ddesc2.as<int4>();
+  // descriptors only when asked for (the emit passes test the pointer)
+  w.ddesc = ctx->keep_desc ? ctx->ddesc.as<int4>() : nullptr;
+  w.ddesc2 = ctx->keep_desc ? ctx->ddesc2.as<int4>() : nullptr;
   w.pair_cap = ctx->cap[M_P];
   w.probe_cap = ctx->cap[M_Q];
   w.dot_cap = ctx->cap[M_D];
@@ -945,6 +950,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CKL();
   ctx->generated = true;
   ctx->gen_all_atoms = mode;
+  ctx->gen_desc = ctx->keep_desc;
   if (exact) {
     ++ctx->n_exact;
     ctx->learned[mode] = true;
@@ -1347,7 +1353,8 @@ static std::vector<uint64_t> run_key(cms_ctx* ctx, int precision, double* per_at
                              (uint64_t)(uintptr_t)per_atom, (uint64_t)(uintptr_t)totals,
                              (uint64_t)(uintptr_t)sc,  ctx->layout_epoch,       workspace_signature(ctx)};
   for (int i = M_R; i <= M_T; ++i) k.push_back((uint64_t)ctx->est[i]);
-  for (int v : {ctx->trim_lanes, ctx->probe_count_mode, ctx->probe_slots, ctx->cell_sweep, ctx->search_split})
+  for (int v : {ctx->trim_lanes, ctx->probe_count_mode, ctx->probe_slots, ctx->cell_sweep, ctx->search_split,
+                (int)ctx->keep_desc})
     k.push_back((uint64_t)(int64_t)v);
   const double sv[6] = {ctx->s.rp, ctx->s.band, ctx->s.sep, ctx->s.weight, ctx->s.binwidth_dist,
                         ctx->s.binwidth_norm};
@@ -1434,6 +1441,7 @@ int cms_run(cms_ctx* ctx, int precision, double* per_atom, double* totals, doubl
       fin = ctx->gstream;
       ctx->generated = true;
       ctx->gen_all_atoms = 0;
+      ctx->gen_desc = ctx->keep_desc;
       ctx->lists_ready = true;
       ctx->float_lists = precision == CMS_PREC_FP32_RESCORE;
       ctx->cms_ready = true;
@@ -1473,6 +1481,12 @@ int cms_set_graph(cms_ctx* ctx, int enable) {
   if (!ctx) return CMS_ERR_INVALID;
   ctx->use_graph = enable != 0;
   if (!ctx->use_graph) drop_graph(ctx);
+  return CMS_OK;
+}
+
+int cms_set_keep_descriptors(cms_ctx* ctx, int keep) {
+  if (!ctx) return CMS_ERR_INVALID;
+  ctx->keep_desc = keep != 0;
   return CMS_OK;
 }
 
@@ -1552,6 +1566,7 @@ int cms_copy_dot_desc(cms_ctx* ctx, int c, int mol, int32_t* d1, int32_t* d2) {
   int r = cms_dot_count(ctx, c, mol, &n);
   if (r) return r;
   if (!d1 || !d2) return CMS_ERR_INVALID;
+  if (!ctx->gen_desc) return fail(ctx, CMS_ERR_STATE, "cms_copy_dot_desc: %s", NO_DESC);
   if (n == 0) return CMS_OK;
   const int g = 2 * c + mol;
   k_unpack_desc<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>((int)n, ctx->h_dgrp[g], ctx->ddesc.as<int4>(),
@@ -1999,6 +2014,7 @@ int cms_backward_cms(cms_ctx* ctx, const double* g_totals, const double* g_per_a
   if (!ctx || !grad_xyz) return CMS_ERR_INVALID;
   if (!ctx->cms_ready)
     return fail(ctx, CMS_ERR_STATE, "cms_backward_cms: run cms_contact_ms on the current surface first");
+  if (!ctx->gen_desc) return fail(ctx, CMS_ERR_STATE, "cms_backward_cms: %s", NO_DESC);
   CK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int* meta = ctx->meta.as<int>();
@@ -2029,6 +2045,7 @@ int cms_frozen_cms(cms_ctx* ctx, const double* xyz_eval, double* totals) {
   if (!ctx || !xyz_eval || !totals) return CMS_ERR_INVALID;
   if (!ctx->cms_ready)
     return fail(ctx, CMS_ERR_STATE, "cms_frozen_cms: run cms_contact_ms on the current surface first");
+  if (!ctx->gen_desc) return fail(ctx, CMS_ERR_STATE, "cms_frozen_cms: %s", NO_DESC);
   CK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int n = ctx->ntot, G = 2 * ctx->B;
@@ -2057,6 +2074,7 @@ int cms_frozen_cms(cms_ctx* ctx, const double* xyz_eval, double* totals) {
 int cms_check_dots(cms_ctx* ctx, int64_t* mismatches, double* maxdev) {
   if (!ctx || !mismatches || !maxdev) return CMS_ERR_INVALID;
   if (!ctx->generated) return fail(ctx, CMS_ERR_STATE, "cms_check_dots: no surface");
+  if (!ctx->gen_desc) return fail(ctx, CMS_ERR_STATE, "cms_check_dots: %s", NO_DESC);
   CK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   CK(ctx->chk.ensure(16));

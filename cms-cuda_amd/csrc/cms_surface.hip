@@ -1088,8 +1088,10 @@ __device__ __forceinline__ void write_dot(const Work& w, int64_t o, const DotRec
   w.dbur[o] = d.bur;
   w.dtype[o] = d.type;
   w.datom[o] = d.atom;
-  w.ddesc[o] = d.desc;
-  w.ddesc2[o] = d.desc2;
+  if (w.ddesc) {
+    w.ddesc[o] = d.desc;
+    w.ddesc2[o] = d.desc2;
+  }
 }
 __device__ __forceinline__ void store_dot(const BatchView& b, const Settings& s, const Work& w,
                                           int64_t o, V3 pt, V3 pc, double area, int owner,

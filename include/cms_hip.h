@@ -189,6 +189,12 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results);
 int cms_run(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc_results);
 /* Enable (1) or disable (0) graph capture / replay in cms_run. */
 int cms_set_graph(cms_ctx* ctx, int enable);
+/* Whether surface generation writes each dot's descriptors (default 1).
+   Only cms_copy_dot_desc, cms_backward_cms, cms_frozen_cms and
+   cms_check_dots read them; they return CMS_ERR_STATE for a surface
+   generated with 0.  A forward-only caller saves 32 B of writes per dot.
+   No reference counterpart (the reference has no backward pass). */
+int cms_set_keep_descriptors(cms_ctx* ctx, int keep);
 /* out[4]: graphs captured, graph replays, exact (host-sized) surface runs,
    learned runs repeated after a buffer overflow. */
 int cms_run_stats(cms_ctx* ctx, int64_t* out);

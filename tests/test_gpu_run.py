@@ -240,9 +240,9 @@ def test_group_totals_numpy_order(golden_complex):
 
 def test_large_batch_kernels_match_small(golden_complex):
     """Batches of at least 200,000 atoms take the per-complex cell-list
-    kernels and the thread-per-atom attention and region sweeps (column
-    runs, early stop of the attention sweep); smaller batches take the
-    wavefront-per-atom forms.  The same complexes as one large batch and in
+    kernels, the thread-per-atom attention sweep (column runs, early stop,
+    cyclic coincidence test) and the region sweep over the attention pass's
+    list of non-blockers; smaller batches take the wavefront-per-atom forms.  The same complexes as one large batch and in
     chunks of 20: identical results."""
     cs = _batch(golden_complex, [(1800 + 40 * k, 200 + 53 * (k % 9)) for k in range(80)], 1200)
     n = sum(len(c[1]) + len(c[3]) for c in cs)
@@ -274,3 +274,39 @@ def test_trim_lanes_agree(golden_complex, monkeypatch):
     assert (a["sc"][:, 13] + a["sc"][:, 21]).sum() > 500
     for b in outs[1:]:
         assert _same(a, b)
+
+
+def test_descriptors_optional(golden_complex):
+    """Generation with and without dot descriptors (cms_set_keep_descriptors):
+    identical results; the entry points that read descriptors refuse a
+    surface generated without them and accept one generated with them."""
+    import ctypes
+    import torch
+    from cms_cuda.batch import BatchCalculator
+    from cms_cuda import _lib
+    cs = _batch(golden_complex, [(700, 200), (1200, 90)], 1500)
+    outs = []
+    for keep in (True, False):
+        say("dot descriptors kept: %s" % keep)
+        bc = BatchCalculator(keep_descriptors=keep)
+        bc.load(cs)
+        outs.append(bc.run(_lib.PREC_FP32_RESCORE))
+        n = ctypes.c_int64()
+        bc.ctx("cms_dot_count", 1, 0, ctypes.byref(n))
+        assert n.value > 100
+        d1 = torch.empty((n.value, 4), dtype=torch.int32, device="cuda")
+        d2 = torch.empty((n.value, 4), dtype=torch.int32, device="cuda")
+        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        bad, dev = ctypes.c_int64(), ctypes.c_double()
+        if keep:
+            bc.ctx("cms_copy_dot_desc", 1, 0, P(d1), P(d2))
+            bc.ctx("cms_check_dots", ctypes.byref(bad), ctypes.byref(dev))
+            assert bad.value == 0
+            assert (d1[:, 0].cpu() & 15).min() >= 1
+        else:
+            with pytest.raises(_lib.CmsError, match="without dot descriptors"):
+                bc.ctx("cms_copy_dot_desc", 1, 0, P(d1), P(d2))
+            with pytest.raises(_lib.CmsError, match="without dot descriptors"):
+                bc.ctx("cms_check_dots", ctypes.byref(bad), ctypes.byref(dev))
+        bc.close()
+    assert _same(outs[0], outs[1])
