@@ -732,9 +732,9 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   const int64_t NOLIMIT = INT64_MAX;
   if (!exact && (r = plan_learned(ctx, mode))) return r;
 
-  fill_int(ctx->status.as<int>(), B, 0, st);
-  fill_u64(ctx->coinc.as<unsigned long long>(), B, ~0ull, st);
-  fill_int(meta, M_WORDS, 0, st);
+  fill_batch({fill_of(ctx->status.as<int>(), B, 0), fill_of(ctx->coinc.as<unsigned long long>(), B, ~0ull),
+              fill_of(meta, M_WORDS, 0)},
+             st);
   if (n > 0) {
     // cell lists: per-complex grids, atoms sorted by cell
     GridInfo* grid = ctx->grid.as<GridInfo>();
@@ -744,8 +744,8 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     scan_excl(ctx->ncellc.as<int>(), ctx->ncscan.as<int>(), B, part, st);
     k_grid_offsets<<<grid_for(B, 256, 4096), 256, 0, st>>>(B, ctx->ncscan.as<int>(), grid);
     const Count cCells = dev_count(ctx->ncscan.as<int>() + B, 2);   // two slots per cell
-    fill_int(ctx->ccount.as<int>(), cCells, cell_cap, 0, st);
-    fill_int(ctx->ccur.as<int>(), cCells, cell_cap, 0, st);
+    fill_batch({fill_of(ctx->ccount.as<int>(), cCells, cell_cap, 0), fill_of(ctx->ccur.as<int>(), cCells, cell_cap, 0)},
+               st);
     // small batches: cell lists thread per atom and one wavefront per atom
     // in the sweeps (parallelism); large batches: cell lists workgroup per
     // complex (LDS counters), one thread per cell-sorted atom in the
@@ -772,14 +772,15 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     // every atom is in the region in all-atoms / full-state runs
     const int full = (full_state || all_atoms) ? 1 : 0;
     // the attention passes store only the non-blockers over these
-    fill_int8(ctx->atten.as<int8_t>(), n, all_atoms ? ATTEN_BURIED : ATTEN_BLOCKER, st);
-    fill_int8(ctx->access.as<int8_t>(), n, 0, st);
-    fill_int(ctx->need.as<int>(), (int64_t)n, full, st);
+    // (and the count of the region pass's list of non-blockers)
+    fill_batch({fill_of(ctx->atten.as<int8_t>(), n, (int8_t)(all_atoms ? ATTEN_BURIED : ATTEN_BLOCKER)),
+                fill_of(ctx->access.as<int8_t>(), n, (int8_t)0), fill_of(ctx->need.as<int>(), (int64_t)n, full),
+                fill_of(ctx->rseln.as<int>(), 1, 0)},
+               st);
     if (per_thread) {
       // the attention pass lists the non-blockers' cell slots (a few per
       // cent of the atoms) for the region pass
       int* list = full ? nullptr : ctx->rsel.as<int>();
-      if (list) fill_int(ctx->rseln.as<int>(), 1, 0, st);
       TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
           b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
           ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>(), list,
@@ -847,13 +848,13 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   if (est[M_R] > 0 && !by_row) TIMED("pairs_emit", k_pairs<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
   if (est[M_P] > 0) {
     if (by_row) {
-      fill_int(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0, st);
-      fill_int8(ctx->rtor.as<int8_t>(), ctx->cap[M_R], 0, st);
       // the torus pass writes the toroidal counts of the pairs of rows with
       // a live toroid-queue pair only
-      fill_int(ctx->t1cnt.as<int>(), cP, ctx->cap[M_P], 0, st);
-      fill_int(ctx->t2cnt.as<int>(), cP, ctx->cap[M_P], 0, st);
-      fill_int8(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0, st);
+      fill_batch({fill_of(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0), fill_of(ctx->rtor.as<int8_t>(), ctx->cap[M_R], 0),
+                  fill_of(ctx->t1cnt.as<int>(), cP, ctx->cap[M_P], 0),
+                  fill_of(ctx->t2cnt.as<int>(), cP, ctx->cap[M_P], 0),
+                  fill_of(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0)},
+                 st);
       // rows of at most RC_SMALL neighbours, then the longer ones (LDS staging
       // sized for each); then the torus points of the live toroid-queue
       // pairs with the reach masks of the pass before
@@ -1156,8 +1157,7 @@ static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* 
   CK(ctx->ocnt.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ostart.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ocur.ensure(sizeof(int) * (n + 1)));
-  fill_int(ctx->ocnt.as<int>(), n, 0, st);
-  fill_int(ctx->ocur.as<int>(), n, 0, st);
+  fill_batch({fill_of(ctx->ocnt.as<int>(), n, 0), fill_of(ctx->ocur.as<int>(), n, 0)}, st);
   const int ig = grid_for(std::max<int64_t>(est_items, 1), 256, 4096);
   k_owner_count<<<ig, 256, 0, st>>>(nitems, list, ctx->datom.as<int>(), ctx->ocnt.as<int>());
   scan_excl(ctx->ocnt.as<int>(), ctx->ostart.as<int>(), n, ctx->part.as<int>(), st);
@@ -1265,8 +1265,8 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   scan_excl(ctx->tncell.as<int>(), ctx->tncscan.as<int>(), G, part, st);
   k_grid_offsets<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, ctx->tncscan.as<int>(), tg);
   const Count cCells = dev_count(ctx->tncscan.as<int>() + G);
-  fill_int(ctx->tcount.as<int>(), cCells, cells_cap, 0, st);
-  fill_int(ctx->tcur.as<int>(), cCells, cells_cap, 0, st);
+  fill_batch({fill_of(ctx->tcount.as<int>(), cCells, cells_cap, 0), fill_of(ctx->tcur.as<int>(), cCells, cells_cap, 0)},
+             st);
   // the accessible dots read in place from the dot arrays
   const Count cD = dev_count(meta + M_D);
   const int gD = grid_for(std::max<int64_t>(ctx->est[M_D], 1), 256, 4096);

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
 #include <string>
 
 #include "cms_internal.h"
@@ -22,6 +23,24 @@ void fill_int8(int8_t* p, int64_t n, int8_t v, hipStream_t st);
 // fill of a device-counted range; cap bounds the count and sizes the grid
 void fill_int(int* p, Count n, int64_t cap, int v, hipStream_t st);
 void fill_double(double* p, int64_t n, double v, hipStream_t st);
+// Several fills in one launch: each entry sets p[0 .. n) to v, an element
+// of `bytes` bytes (1, 4 or 8); n is read on the device when it is a device
+// count, cap bounds it (and sizes the grid).
+struct FillSpec {
+  void* p;
+  Count n;
+  int64_t cap;
+  unsigned long long v;
+  int bytes;
+};
+constexpr int FILL_BATCH = 6;
+void fill_batch(std::initializer_list<FillSpec> fills, hipStream_t st);
+inline FillSpec fill_of(int* p, int64_t n, int v) { return FillSpec{p, host_count((int)n), n, (unsigned)v, 4}; }
+inline FillSpec fill_of(int* p, Count n, int64_t cap, int v) { return FillSpec{p, n, cap, (unsigned)v, 4}; }
+inline FillSpec fill_of(int8_t* p, int64_t n, int8_t v) { return FillSpec{p, host_count((int)n), n, (uint8_t)v, 1}; }
+inline FillSpec fill_of(unsigned long long* p, int64_t n, unsigned long long v) {
+  return FillSpec{p, host_count((int)n), n, v, 8};
+}
 
 __global__ void k_pack_atoms(int ntot, const double* xyz, const double* rad, const double* dens_in,
                              double dens_default, const int* aoff, const int* nb0, int B, int full,

@@ -68,27 +68,23 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partial(const int* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int* __restrict__ part, int nb) {
-  int carry = 0;
-  for (int base = 0; base < nb; base += SCAN_THREADS) {
-    const int i = base + threadIdx.x;
-    const int v = i < nb ? part[i] : 0;
-    int tot;
-    const int e = block_excl_scan(v, &tot);
-    if (i < nb) part[i] = carry + e;
-    carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[nb] = carry;
-}
-
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restrict__ in, int* __restrict__ out,
                                                             Count cnt, int nb, const int* __restrict__ part) {
   const int64_t n = cnt.get();
   const int64_t chunk = scan_chunk(n, nb);
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
-  int carry = part[blockIdx.x];
+  // the block's carry: the totals of the blocks before it (nb <= 1024, so
+  // each block adds them up itself instead of a separate top-level pass)
+  int before = 0, all = 0;
+  for (int i = threadIdx.x; i < nb; i += SCAN_THREADS) {
+    const int v = part[i];
+    if (i < (int)blockIdx.x) before += v;
+    all += v;
+  }
+  int carry, total;
+  block_excl_scan(before, &carry);
+  block_excl_scan(all, &total);
   for (int64_t base = lo; base < hi; base += SCAN_TILE) {
     int v[SCAN_PER];
     scan_load(in, base, hi, v);
@@ -117,7 +113,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(const int* __restri
     }
     carry += tot;
   }
-  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = part[nb];
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = total;
 }
 
 // Small arrays: one workgroup of 1024 threads, each owning a contiguous run
@@ -161,7 +157,6 @@ void scan_excl(const int* in, int* out, Count n, int64_t cap, int* part, hipStre
   if (nb < 1) nb = 1;
   if (nb > 1024) nb = 1024;
   k_scan_partial<<<nb, SCAN_THREADS, 0, st>>>(in, n, nb, part);
-  k_scan_top<<<1, SCAN_THREADS, 0, st>>>(part, nb);
   k_scan_final<<<nb, SCAN_THREADS, 0, st>>>(in, out, n, nb, part);
 }
 
@@ -253,6 +248,40 @@ void fill_double(double* p, int64_t n, double v, hipStream_t st) {
   int g = (int)((n + 255) / 256);
   if (g > 4096) g = 4096;
   k_fill_double<<<g, 256, 0, st>>>(p, n, v);
+}
+
+struct FillArgs {
+  FillSpec e[FILL_BATCH];
+};
+__global__ void k_fill_batch(FillArgs f) {
+  const FillSpec& s = f.e[blockIdx.y];
+  const int64_t c = s.n.get();
+  const int64_t n = c < s.cap ? c : s.cap;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (s.bytes == 1)
+      static_cast<uint8_t*>(s.p)[i] = (uint8_t)s.v;
+    else if (s.bytes == 4)
+      static_cast<uint32_t*>(s.p)[i] = (uint32_t)s.v;
+    else
+      static_cast<unsigned long long*>(s.p)[i] = s.v;
+  }
+}
+void fill_batch(std::initializer_list<FillSpec> fills, hipStream_t st) {
+  FillArgs a{};
+  int k = 0;
+  int64_t cap = 0;
+  for (const FillSpec& f : fills) {
+    if (f.cap <= 0) continue;
+    if (k == FILL_BATCH) {
+      // more entries than one launch holds: launch these, start again
+      k_fill_batch<<<dim3((unsigned)std::min<int64_t>((cap + 255) / 256, 4096), k), 256, 0, st>>>(a);
+      k = 0;
+      cap = 0;
+    }
+    a.e[k++] = f;
+    cap = std::max(cap, f.cap);
+  }
+  if (k) k_fill_batch<<<dim3((unsigned)std::min<int64_t>((cap + 255) / 256, 4096), k), 256, 0, st>>>(a);
 }
 
 // Pack (x, y, z) + radius into double4 and derive per-atom complex id and
