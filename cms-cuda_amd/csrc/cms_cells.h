@@ -73,9 +73,8 @@ __global__ void k_grid_offsets(int B, const int* cscan, GridInfo* grid);
 __global__ void k_cell_count_atoms(BatchView b, const GridInfo* grid, int* acell, int* ccount);
 __global__ void k_cell_scatter_atoms(int n, const int* acell, const int* cstart, int* cursor, int* catom,
                                      const double4* atom, double4* cxyz);
-__global__ void k_cell_count(BatchView b, const GridInfo* grid, int* acell, int* ccount);
-__global__ void k_cell_scatter(BatchView b, const GridInfo* grid, const int* acell, const int* cstart, int* cursor,
-                               int* catom, double4* cxyz);
+__global__ void k_cell_build(BatchView b, const GridInfo* grid, int* ccount, int* acell, int* arank, int* cstart,
+                             int* catom, double4* cxyz);
 __global__ void k_attention_cells(BatchView b, Settings s, int all_atoms, int full, const GridInfo* grid,
                                   const int* cstart, const int* catom, const double4* cxyz, int8_t* atten,
                                   int8_t* access, int* need, unsigned long long* coinc, int* status);

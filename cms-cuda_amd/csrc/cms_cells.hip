@@ -125,19 +125,24 @@ __global__ void k_cell_scatter_atoms(int n, const int* __restrict__ acell, const
   }
 }
 
-// The same for large batches, one workgroup per complex:
-// the (cell, molecule) counters and cursors live in LDS when the complex's
-// grid has at most CELL_LDS of them (every grid of the benchmark batches),
-// otherwise device atomics on the zero-filled ccount / cursor arrays.  The
-// order of atoms within a cell is whatever the atomics give; nothing that
-// reads the lists depends on it.
 constexpr int CELL_LDS = 4096;
 constexpr int CELL_NT = 512;    // threads per complex
 constexpr int CELL_U = 4;       // atoms per thread per round (loads in flight)
 
-__global__ __launch_bounds__(CELL_NT) void k_cell_count(BatchView b, const GridInfo* __restrict__ grid,
-                                                        int* __restrict__ acell, int* __restrict__ ccount) {
+// Count, scan and scatter of the large-batch cell lists in one pass, one
+// workgroup per complex.  A complex's atoms occupy the atom slots
+// aoff[c] .. aoff[c + 1] and its cell slots are consecutive, so the cell
+// starts are the complex's atom offset plus a scan over its own cells: no
+// batch-wide scan.  Each atom's rank within its cell comes from the
+// counting atomic (LDS counters when the grid has at most CELL_LDS cell
+// slots, otherwise device atomics on the zero-filled ccount); within a cell
+// the order is whatever the atomics give, which nothing reads.
+__global__ __launch_bounds__(CELL_NT) void k_cell_build(BatchView b, const GridInfo* __restrict__ grid,
+                                                        int* __restrict__ ccount, int* __restrict__ acell,
+                                                        int* __restrict__ arank, int* __restrict__ cstart,
+                                                        int* __restrict__ catom, double4* __restrict__ cxyz) {
   __shared__ int cnt[CELL_LDS];
+  __shared__ int wsum[CELL_NT / WAVE];
   for (int c = blockIdx.x; c < b.B; c += gridDim.x) {
     const GridInfo gi = grid[c];
     const int ns = 2 * gi.nx * gi.ny * gi.nz;
@@ -146,8 +151,9 @@ __global__ __launch_bounds__(CELL_NT) void k_cell_count(BatchView b, const GridI
     if (lds)
       for (int k = threadIdx.x; k < ns; k += CELL_NT) cnt[k] = 0;
     __syncthreads();
-    const int a1 = b.aoff[c + 1];
-    for (int g0 = b.aoff[c] + threadIdx.x; g0 < a1; g0 += CELL_U * CELL_NT) {
+    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
+    // ranks
+    for (int g0 = a0 + threadIdx.x; g0 < a1; g0 += CELL_U * CELL_NT) {
       double4 a[CELL_U];
       int8_t mol[CELL_U];
 #pragma unroll
@@ -162,65 +168,61 @@ __global__ __launch_bounds__(CELL_NT) void k_cell_count(BatchView b, const GridI
           int ix, iy, iz;
           cell_coords(gi, a[u].x, a[u].y, a[u].z, &ix, &iy, &iz);
           const int k = 2 * ((ix * gi.ny + iy) * gi.nz + iz) + mol[u];
-          acell[g] = base + k;
-          if (lds) atomicAdd(&cnt[k], 1);
-          else atomicAdd(&ccount[base + k], 1);
+          acell[g] = k;
+          arank[g] = lds ? atomicAdd(&cnt[k], 1) : atomicAdd(&ccount[base + k], 1);
         }
       }
     }
     __syncthreads();
-    if (lds)
-      for (int k = threadIdx.x; k < ns; k += CELL_NT) ccount[base + k] = cnt[k];
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(CELL_NT) void k_cell_scatter(BatchView b, const GridInfo* __restrict__ grid,
-                                                          const int* __restrict__ acell,
-                                                          const int* __restrict__ cstart, int* __restrict__ cursor,
-                                                          int* __restrict__ catom, double4* __restrict__ cxyz) {
-  __shared__ int cur[CELL_LDS];
-  for (int c = blockIdx.x; c < b.B; c += gridDim.x) {
-    const GridInfo gi = grid[c];
-    const int ns = 2 * gi.nx * gi.ny * gi.nz;
-    const int base = 2 * gi.cell0;
-    const bool lds = ns <= CELL_LDS;
-    if (lds)
-      for (int k = threadIdx.x; k < ns; k += CELL_NT) cur[k] = cstart[base + k];
-    __syncthreads();
-    const int a0 = b.aoff[c], a1 = b.aoff[c + 1];
-    for (int g0 = a0 + threadIdx.x; g0 < a1; g0 += CELL_U * CELL_NT) {
-      int cell[CELL_U];
-#pragma unroll
-      for (int u = 0; u < CELL_U; ++u) {
-        const int g = g0 + u * CELL_NT;
-        cell[u] = g < a1 ? acell[g] : 0;
+    // cell starts: the complex's atom offset + an exclusive scan of its
+    // counts (kept in cnt / ccount for the placement)
+    int carry = a0;
+    for (int k0 = 0; k0 < ns; k0 += CELL_NT) {
+      const int k = k0 + threadIdx.x;
+      const int v = k < ns ? (lds ? cnt[k] : __hip_atomic_load(&ccount[base + k], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT))
+                           : 0;
+      const int inc = wave_incl_scan(v);
+      if (lane() == WAVE - 1) wsum[threadIdx.x / WAVE] = inc;
+      __syncthreads();
+      int off = 0, tot = 0;
+      for (int w = 0; w < CELL_NT / WAVE; ++w) {
+        off += w < (int)(threadIdx.x / WAVE) ? wsum[w] : 0;
+        tot += wsum[w];
       }
-#pragma unroll
-      for (int u = 0; u < CELL_U; ++u) {
-        const int g = g0 + u * CELL_NT;
-        if (g < a1) {
-          const int pos = lds ? atomicAdd(&cur[cell[u] - base], 1) : cstart[cell[u]] + atomicAdd(&cursor[cell[u]], 1);
-          catom[pos] = g;
-        }
+      if (k < ns) {
+        const int start = carry + off + inc - v;
+        cstart[base + k] = start;
+        if (lds)
+          cnt[k] = start;
+        else
+          ccount[base + k] = start;
       }
+      carry += tot;
+      __syncthreads();
     }
-    // the complex's slots hold its atoms at cstart[base] .. + atom count;
-    // their coordinates are copied in slot order (coalesced stores) once
-    // this workgroup's index stores are visible to it
+    // the slot after the complex's last one: the next complex's first
+    // start, and after the last complex the batch total
+    if (threadIdx.x == 0) cstart[base + ns] = a1;
     __syncthreads();
-    const int p0 = cstart[base], np = a1 - a0;
-    for (int t0 = threadIdx.x; t0 < np; t0 += CELL_U * CELL_NT) {
+    for (int g = a0 + threadIdx.x; g < a1; g += CELL_NT) {
+      const int k = acell[g];
+      catom[(lds ? cnt[k] : ccount[base + k]) + arank[g]] = g;
+    }
+    // coordinates in slot order (coalesced stores) once this workgroup's
+    // index stores are visible to it
+    __syncthreads();
+    for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += CELL_U * CELL_NT) {
       int src[CELL_U];
 #pragma unroll
       for (int u = 0; u < CELL_U; ++u) {
         const int t = t0 + u * CELL_NT;
-        src[u] = t < np ? catom[p0 + t] : 0;
+        src[u] = t < a1 ? catom[t] : 0;
       }
 #pragma unroll
       for (int u = 0; u < CELL_U; ++u) {
         const int t = t0 + u * CELL_NT;
-        if (t < np) cxyz[p0 + t] = b.atom[src[u]];
+        if (t < a1) cxyz[t] = b.atom[src[u]];
       }
     }
     __syncthreads();

@@ -75,7 +75,7 @@ struct cms_ctx {
   int nR = 0, nP = 0, nQ = 0, nD = 0;
   std::vector<int> h_dgrp, h_prgrp;
   DBuf atten, access, status, coinc, rsel, rseln, need, rpos, ratom, part;
-  DBuf grid, gframe, ncellc, ncscan, acell, ccount, cstart, ccur, catom, cxyz;
+  DBuf grid, gframe, ncellc, ncscan, acell, arank, ccount, cstart, ccur, catom, cxyz;
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
@@ -286,7 +286,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
   return {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
                   &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
                   &ctx->rsel, &ctx->rseln, &ctx->grid, &ctx->gframe, &ctx->ncellc,
-                  &ctx->ncscan, &ctx->acell, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
+                  &ctx->ncscan, &ctx->acell, &ctx->arank, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag, &ctx->pstage, &ctx->nps,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
@@ -719,6 +719,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CK(ctx->ncellc.ensure(sizeof(int) * (B + 1)));
   CK(ctx->ncscan.ensure(sizeof(int) * (B + 1)));
   CK(ctx->acell.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->arank.ensure(sizeof(int) * (n + 1)));
   CK(ctx->catom.ensure(sizeof(int) * (n + 1)));
   CK(ctx->cxyz.ensure(sizeof(double4) * (n + 1)));
   CK(ctx->ccount.ensure(sizeof(int) * (cell_cap + 1)));
@@ -752,20 +753,18 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     // attention sweep (cell data shared within a wavefront) and one
     // wavefront per listed non-blocker in the region sweep
     const bool per_thread = ctx->cell_sweep ? ctx->cell_sweep == 2 : n >= 200000;
-    if (per_thread)
-      k_cell_count<<<grid_for(B, 1, 65536), 512, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->ccount.as<int>());
-    else
+    if (per_thread) {
+      TIMED("cell_lists", k_cell_build<<<grid_for(B, 1, 65536), 512, 0, st>>>(
+                              b, grid, ctx->ccount.as<int>(), ctx->acell.as<int>(), ctx->arank.as<int>(),
+                              ctx->cstart.as<int>(), ctx->catom.as<int>(), ctx->cxyz.as<double4>()));
+    } else {
       k_cell_count_atoms<<<grid_for(n, 256, 4096), 256, 0, st>>>(b, grid, ctx->acell.as<int>(),
                                                                  ctx->ccount.as<int>());
-    scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cCells, cell_cap, part, st);
-    if (per_thread)
-      k_cell_scatter<<<grid_for(B, 1, 65536), 512, 0, st>>>(b, grid, ctx->acell.as<int>(), ctx->cstart.as<int>(),
-                                                             ctx->ccur.as<int>(), ctx->catom.as<int>(),
-                                                             ctx->cxyz.as<double4>());
-    else
+      scan_excl(ctx->ccount.as<int>(), ctx->cstart.as<int>(), cCells, cell_cap, part, st);
       k_cell_scatter_atoms<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->acell.as<int>(), ctx->cstart.as<int>(),
                                                                    ctx->ccur.as<int>(), ctx->catom.as<int>(),
                                                                    ctx->atom.as<double4>(), ctx->cxyz.as<double4>());
+    }
     const int* cst = ctx->cstart.as<int>();
     const int* cat = ctx->catom.as<int>();
     const double4* cxyz = ctx->cxyz.as<double4>();
