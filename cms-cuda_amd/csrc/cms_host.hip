@@ -85,7 +85,7 @@ struct cms_ctx {
   bool lists_ready = false;
   int nB = 0, nA = 0;
   DBuf bflag, bscan, blist, bxyz, bgrp, axyz, agrpd;
-  DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm;
+  DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm, omulti, onmulti;
   DBuf trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
   DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;   // frame: cms_nearest's point frame
@@ -296,7 +296,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
                   &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
-                  &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm,
+                  &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm, &ctx->omulti, &ctx->onmulti,
                   &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
                   &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
@@ -1156,7 +1156,11 @@ static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* 
   CK(ctx->ocnt.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ostart.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ocur.ensure(sizeof(int) * (n + 1)));
-  fill_batch({fill_of(ctx->ocnt.as<int>(), n, 0), fill_of(ctx->ocur.as<int>(), n, 0)}, st);
+  CK(ctx->omulti.ensure(sizeof(int) * (n + 1)));
+  CK(ctx->onmulti.ensure(sizeof(int)));
+  fill_batch({fill_of(ctx->ocnt.as<int>(), n, 0), fill_of(ctx->ocur.as<int>(), n, 0),
+              fill_of(ctx->onmulti.as<int>(), 1, 0)},
+             st);
   const int ig = grid_for(std::max<int64_t>(est_items, 1), 256, 4096);
   k_owner_count<<<ig, 256, 0, st>>>(nitems, list, ctx->datom.as<int>(), ctx->ocnt.as<int>());
   scan_excl(ctx->ocnt.as<int>(), ctx->ostart.as<int>(), n, ctx->part.as<int>(), st);
@@ -1164,10 +1168,10 @@ static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* 
                                       ctx->ocur.as<int>(), ctx->operm.as<int>());
   if (n > 0) {
     k_owner_small<<<grid_for(n, 256, 4096), 256, 0, st>>>(n, ctx->ostart.as<int>(), ctx->operm.as<int>(), val, vstride,
-                                                           per_atom);
-    TIMED("owner_sums", k_owner_sums<<<grid_for(n, 64, 16384), 64, 0, st>>>(n, ctx->ostart.as<int>(),
-                                                                           ctx->operm.as<int>(), val, vstride,
-                                                                           per_atom));
+                                                           per_atom, ctx->omulti.as<int>(), ctx->onmulti.as<int>());
+    TIMED("owner_sums", k_owner_sums<<<grid_for(std::max<int64_t>(est_items / 2, 1), 1, 16384), 64, 0, st>>>(
+                            ctx->omulti.as<int>(), ctx->onmulti.as<int>(), ctx->ostart.as<int>(),
+                            ctx->operm.as<int>(), val, vstride, per_atom));
   }
   CKL();
   // np.sum of the per-atom values of every molecule group; gstride 1 keeps

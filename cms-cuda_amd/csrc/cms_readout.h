@@ -31,8 +31,8 @@ __global__ void k_owner_count(Count n, const int* list, const int* datom, int* c
 __global__ void k_owner_scatter(Count n, const int* list, const int* datom, const int* start, int* cur,
                                 int* perm);
 __global__ void k_owner_small(int natoms, const int* start, const int* perm, const double* val, int64_t vstride,
-                              double* out);
-__global__ void k_owner_sums(int natoms, const int* start, int* perm, const double* val,
+                              double* out, int* multi, int* nmulti);
+__global__ void k_owner_sums(const int* multi, const int* nmulti, const int* start, int* perm, const double* val,
                              int64_t vstride, double* out);
 __global__ void k_group_np_sum(int G, const int* start, const double* vals, int64_t stride, double* out);
 __global__ void k_trim_flag(Count n, const double* d2acc, double band2, int* flag);

@@ -159,22 +159,55 @@ __global__ void k_owner_scatter(Count cnt, const int* __restrict__ list, const i
 // bitonic network in LDS -- their values gathered in that order and summed
 // left to right by lane 0.
 // Atoms with no item (0.0) or one (0.0 + value), one thread each.
-__global__ void k_owner_small(int natoms, const int* __restrict__ start, const int* __restrict__ perm,
-                              const double* __restrict__ val, int64_t vstride, double* __restrict__ out) {
-  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < natoms; a += gridDim.x * blockDim.x) {
-    const int lo = start[a], n = start[a + 1] - lo;
-    if (n <= 1) out[a] = n == 1 ? 0.0 + val[(int64_t)perm[lo] * vstride] : 0.0;
+// Atoms with at most one item, one thread each; the others are listed
+// (multi, *nmulti of them, in any order) for k_owner_sums, gathered per
+// workgroup in LDS and appended with one atomic per workgroup and flush.
+constexpr int OWN_LIST = 1024;
+__global__ __launch_bounds__(256) void k_owner_small(int natoms, const int* __restrict__ start,
+                                                     const int* __restrict__ perm, const double* __restrict__ val,
+                                                     int64_t vstride, double* __restrict__ out,
+                                                     int* __restrict__ multi, int* __restrict__ nmulti) {
+  __shared__ int lst[OWN_LIST];
+  __shared__ int nl, base;
+  if (threadIdx.x == 0) nl = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t end = ((int64_t)natoms + blockDim.x - 1) / blockDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t - threadIdx.x < end; t += stride) {
+    const int a = (int)t;
+    bool many = false;
+    if (a < natoms) {
+      const int lo = start[a], n = start[a + 1] - lo;
+      if (n <= 1)
+        out[a] = n == 1 ? 0.0 + val[(int64_t)perm[lo] * vstride] : 0.0;
+      else
+        many = true;
+    }
+    if (many) lst[atomicAdd(&nl, 1)] = a;
+    __syncthreads();
+    // flush when another round could overflow the list, and at the end
+    const bool last = t - threadIdx.x + stride >= end;
+    if (nl > OWN_LIST - (int)blockDim.x || last) {
+      if (threadIdx.x == 0) base = nl ? atomicAdd(nmulti, nl) : 0;
+      __syncthreads();
+      for (int k = threadIdx.x; k < nl; k += blockDim.x) multi[base + k] = lst[k];
+      __syncthreads();
+      if (threadIdx.x == 0) nl = 0;
+      __syncthreads();
+    }
   }
 }
 constexpr int OWN_CAP = 1024;
-__global__ __launch_bounds__(64) void k_owner_sums(int natoms, const int* __restrict__ start,
-                                                   int* __restrict__ perm, const double* __restrict__ val,
-                                                   int64_t vstride, double* __restrict__ out) {
+__global__ __launch_bounds__(64) void k_owner_sums(const int* __restrict__ multi, const int* __restrict__ nmulti,
+                                                   const int* __restrict__ start, int* __restrict__ perm,
+                                                   const double* __restrict__ val, int64_t vstride,
+                                                   double* __restrict__ out) {
   __shared__ int key[OWN_CAP];
   __shared__ double v[OWN_CAP];
-  // atoms with at most one item are summed by k_owner_small
-  auto several = [&](int64_t a) { return start[a + 1] - start[a] >= 2; };
-  CMS_SPARSE_LOOP(a, natoms, several) {
+  // the atoms with two items or more, listed by k_owner_count (the others
+  // are summed by k_owner_small)
+  CMS_UNIT_LOOP(u, *nmulti) {
+    const int a = multi[u];
     const int lo = start[a], n = start[a + 1] - lo;
     if (n > OWN_CAP) {
       // very long segment: in-place insertion sort by one lane
