@@ -79,7 +79,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
+  DBuf elist, nelist, ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -252,6 +252,8 @@ static Work work(cms_ctx* ctx) {
   w.ccnt = ctx->ccnt.as<int>();
   w.cscan = ctx->cscan.as<int>();
   w.cmask = ctx->cmask.as<uint64_t>();
+  w.elist = ctx->elist.as<int>();
+  w.nelist = ctx->nelist.as<int>();
   w.cvf = ctx->cvf.as<double4>();
   w.t1cnt = ctx->t1cnt.as<int>();
   w.t1scan = ctx->t1scan.as<int>();
@@ -290,7 +292,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
                   &ctx->pscan, &ctx->pi, &ctx->pj, &ctx->pu, &ctx->pt, &ctx->pflag, &ctx->pstage, &ctx->nps,
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
-                  &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->ccnt, &ctx->cvf,
+                  &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->elist, &ctx->nelist, &ctx->ccnt, &ctx->cvf,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
                   &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->ccf, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
@@ -553,6 +555,8 @@ static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->tflag.ensure(c + 1));
   CK(ctx->tmask.ensure(sizeof(uint64_t) * 2 * (c + 1)));
   CK(ctx->trel.ensure(sizeof(uint64_t) * (KCAP / WAVE) * (c + 1)));
+  CK(ctx->elist.ensure(sizeof(int) * (c + 1)));
+  CK(ctx->nelist.ensure(sizeof(int)));
   CK(ctx->tef.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tes.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tel.ensure(sizeof(int) * (c + 1)));
@@ -850,6 +854,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
       // the torus pass writes the toroidal counts of the pairs of rows with
       // a live toroid-queue pair only
       fill_batch({fill_of(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0), fill_of(ctx->rtor.as<int8_t>(), ctx->cap[M_R], 0),
+                  fill_of(ctx->nelist.as<int>(), 1, 0),
                   fill_of(ctx->t1cnt.as<int>(), cP, ctx->cap[M_P], 0),
                   fill_of(ctx->t2cnt.as<int>(), cP, ctx->cap[M_P], 0),
                   fill_of(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0)},
@@ -882,9 +887,14 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   }
   // the row pass staged the probes of large batches (pairs it could not
   // stage keep flag bit 8 for the emit pass)
-  if (est[M_P] > 0)
+  if (est[M_P] > 0) {
+    // the row pass listed those pairs; the per-pair count pass only flags them
+    Work we = w;
+    if (!by_row) we.elist = nullptr;
     TIMED("probes_emit", if (by_row) k_probe_place<<<grid_for(est[M_R] * PSLOTS, 256, 16384), 256, 0, st>>>(b, w);
-                         k_probes<true><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
+                         k_probes<true><<<by_row ? grid_for(est[M_P] / 256 + 1, 1, 4096) : grid_for(est[M_P], 1), 64,
+                                          0, st>>>(b, s, we));
+  }
   if (est[M_Q] > 0) TIMED("near", k_near<<<grid_for(est[M_Q], 1), 64, 0, st>>>(b, s, w));
   CKL();
   // ---- dot counts

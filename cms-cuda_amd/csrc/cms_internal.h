@@ -123,6 +123,8 @@ struct Work {
   int* nnear;
   ProbeRec* pstage;       // PSLOTS staged probe candidates per row (large batches)
   int* nps;               // per row: slots used
+  int* elist;             // per-row pass: pairs left to the emit pass (null: by flag)
+  int* nelist;            // their number
   int pslots;             // slots per row in use (<= PSLOTS)
   // dot counts
   int* ccnt;

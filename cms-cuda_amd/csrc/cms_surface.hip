@@ -181,10 +181,13 @@ __global__ __launch_bounds__(64, 4) void k_probes(BatchView b, Settings s, Work 
   __shared__ double4 lx[KCAP];
   constexpr int NCH = KCAP / WAVE;
   const int nP = *w.nP;
-  // the emit pass visits only pairs with surviving probes (flag bit 8)
-  auto live = [&](int64_t q) { return !EMIT || (w.pflag[q] & 8) != 0; };
+  // the emit pass visits only pairs with surviving probes (flag bit 8); after
+  // the per-row count pass those pairs are listed (elist, a few)
+  const bool listed = EMIT && w.elist != nullptr;
+  auto live = [&](int64_t q) { return listed || !EMIT || (w.pflag[q] & 8) != 0; };
   int staged = -1;   // neighbour row held in lj / lx (consecutive pairs share it)
-  CMS_SPARSE_LOOP(p, nP, live) {
+  CMS_SPARSE_LOOP(u, listed ? *w.nelist : nP, live) {
+    const int64_t p = listed ? w.elist[u] : u;
     const int i = w.pi[p], j = w.pj[p];
     const int grp = group_of_atom(b, i);
     const int g0 = w.pgrp[grp], g1 = w.pgrp[grp + 1];
@@ -760,7 +763,10 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
         } else if (!(f & PB_KILLED)) {
           np = bnp[lane()];
           nm = bnm[lane()];
-          if (np + nm > 0 && (f & PB_EMIT)) pf |= 8;
+          if (np + nm > 0 && (f & PB_EMIT)) {
+            pf |= 8;
+            w.elist[atomicAdd(w.nelist, 1)] = p;   // a pair with a probe past the slots: rare
+          }
         }
         // the pair record (what the pair pass's emit would write); the
         // emit passes read the geometry only of pairs with probes, the
