@@ -90,7 +90,7 @@ struct cms_ctx {
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
   DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;   // frame: cms_nearest's point frame
   DBuf bidx, bgof, adj, eatom, evpos, chk, qlast, tqidx;
-  DBuf tgrid, tncell, tncscan, tacell, tcount, tstart, tcur, tsorted;   // trim grid
+  DBuf tgrid, tncell, tncscan, tcount, tstart, tsorted;   // trim grid
   bool cms_ready = false;
   bool sc_ready = false;            // trimmed lists of the current surface exist
   bool host_tgroups = false;        // h_tgrp holds the trimmed-list offsets
@@ -303,8 +303,8 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
                   &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
                   &ctx->bxyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
-                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->tqidx, &ctx->tgrid, &ctx->tncell, &ctx->tncscan, &ctx->tacell,
-          &ctx->tcount, &ctx->tstart, &ctx->tcur, &ctx->tsorted, &ctx->meta};
+                  &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->tqidx, &ctx->tgrid, &ctx->tncell, &ctx->tncscan,
+          &ctx->tcount, &ctx->tstart, &ctx->tsorted, &ctx->meta};
 }
 
 // Identity of the workspace: a captured graph holds these addresses.
@@ -592,13 +592,11 @@ static int size_dots(cms_ctx* ctx, int64_t c) {
                 &ctx->bgof};
   for (DBuf* b : i1) CK(b->ensure(sizeof(int) * (c + 1)));
   DBuf* f8[] = {&ctx->bmin, &ctx->wv, &ctx->tmin, &ctx->tdist, &ctx->trr};
-  CK(ctx->tacell.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tsorted.ensure(sizeof(double4) * (c + 1)));
   // trim-grid cells: at most 2 per accessible dot + 64 per group
-  const int64_t cells = 2 * c + 64ll * 2 * ctx->B + 1;
+  const int64_t cells = 2 * c + 65ll * 2 * ctx->B + 1;
   CK(ctx->tcount.ensure(sizeof(int) * (cells + 1)));
   CK(ctx->tstart.ensure(sizeof(int) * (cells + 1)));
-  CK(ctx->tcur.ensure(sizeof(int) * (cells + 1)));
   for (DBuf* b : f8) CK(b->ensure(sizeof(double) * (c + 1)));
   CK(ctx->dbur.ensure(c + 1));
   CK(ctx->dtype.ensure(c + 1));
@@ -1269,7 +1267,7 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   CK(ctx->tgrid.ensure(sizeof(GridInfo) * (G + 1)));
   CK(ctx->tncell.ensure(sizeof(int) * (G + 1)));
   CK(ctx->tncscan.ensure(sizeof(int) * (G + 1)));
-  const int64_t cells_cap = 2 * capD + 64ll * G;
+  const int64_t cells_cap = 2 * capD + 65ll * G;
   const int gB = grid_for(std::max<int64_t>(estB, 1), 256, 4096);
   GridInfo* tg = ctx->tgrid.as<GridInfo>();
   k_dot_grid_setup<<<grid_for(G, 1, 4096), 256, 0, st>>>(G, ctx->agrpd.as<int>(), ctx->bgrp.as<int>(),
@@ -1278,16 +1276,12 @@ static int sc_pass(cms_ctx* ctx, int precision, double* results) {
   scan_excl(ctx->tncell.as<int>(), ctx->tncscan.as<int>(), G, part, st);
   k_grid_offsets<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, ctx->tncscan.as<int>(), tg);
   const Count cCells = dev_count(ctx->tncscan.as<int>() + G);
-  fill_batch({fill_of(ctx->tcount.as<int>(), cCells, cells_cap, 0), fill_of(ctx->tcur.as<int>(), cCells, cells_cap, 0)},
-             st);
-  // the accessible dots read in place from the dot arrays
-  const Count cD = dev_count(meta + M_D);
-  const int gD = grid_for(std::max<int64_t>(ctx->est[M_D], 1), 256, 4096);
-  k_dot_cell_count<<<gD, 256, 0, st>>>(cD, ctx->dpos.as<double4>(), ctx->dbur.as<int8_t>(), ctx->dgrp.as<int>(), G,
-                                       tg, ctx->tacell.as<int>(), ctx->tcount.as<int>());
-  scan_excl(ctx->tcount.as<int>(), ctx->tstart.as<int>(), cCells, cells_cap, part, st);
-  k_dot_cell_scatter<<<gD, 256, 0, st>>>(cD, ctx->tacell.as<int>(), ctx->tstart.as<int>(), ctx->tcur.as<int>(),
-                                         ctx->dpos.as<double4>(), ctx->tsorted.as<double4>());
+  // (device counters only for grids too large for LDS)
+  fill_int(ctx->tcount.as<int>(), cCells, cells_cap, 0, st);
+  // the accessible dots read in place from the dot arrays, binned per group
+  TIMED("trim_grid", k_dot_grid_build<<<grid_for(G, 1, 65536), 256, 0, st>>>(
+                         G, ctx->dgrp.as<int>(), ctx->dpos.as<double4>(), ctx->dbur.as<int8_t>(), tg,
+                         ctx->tcount.as<int>(), ctx->tstart.as<int>(), ctx->tsorted.as<double4>()));
   {
     // lanes per buried dot: enough threads to fill the GPU for small
     // batches; four read a dot's candidates together (measured on

@@ -53,10 +53,8 @@ __global__ void k_nearest32(const float4* Q32, const double4* Q64, const int* qg
                             unsigned long long* nrescored);
 __global__ void k_dot_grid_setup(int G, const int* agrp, const int* bgrp, const double4* bxyz, double band,
                                  GridInfo* grid, int* ncell);
-__global__ void k_dot_cell_count(Count nd, const double4* dpos, const int8_t* dbur, const int* dgrp, int G,
-                                 const GridInfo* grid, int* acell, int* ccount);
-__global__ void k_dot_cell_scatter(Count nd, const int* acell, const int* cstart, int* cursor,
-                                   const double4* dpos, double4* sorted);
+__global__ void k_dot_grid_build(int G, const int* dgrp, const double4* dpos, const int8_t* dbur,
+                                 const GridInfo* grid, int* tcount, int* tstart, double4* sorted);
 template <int L>
 __global__ void k_trim_grid(Count nb, const double4* bxyz, const int* bgrp, int G, const GridInfo* grid,
                             const int* cstart, const double4* sorted, double band2, int* flag);

@@ -692,7 +692,8 @@ __global__ __launch_bounds__(256) void k_dot_grid_setup(int G, const int* __rest
       gi.cell0 = 0;
       gi.rmax = 0.0;
       grid[g] = gi;
-      ncell[g] = d[0] * d[1] * d[2];
+      // one slot past the grid's cells: the end of the group's last cell
+      ncell[g] = d[0] * d[1] * d[2] + 1;
     }
     __syncthreads();
   }
@@ -733,35 +734,76 @@ __device__ __forceinline__ bool dot_cell_in(const GridInfo& gi, double x, double
   return true;
 }
 
-// Cells of the accessible dots (all dots, the buried ones skipped; dgrp:
-// the dots' group offsets) inside their group's trim grid, -1 otherwise.
-__global__ void k_dot_cell_count(Count nd, const double4* __restrict__ dpos, const int8_t* __restrict__ dbur,
-                                 const int* __restrict__ dgrp, int G, const GridInfo* __restrict__ grid,
-                                 int* __restrict__ acell, int* __restrict__ ccount) {
-  const int n = nd.get();
-  CMS_WAVE_LOOP(i, n) {
-    int cell = -1;
-    if (i < n) {
-      const GridInfo gi = grid[group_of_item_wave(dgrp, G, (int)i, (int)(i - lane()))];
-      if (!dbur[i]) {
-        const double4 a = dpos[i];
-        int ix, iy, iz;
-        if (dot_cell_in(gi, a.x, a.y, a.z, &ix, &iy, &iz)) cell = gi.cell0 + (ix * gi.ny + iy) * gi.nz + iz;
-      }
-      acell[i] = cell;
+// The accessible dots of each group inside its trim grid, sorted by cell
+// (sorted[tstart[cell] .. tstart[cell + 1])), one workgroup per group: a
+// count per cell (LDS counters for grids of at most TG_LDS cells, otherwise
+// device atomics on the zero-filled tcount), a scan over the group's own
+// cells, the dots placed through the counters turned cursors.  A group's
+// sorted dots sit at its own dot offset dgrp[g] (there are at most as many
+// as it has dots), and the slot after its cells holds their end.  Within a
+// cell the order is whatever the atomics give; the trim test only asks
+// whether any dot lies within the band.
+constexpr int TG_LDS = 8192;
+constexpr int TG_NT = 256;
+__global__ __launch_bounds__(TG_NT) void k_dot_grid_build(int G, const int* __restrict__ dgrp,
+                                                          const double4* __restrict__ dpos,
+                                                          const int8_t* __restrict__ dbur,
+                                                          const GridInfo* __restrict__ grid,
+                                                          int* __restrict__ tcount, int* __restrict__ tstart,
+                                                          double4* __restrict__ sorted) {
+  __shared__ int cnt[TG_LDS];
+  __shared__ int wsum[TG_NT / WAVE];
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    const GridInfo gi = grid[g];
+    const int nc = gi.nx * gi.ny * gi.nz;
+    const int base = gi.cell0;
+    const bool lds = nc <= TG_LDS;
+    int* const c = lds ? cnt : tcount + base;
+    if (lds)
+      for (int k = threadIdx.x; k < nc; k += TG_NT) cnt[k] = 0;
+    __syncthreads();
+    const int d0 = dgrp[g], d1 = dgrp[g + 1];
+    auto cell_of_dot = [&](int d, double4* a) {
+      if (dbur[d]) return -1;
+      *a = dpos[d];
+      int ix, iy, iz;
+      return dot_cell_in(gi, a->x, a->y, a->z, &ix, &iy, &iz) ? (ix * gi.ny + iy) * gi.nz + iz : -1;
+    };
+    for (int d = d0 + threadIdx.x; d < d1; d += TG_NT) {
+      double4 a;
+      const int k = cell_of_dot(d, &a);
+      if (k >= 0) atomicAdd(&c[k], 1);
     }
-    wave_counter_inc(ccount, max(cell, 0), cell >= 0);
-  }
-}
-
-__global__ void k_dot_cell_scatter(Count nd, const int* __restrict__ acell, const int* __restrict__ cstart,
-                                   int* __restrict__ cursor, const double4* __restrict__ dpos,
-                                   double4* __restrict__ sorted) {
-  const int n = nd.get();
-  CMS_WAVE_LOOP(i, n) {
-    const int cell = i < n ? acell[i] : -1;
-    const int slot = wave_counter_add(cursor, max(cell, 0), cell >= 0);
-    if (cell >= 0) sorted[cstart[cell] + slot] = dpos[i];
+    __syncthreads();
+    int carry = d0;
+    for (int k0 = 0; k0 < nc; k0 += TG_NT) {
+      const int k = k0 + threadIdx.x;
+      const int v = k < nc ? (lds ? cnt[k] : __hip_atomic_load(&c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                           : 0;
+      const int inc = wave_incl_scan(v);
+      if (lane() == WAVE - 1) wsum[threadIdx.x / WAVE] = inc;
+      __syncthreads();
+      int off = 0, tot = 0;
+      for (int w = 0; w < TG_NT / WAVE; ++w) {
+        off += w < (int)(threadIdx.x / WAVE) ? wsum[w] : 0;
+        tot += wsum[w];
+      }
+      if (k < nc) {
+        const int start = carry + off + inc - v;
+        tstart[base + k] = start;
+        c[k] = start;
+      }
+      carry += tot;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tstart[base + nc] = carry;
+    __syncthreads();
+    for (int d = d0 + threadIdx.x; d < d1; d += TG_NT) {
+      double4 a;
+      const int k = cell_of_dot(d, &a);
+      if (k >= 0) sorted[atomicAdd(&c[k], 1)] = a;
+    }
+    __syncthreads();
   }
 }
 
