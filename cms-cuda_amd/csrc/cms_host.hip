@@ -1185,8 +1185,8 @@ static int owner_sums(cms_ctx* ctx, Count nitems, int64_t est_items, const int* 
   // np.sum of the per-atom values of every molecule group; gstride 1 keeps
   // all 2B group totals, gstride 2 only the molecule-0 ones (max-CMS)
   CK(ctx->gtot.ensure(sizeof(double) * (G_tot + 1)));
-  k_group_np_sum<<<grid_for(G_tot, 1, 4096), 256, 0, st>>>(G_tot, ctx->agrp.as<int>(), per_atom, 1,
-                                                            ctx->gtot.as<double>());
+  TIMED("group_sums", k_group_np_sum<<<grid_for(G_tot, 1, 4096), 256, 0, st>>>(G_tot, ctx->agrp.as<int>(),
+                                                                                per_atom, 1, ctx->gtot.as<double>()));
   k_pick_stride<<<grid_for(G_tot / gstride, 256, 4096), 256, 0, st>>>(G_tot / gstride, ctx->gtot.as<double>(),
                                                                        gstride, totals);
   CKL();

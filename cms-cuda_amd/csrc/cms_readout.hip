@@ -287,24 +287,30 @@ __device__ double np_leaf_sum(const double* a, int64_t stride, int n) {
   return res;
 }
 
-constexpr int PW_MAXN = 1024;   // tree nodes held in LDS (n up to ~32k values)
 constexpr int PW_MAXLEV = 32;
 
-// Block-wide np.sum of a[0..n) (stride in doubles), the pairwise tree of
-// np_leaf_sum leaves.  Wavefront 0 lays the tree out level by level (the
-// children of a level's split nodes, in order, form the next level), all
-// threads sum the leaves, and the internal nodes are added bottom-up one
-// level at a time as left + right -- the association of the recursion.  A
-// tree with more than PW_MAXN nodes is followed by one thread instead.
-// Returns the sum in every thread.
-__device__ double block_np_sum(const double* a, int64_t stride, int n) {
-  __shared__ int nd_lo[PW_MAXN], nd_m[PW_MAXN], nd_ch[PW_MAXN];
-  __shared__ double nd_v[PW_MAXN];
+// Block-wide np.sum of K arrays of the same length n (a[k][i * stride[k]]),
+// each the pairwise tree of np_leaf_sum leaves; the tree depends only on n,
+// so it is laid out once for the K arrays.  Wavefront 0 lays it out level by
+// level (the children of a level's split nodes, in order, form the next
+// level); each leaf is summed by 8 lanes, lane j holding the accumulator of
+// the values j, j + 8, ... (at most 16, all loaded before the first add),
+// the 8 accumulators combined by a butterfly that forms
+// ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) (addition commutes
+// exactly), the leaf's last m % 8 values added in order by its first lane;
+// the internal nodes are added bottom-up one level at a time as left + right
+// -- the association of the recursion.  A tree with more than MAXN nodes is
+// followed by one thread instead.  Every thread gets the sums.  Expects
+// blockDim.x a multiple of WAVE.
+template <int K, int MAXN>
+__device__ void block_np_sums(const double* const (&a)[K], const int64_t (&stride)[K], int n, double (&out)[K]) {
+  __shared__ int nd_lo[MAXN], nd_m[MAXN], nd_ch[MAXN];
+  __shared__ double nd_v[K][MAXN];
   __shared__ int lev_start[PW_MAXLEV + 1];
   __shared__ int nlev, overflow;
   __shared__ int stk_n[64], stk_lo[64], stk_v[64];
   __shared__ double vstk[64];
-  __shared__ double result;
+  __shared__ double result[K];
   if (threadIdx.x < WAVE) {
     int s0 = 0, cnt = 1, lev = 0, of = 0;
     if (lane() == 0) {
@@ -324,7 +330,7 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
         const int inc = wave_incl_scan(c);
         const int pos = next + ncnt + inc - c;
         if (k < cnt) nd_ch[s0 + k] = split ? pos : -1;
-        if (split && pos + 1 < PW_MAXN) {
+        if (split && pos + 1 < MAXN) {
           const int lo = nd_lo[s0 + k];
           int n2 = m / 2;
           n2 -= n2 % 8;
@@ -336,7 +342,7 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
         ncnt += __shfl(inc, WAVE - 1, WAVE);
       }
       if (ncnt == 0) break;
-      if (next + ncnt > PW_MAXN || lev + 1 >= PW_MAXLEV) { of = 1; break; }
+      if (next + ncnt > MAXN || lev + 1 >= PW_MAXLEV) { of = 1; break; }
       s0 = next;
       cnt = ncnt;
       ++lev;
@@ -352,46 +358,81 @@ __device__ double block_np_sum(const double* a, int64_t stride, int n) {
   if (overflow) {
     // very long arrays: one thread follows the same tree
     if (threadIdx.x == 0) {
-      int sp = 0, vp = 0;
-      stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
-      while (sp >= 0) {
-        const int m = stk_n[sp], lo = stk_lo[sp], v = stk_v[sp];
-        --sp;
-        if (m <= 128) {
-          vstk[vp++] = np_leaf_sum(a + (int64_t)lo * stride, stride, m);
-        } else if (!v) {
-          int n2 = m / 2;
-          n2 -= n2 % 8;
-          ++sp; stk_n[sp] = m; stk_lo[sp] = lo; stk_v[sp] = 1;
-          ++sp; stk_n[sp] = m - n2; stk_lo[sp] = lo + n2; stk_v[sp] = 0;
-          ++sp; stk_n[sp] = n2; stk_lo[sp] = lo; stk_v[sp] = 0;
-        } else {
-          const double b2 = vstk[--vp];
-          const double b1 = vstk[--vp];
-          vstk[vp++] = b1 + b2;
+      for (int q = 0; q < K; ++q) {
+        int sp = 0, vp = 0;
+        stk_n[0] = n; stk_lo[0] = 0; stk_v[0] = 0;
+        while (sp >= 0) {
+          const int m = stk_n[sp], lo = stk_lo[sp], v = stk_v[sp];
+          --sp;
+          if (m <= 128) {
+            vstk[vp++] = np_leaf_sum(a[q] + (int64_t)lo * stride[q], stride[q], m);
+          } else if (!v) {
+            int n2 = m / 2;
+            n2 -= n2 % 8;
+            ++sp; stk_n[sp] = m; stk_lo[sp] = lo; stk_v[sp] = 1;
+            ++sp; stk_n[sp] = m - n2; stk_lo[sp] = lo + n2; stk_v[sp] = 0;
+            ++sp; stk_n[sp] = n2; stk_lo[sp] = lo; stk_v[sp] = 0;
+          } else {
+            const double b2 = vstk[--vp];
+            const double b1 = vstk[--vp];
+            vstk[vp++] = b1 + b2;
+          }
         }
+        result[q] = vstk[0];
       }
-      result = vstk[0];
     }
     __syncthreads();
-    const double r = result;
+    for (int q = 0; q < K; ++q) out[q] = result[q];
     __syncthreads();
-    return r;
+    return;
   }
   const int nodes = lev_start[nlev];
-  for (int k = threadIdx.x; k < nodes; k += blockDim.x)
-    if (nd_ch[k] < 0) nd_v[k] = np_leaf_sum(a + (int64_t)nd_lo[k] * stride, stride, nd_m[k]);
+  const int sub = threadIdx.x & 7;
+  const int lbase = lane() & ~7;
+  for (int k0 = 0; k0 < nodes; k0 += blockDim.x / 8) {
+    // uniform trip count: every lane reaches the shuffles
+    const int k = k0 + (int)(threadIdx.x >> 3);
+    const bool leaf = k < nodes && nd_ch[k] < 0;
+    const int lo = leaf ? nd_lo[k] : 0, m = leaf ? nd_m[k] : 0;
+    const int m8 = m < 8 ? 0 : m - m % 8;
+    const int cnt = m8 / 8;   // values per lane, at most 16
+    const int tcnt = m - m8;  // the leaf's last values, added in order
+    double v[K][16], tv[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) v[q][t] = t < cnt ? a[q][(int64_t)(lo + 8 * t + sub) * stride[q]] : 0.0;
+      tv[q] = sub < tcnt ? a[q][(int64_t)(lo + m8 + sub) * stride[q]] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      double r = v[q][0];
+#pragma unroll
+      for (int t = 1; t < 16; ++t)
+        if (t < cnt) r += v[q][t];
+      r += __shfl_xor(r, 1, WAVE);
+      r += __shfl_xor(r, 2, WAVE);
+      r += __shfl_xor(r, 4, WAVE);
+      double res = m < 8 ? 0.0 : r;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const double x = __shfl(tv[q], lbase + j, WAVE);
+        if (j < tcnt) res += x;
+      }
+      if (leaf && sub == 0) nd_v[q][k] = res;
+    }
+  }
   __syncthreads();
   for (int lev = nlev - 2; lev >= 0; --lev) {
     for (int k = lev_start[lev] + threadIdx.x; k < lev_start[lev + 1]; k += blockDim.x) {
       const int c = nd_ch[k];
-      if (c >= 0) nd_v[k] = nd_v[c] + nd_v[c + 1];
+      if (c >= 0)
+        for (int q = 0; q < K; ++q) nd_v[q][k] = nd_v[q][c] + nd_v[q][c + 1];
     }
     __syncthreads();
   }
-  const double r = nd_v[0];
+  for (int q = 0; q < K; ++q) out[q] = nd_v[q][0];
   __syncthreads();
-  return r;
 }
 
 // out[g] = np.sum(vals[start[g] .. start[g+1]))
@@ -400,8 +441,11 @@ __global__ __launch_bounds__(256) void k_group_np_sum(int G, const int* __restri
                                                       double* __restrict__ out) {
   for (int g = blockIdx.x; g < G; g += gridDim.x) {
     const int lo = start[g], hi = start[g + 1];
-    const double s = block_np_sum(vals + (int64_t)lo * stride, stride, hi - lo);
-    if (threadIdx.x == 0) out[g] = s;
+    const double* a[1] = {vals + (int64_t)lo * stride};
+    const int64_t st[1] = {stride};
+    double s[1];
+    block_np_sums<1, 1024>(a, st, hi - lo, s);
+    if (threadIdx.x == 0) out[g] = s[0];
   }
 }
 
@@ -460,109 +504,182 @@ __global__ void k_group_of_items(int G, const int* __restrict__ start, int* __re
 // ------------------------------------------------------------ binned median
 // Grouped-data median of v[0..n) (_core.py:1080-1104): bins floor(v/bw),
 // first bin whose cumulative percentage reaches 50, linear interpolation.
-constexpr int HBINS = 4096;
-__device__ double block_binned_median(const double* v, int n, double bw) {
-  __shared__ long long smin, smax;
-  __shared__ int hist[HBINS];
-  __shared__ int wsum[4];
-  __shared__ long long found_bin;
-  __shared__ long long found_cum, found_cnt;
-  if (n == 0) return 0.0;
-  if (threadIdx.x == 0) { smin = LLONG_MAX; smax = LLONG_MIN; found_bin = LLONG_MAX; }
-  __syncthreads();
-  long long lmin = LLONG_MAX, lmax = LLONG_MIN;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const long long b = (long long)floor(v[i] / bw);
-    lmin = b < lmin ? b : lmin;
-    lmax = b > lmax ? b : lmax;
+// K arrays of the same length n at once (their passes share the loads'
+// latency and the barriers).  Histograms of up to HBINS bins in LDS, a
+// binary search on the bin value with block-wide counts for a wider range.
+// Expects blockDim.x == 256 and n > 0.
+constexpr int HBINS = 1024;
+constexpr int MED_UNROLL = 4;
+template <int K>
+__device__ void block_binned_medians(const double* const (&v)[K], const double (&bw)[K], int n, double (&med)[K]) {
+  static_assert(HBINS % 256 == 0, "bins per thread");
+  __shared__ long long smin[K], smax[K];
+  __shared__ int hist[K][HBINS];
+  __shared__ int wsum[K][4];
+  __shared__ unsigned long long fkey[K];
+  __shared__ int bcount[K];
+  if (threadIdx.x < K) {
+    smin[threadIdx.x] = LLONG_MAX;
+    smax[threadIdx.x] = LLONG_MIN;
+    fkey[threadIdx.x] = ~0ull;
   }
-  atomicMin(&smin, lmin);
-  atomicMax(&smax, lmax);
+  long long lmin[K], lmax[K];
+  for (int q = 0; q < K; ++q) { lmin[q] = LLONG_MAX; lmax[q] = LLONG_MIN; }
+  for (int i0 = threadIdx.x; i0 < n; i0 += MED_UNROLL * blockDim.x) {
+    double x[K][MED_UNROLL];
+#pragma unroll
+    for (int u = 0; u < MED_UNROLL; ++u) {
+      const int i = i0 + u * blockDim.x;
+#pragma unroll
+      for (int q = 0; q < K; ++q) x[q][u] = i < n ? v[q][i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < MED_UNROLL; ++u)
+      if (i0 + u * blockDim.x < n)
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const long long b = (long long)floor(x[q][u] / bw[q]);
+          lmin[q] = b < lmin[q] ? b : lmin[q];
+          lmax[q] = b > lmax[q] ? b : lmax[q];
+        }
+  }
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const long long a = __shfl_xor(lmin[q], o, WAVE), b = __shfl_xor(lmax[q], o, WAVE);
+      lmin[q] = a < lmin[q] ? a : lmin[q];
+      lmax[q] = b > lmax[q] ? b : lmax[q];
+    }
   __syncthreads();
-  const long long bmin = smin, bmax = smax;
+  if (lane() == 0)
+    for (int q = 0; q < K; ++q) {
+      atomicMin(&smin[q], lmin[q]);
+      atomicMax(&smax[q], lmax[q]);
+    }
+  __syncthreads();
+  long long bmin[K];
+  bool fits[K];
+  bool any = false;
+  for (int q = 0; q < K; ++q) {
+    bmin[q] = smin[q];
+    fits[q] = smax[q] - bmin[q] + 1 <= HBINS;
+    any = any || fits[q];
+  }
   const double pct = 100.0 / (double)n;
-  if (bmax - bmin + 1 <= HBINS) {
-    const int nbin = (int)(bmax - bmin + 1);
-    for (int i = threadIdx.x; i < HBINS; i += blockDim.x) hist[i] = 0;
+  long long fbin[K], fcum[K], fcnt[K];
+  if (any) {
+    for (int q = 0; q < K; ++q)
+      if (fits[q])
+        for (int i = threadIdx.x; i < HBINS; i += blockDim.x) hist[q][i] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const long long b = (long long)floor(v[i] / bw);
-      atomicAdd(&hist[b - bmin], 1);
+    for (int i0 = threadIdx.x; i0 < n; i0 += MED_UNROLL * blockDim.x) {
+      double x[K][MED_UNROLL];
+#pragma unroll
+      for (int u = 0; u < MED_UNROLL; ++u) {
+        const int i = i0 + u * blockDim.x;
+#pragma unroll
+        for (int q = 0; q < K; ++q) x[q][u] = i < n && fits[q] ? v[q][i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < MED_UNROLL; ++u)
+        if (i0 + u * blockDim.x < n)
+#pragma unroll
+          for (int q = 0; q < K; ++q)
+            if (fits[q]) atomicAdd(&hist[q][(long long)floor(x[q][u] / bw[q]) - bmin[q]], 1);
     }
     __syncthreads();
-    const int per = HBINS / 256;
+    // each thread scans its HBINS / 256 consecutive bins; the first bin whose
+    // cumulative count reaches 50 % is kept with its count as one key
+    constexpr int per = HBINS / 256;
     const int b0 = threadIdx.x * per;
-    int loc = 0;
-    for (int k = 0; k < per; ++k) loc += hist[b0 + k];
-    const int inc = wave_incl_scan(loc);
     const int wid = threadIdx.x / WAVE;
-    if (lane() == WAVE - 1) wsum[wid] = inc;
+    int loc[K], inc[K];
+    for (int q = 0; q < K; ++q) {
+      loc[q] = 0;
+      if (fits[q])
+        for (int k = 0; k < per; ++k) loc[q] += hist[q][b0 + k];
+      inc[q] = wave_incl_scan(loc[q]);
+      if (lane() == WAVE - 1) wsum[q][wid] = inc[q];
+    }
     __syncthreads();
-    int cum = inc - loc;
-    for (int k = 0; k < wid; ++k) cum += wsum[k];
-    for (int k = 0; k < per; ++k) {
-      const int b = b0 + k;
-      const int c = hist[b];
-      if (c > 0 && b < nbin) {
-        cum += c;
-        if ((double)cum * pct >= 50.0) {
-          atomicMin(&found_bin, (long long)b);
-          break;
+    for (int q = 0; q < K; ++q) {
+      if (!fits[q]) continue;
+      const int nbin = (int)(smax[q] - bmin[q] + 1);
+      int cum = inc[q] - loc[q];
+      for (int k = 0; k < wid; ++k) cum += wsum[q][k];
+      for (int k = 0; k < per; ++k) {
+        const int b = b0 + k;
+        const int c = hist[q][b];
+        if (c > 0 && b < nbin) {
+          cum += c;
+          if ((double)cum * pct >= 50.0) {
+            atomicMin(&fkey[q], ((unsigned long long)b << 32) | (unsigned)cum);
+            break;
+          }
         }
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      long long fb = found_bin;
-      if (fb == LLONG_MAX) fb = nbin - 1;  // no bin reaches 50%: last bin
-      long long cc = 0;
-      for (int b = 0; b <= fb; ++b) cc += hist[b];
-      found_bin = fb + bmin;
-      found_cum = cc;
-      found_cnt = hist[fb];
+    for (int q = 0; q < K; ++q) {
+      if (!fits[q]) continue;
+      const unsigned long long key = fkey[q];
+      long long fb, cc;
+      if (key == ~0ull) {  // no bin reaches 50%: last bin, which holds the n-th value
+        fb = smax[q] - bmin[q];
+        cc = n;
+      } else {
+        fb = (long long)(key >> 32);
+        cc = (long long)(key & 0xffffffffull);
+      }
+      fbin[q] = fb + bmin[q];
+      fcum[q] = cc;
+      fcnt[q] = hist[q][fb];
     }
-    __syncthreads();
-  } else {
+  }
+  for (int q = 0; q < K; ++q) {
+    if (fits[q]) continue;
     // wide range: binary search on the bin value with block-wide counts
-    __shared__ int bcount;
-    long long lo = bmin, hi = bmax;   // smallest b with cum(b)*pct >= 50
+    long long lo = bmin[q], hi = smax[q];   // smallest b with cum(b)*pct >= 50
     while (lo < hi) {
       const long long mid = lo + (hi - lo) / 2;
-      if (threadIdx.x == 0) bcount = 0;
+      if (threadIdx.x == 0) bcount[q] = 0;
       __syncthreads();
       int c = 0;
-      for (int i = threadIdx.x; i < n; i += blockDim.x) c += (long long)floor(v[i] / bw) <= mid;
-      atomicAdd(&bcount, c);
+      for (int i = threadIdx.x; i < n; i += blockDim.x) c += (long long)floor(v[q][i] / bw[q]) <= mid;
+      atomicAdd(&bcount[q], c);
       __syncthreads();
-      const int cum = bcount;
+      const int cum = bcount[q];
       __syncthreads();
       if ((double)cum * pct >= 50.0) hi = mid; else lo = mid + 1;
     }
-    if (threadIdx.x == 0) { bcount = 0; found_cnt = 0; }
+    if (threadIdx.x == 0) { bcount[q] = 0; fkey[q] = 0; }
     __syncthreads();
     int c = 0, e = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const long long b = (long long)floor(v[i] / bw);
+      const long long b = (long long)floor(v[q][i] / bw[q]);
       c += b <= lo;
       e += b == lo;
     }
-    atomicAdd(&bcount, c);
-    atomicAdd((unsigned long long*)&found_cnt, (unsigned long long)e);
+    atomicAdd(&bcount[q], c);
+    atomicAdd(&fkey[q], (unsigned long long)e);
     __syncthreads();
-    if (threadIdx.x == 0) { found_bin = lo; found_cum = bcount; }
-    __syncthreads();
+    fbin[q] = lo;
+    fcum[q] = bcount[q];
+    fcnt[q] = (long long)fkey[q];
   }
-  const double cumperc = (double)found_cum * pct;
-  const double binperc = (double)found_cnt * pct;
-  const double before = cumperc - binperc;
-  const double med = (double)found_bin * bw + (50.0 - before) * bw / binperc;
+  for (int q = 0; q < K; ++q) {
+    const double cumperc = (double)fcum[q] * pct;
+    const double binperc = (double)fcnt[q] * pct;
+    const double before = cumperc - binperc;
+    med[q] = (double)fbin[q] * bw[q] + (50.0 - before) * bw[q] / binperc;
+  }
   __syncthreads();
-  return med;
 }
 
 // Per molecule group (side): trimmed area, distance / normal statistics.
 // side[g*8 + k]: d_mean, d_median, s_mean, s_median, trimmed_area,
-// n_trimmed, n_dots, valid
+// n_trimmed, n_dots, valid.  The three sums share one pairwise tree (same
+// length), the two medians their passes.
 __global__ __launch_bounds__(256) void k_sc_sides(int G, const int* __restrict__ tgrp,
                                                   const double4* __restrict__ txyz,
                                                   const double* __restrict__ dist,
@@ -573,21 +690,26 @@ __global__ __launch_bounds__(256) void k_sc_sides(int G, const int* __restrict__
     const int lo = tgrp[g], n = tgrp[g + 1] - lo;
     const int h = g ^ 1;
     const int no = tgrp[h + 1] - tgrp[h];
-    const double area = block_np_sum(&txyz[lo].w, 4, n);
+    // dist and rr hold 0 where the other side has no trimmed dots
+    const double* a[3] = {&txyz[lo].w, dist + lo, rr + lo};
+    const int64_t st[3] = {4, 1, 1};
+    double sums[3];
+    block_np_sums<3, 512>(a, st, n, sums);
     double dmean = 0.0, dmed = 0.0, smean = 0.0, smed = 0.0;
     if (n > 0 && no > 0) {
-      const double sd = block_np_sum(dist + lo, 1, n);
-      const double sr = block_np_sum(rr + lo, 1, n);
-      dmed = block_binned_median(dist + lo, n, s.binwidth_dist);
-      const double rmed = block_binned_median(rr + lo, n, s.binwidth_norm);
-      dmean = sd / (double)n;
-      smean = -(sr / (double)n);
-      smed = -rmed;
+      const double* mv[2] = {dist + lo, rr + lo};
+      const double bw[2] = {s.binwidth_dist, s.binwidth_norm};
+      double med[2];
+      block_binned_medians<2>(mv, bw, n, med);
+      dmed = med[0];
+      dmean = sums[1] / (double)n;
+      smean = -(sums[2] / (double)n);
+      smed = -med[1];
     }
     if (threadIdx.x == 0) {
       double* o = side + 8 * (int64_t)g;
       o[0] = dmean; o[1] = dmed; o[2] = smean; o[3] = smed;
-      o[4] = area; o[5] = (double)n; o[6] = (double)(dgrp[g + 1] - dgrp[g]); o[7] = 1.0;
+      o[4] = sums[0]; o[5] = (double)n; o[6] = (double)(dgrp[g + 1] - dgrp[g]); o[7] = 1.0;
     }
     __syncthreads();
   }
