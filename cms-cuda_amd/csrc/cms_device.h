@@ -124,6 +124,17 @@ __device__ __forceinline__ bool pair_torus(V3 xi, double ri, V3 xj, double rj, d
   return ok;
 }
 
+// pair_torus's decision alone (same expressions, so the same outcome): the
+// probe can touch both spheres.
+__device__ __forceinline__ bool pair_geom_ok(V3 xi, double ri, V3 xj, double rj, double rp) {
+  const double eri = ri + rp, erj = rj + rp;
+  const V3 diff = xj - xi;
+  const double dsq = dot_alt(diff, diff);
+  const double far_sq = (eri + erj) * (eri + erj) - dsq;
+  const double cont_sq = dsq - (ri - rj) * (ri - rj);
+  return (far_sq > 0.0) && (cont_sq > 0.0);
+}
+
 // A pair matters only for the toroid queue (a non-blocker end) or for probes
 // (some atom of a triple i, j, k > j is a non-blocker); the toroid queue takes
 // pairs with a non-blocker first atom, or a non-blocker second atom when the

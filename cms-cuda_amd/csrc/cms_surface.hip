@@ -135,6 +135,14 @@ __device__ __forceinline__ bool circle_in_one_sphere(const int* lj, const double
   return wave_any(hit);
 }
 
+// w = arccos(clip(dt)) and sin(w) (_core.py:1505-1512, 1545-1550), out of
+// line: the polynomial constants of both are then materialised inside the
+// call instead of being held in registers across the callers' loops.
+__device__ __attribute__((noinline)) double2 acos_sin(double dt) {
+  const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
+  return make_double2(wk, sin(wk));
+}
+
 // Probe positions of the non-degenerate triple (i, j, k) (_core.py:1545-1575):
 // base point on the axis of the pair's torus circle, height h above the
 // plane of the three centres, probes bijk +- h uijk.  False when h^2 <= 0.
@@ -145,8 +153,7 @@ __device__ __forceinline__ bool probe_points(V3 xi, double eri, V3 xk, double er
   const double dik = norm_seq(xk - xi);
   const V3 uik = (xk - xi) / dik;
   const double dt = dot_alt(uij, uik);
-  const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
-  const double sw = sin(wk);
+  const double sw = acos_sin(dt).y;
   const V3 uijk = cross(uij, uik) / sw;
   const V3 utb = cross(uijk, uij);
   const double asymm = (eri * eri - erk * erk) / dik;
@@ -170,8 +177,8 @@ __device__ __forceinline__ bool probe_points(V3 xi, double eri, V3 xk, double er
 // evaluated only near |dt| = 1.
 __device__ __forceinline__ bool triple_degenerate(double dt) {
   if (fabs(dt) < 1.0 - 1e-6) return false;
-  const double wk = acos(fmin(fmax(dt, -1.0), 1.0));
-  const double sw = sin(wk);
+  const double2 ws = acos_sin(dt);
+  const double wk = ws.x, sw = ws.y;
   return (dt >= 1.0) || (dt <= -1.0) || (wk <= 0.0) || (sw <= 0.0);
 }
 
@@ -457,10 +464,11 @@ constexpr int LATT_ATT = 0x3f, LATT_IK = 0x40;
 constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32,
               PB_BETWEEN = 64, PB_ARC2 = 128, PB_EMIT = 256;
 
-// Three wavefronts per SIMD (at most 168 VGPRs): the kernel waits on LDS
-// and global loads between dependent steps, so occupancy is what hides them.
+// Four wavefronts per SIMD (at most 128 VGPRs; LDS under 10 KiB per
+// workgroup for the 128-slot rows): the kernel waits on LDS and global loads
+// between dependent steps, so occupancy is what hides them.
 template <int RC>
-__global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, Work w) {
+__global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, Work w) {
   constexpr int PR_RELW = RC / WAVE;      // reach mask words per pair
   constexpr int PR_QCAP = RC * 7 / 2;     // triangle-test items per batch, bounds the triple queue
   __shared__ int lj[RC];
@@ -469,12 +477,15 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
   __shared__ uint8_t fws[RC];   // forward slots (atom index > i) by ascending atom index
   __shared__ uint8_t frk[RC];   // forward rank of a forward slot
   __shared__ uint8_t pslots[RC];   // row slots of the row's pairs, in pair order
-  __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS], bx[PR_PAIRS];
-  __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], cpair[PR_PAIRS];
+  __shared__ double4 bu[PR_PAIRS], bt[PR_PAIRS];
+  __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], bdel[PR_PAIRS];
+  __shared__ uint8_t bsl[PR_PAIRS];   // row slot of the pair's j
   __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
   __shared__ uint64_t rrel_s[PR_RELW];   // union of the reach masks of the row's live toroid-queue pairs
   __shared__ int rtor_s;                 // the row has a live toroid-queue pair
   __shared__ uint16_t queue[PR_QCAP];
+  __shared__ uint32_t smark_w[(PR_QCAP + 3) / 4];   // item start marks, four per word
+  uint8_t* const smark = reinterpret_cast<uint8_t*>(smark_w);
   static_assert(RC <= 256 && PR_PAIRS <= 32 && RC <= PR_QCAP, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
@@ -531,11 +542,8 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
         const int sl = s0 + lane();
         bool rel = false;
         if (sl < nn && lj[sl] > i) {
-          V3 u, t;
-          double rij;
-          bool between;
           const double4 aj = lx[sl];
-          rel = pair_torus(xi, ai.w, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between) &&
+          rel = pair_geom_ok(xi, ai.w, ld3(aj), aj.w, s.rp) &&
                 pair_relevant(iatt, (int8_t)(latt[sl] & LATT_ATT), nbmax, lj[sl]);
         }
         const uint64_t m = ballot(rel);
@@ -571,7 +579,7 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
         bj[lane()] = j;
         bu[lane()] = make_double4(u.x, u.y, u.z, rij);
         bt[lane()] = make_double4(t.x, t.y, t.z, 0.0);
-        bx[lane()] = aj;
+        bsl[lane()] = (uint8_t)slot;
         const int8_t jatt = (int8_t)(latt[slot] & LATT_ATT);
         const bool bij = iblock && jatt <= ATTEN_BLOCKER;
         bflag[lane()] = (pair_toroid(iatt, jatt, s.rp) ? PB_TOR : 0) | (bij ? PB_BLOCKERS : 0) |
@@ -579,23 +587,38 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
         bnp[lane()] = 0;
         bnm[lane()] = 0;
         for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = 0ull;
-        // the pair's triangle items (pair, forward slot after j)
-        for (int t = 0; t < cnt; ++t) queue[inc - cnt + t] = (uint16_t)((lane() << 8) | fws[first + t]);
+        // the pair's triangle items (pair, forward slot after j) are items
+        // inc - cnt .. inc - 1; item it takes forward position it + bdel
+        bdel[lane()] = first - (inc - cnt);
       }
+      // start marks: pair + 1 at the first item of each pair with items
+      for (int q = lane(); q < (items + 3) / 4; q += WAVE) smark_w[q] = 0u;
+      __syncthreads();
+      if (lane() < nb && cnt > 0) smark[inc - cnt] = (uint8_t)(lane() + 1);
       __syncthreads();
 
-      // 1. triangle tests (_core.py:1478-1500); valid triples are compacted
-      // in place
+      // 1. triangle tests (_core.py:1478-1500), the lanes over the batch's
+      // items in pair order (an item's pair is the last start mark at or
+      // before it); valid triples are queued in the same order
       int nq = 0;
+      int acur = 0;   // pair + 1 of the items just before the chunk
       for (int it0 = 0; it0 < items; it0 += WAVE) {
         const int it = it0 + lane();
+        const int mk = it < items ? (int)smark[it] : 0;
+        const uint64_t S = ballot(mk != 0);
+        const uint64_t upto = S & ((2ull << lane()) - 1ull);   // marks at lanes <= this one
+        const int src = upto ? 63 - __builtin_clzll(upto) : lane();
+        const int mlast = __shfl(mk, src, WAVE);
+        const int apl = upto ? mlast : acur;
+        if (S) acur = __builtin_amdgcn_readlane(mk, 63 - __builtin_clzll(S));
         bool valid = false;
         int e = 0;
         if (it < items) {
-          e = queue[it];
-          const int a = e >> 8, sl = e & 255;
+          const int a = apl - 1;
+          const int sl = fws[it + bdel[a]];
+          e = (a << 8) | sl;
           const int lk = latt[sl];
-          const double4 ak = lx[sl], aj = bx[a];
+          const double4 ak = lx[sl], aj = lx[bsl[a]];
           const double erk = ak.w + s.rp, erj = aj.w + s.rp;
           valid = (lk & LATT_IK) && !((bflag[a] & PB_BLOCKERS) && (lk & LATT_ATT) <= ATTEN_BLOCKER) &&
                   norm_lt(ld3(ak) - ld3(aj), erj + erk);
@@ -638,48 +661,50 @@ __global__ __launch_bounds__(64, 3) void k_probe_rows(BatchView b, Settings s, W
       __syncthreads();
 
       // 3. containment (circle_in_one_sphere, in squared form) and the row
-      // atoms within reach of the circle; near-degenerate pairs take every
-      // row atom
+      // atoms within reach of the circle, for the pairs with candidates or
+      // in the toroid queue: one pair at a time, the lanes over the row
+      // slots, the reach mask words by ballot; near-degenerate pairs take
+      // every row atom
       {
         bool need = false;
         if (lane() < nb) {
           const int f = bflag[lane()];
-          const bool probes = (f & PB_CAND) && !(f & PB_KILLED);
-          need = probes || (f & PB_TOR);
+          need = ((f & PB_CAND) && !(f & PB_KILLED)) || (f & PB_TOR);
         }
-        const uint64_t m = ballot(need);
-        if (need) cpair[lanes_below(m)] = lane();
-        const int citems = __popcll(m) * nn;
-        __syncthreads();
-        int c = lane() / nn, sl = lane() - (lane() / nn) * nn;
-        for (int it0 = 0; it0 < citems; it0 += WAVE) {
-          if (it0 + lane() < citems) {
-            const int a = cpair[c];
-            if (lj[sl] != bj[a]) {
-              const int f = bflag[a];
-              const double4 ax = lx[sl], pu = bu[a];
-              const V3 v = ld3(ax) - ld3(bt[a]);
+        uint64_t m = ballot(need);
+        while (m) {
+          const int a = __builtin_ctzll(m);
+          m &= m - 1;
+          const int f = bflag[a];
+          const int sj = __builtin_amdgcn_readlane(slot, a);   // j's row slot
+          const double4 pu = bu[a], pt = bt[a];
+          const double rho2 = pu.w * pu.w;
+          bool cov = false;
+#pragma unroll
+          for (int wd = 0; wd < PR_RELW; ++wd) {
+            const int sl = wd * WAVE + lane();
+            bool hit = false;
+            if (sl < nn && sl != sj) {
+              const double4 ax = lx[sl];
+              const V3 v = ld3(ax) - ld3(pt);
               const double aa = dot_seq(v, ld3(pu));
               const double d2 = fmax(dot_seq(v, v) - aa * aa, 0.0);
-              const double rho2 = pu.w * pu.w;
               const double rr = ax.w + s.rp;
               if (!(f & PB_NEARDEG)) {
                 // farthest circle point within R: (d + rho)^2 + a^2 < R^2
                 const double R = rr - COVER_MARGIN;
                 const double D = ((R * R - aa * aa) - d2) - rho2;
-                if (R > 0.0 && D > 0.0 && 4.0 * d2 * rho2 < D * D) atomicOr(&bflag[a], PB_COVERED);
+                cov = cov || (R > 0.0 && D > 0.0 && 4.0 * d2 * rho2 < D * D);
               }
-              if (((f & PB_CAND) && !(f & PB_KILLED)) || (f & PB_TOR)) {
-                // nearest circle point within reach: (d - rho)^2 + a^2 < reach^2
-                const double reach = rr + REACH_MARGIN;
-                const double E = ((d2 + rho2) + aa * aa) - reach * reach;
-                if ((f & PB_NEARDEG) || E < 0.0 || 4.0 * d2 * rho2 > E * E)
-                  atomicOr((unsigned long long*)&brel[a][sl / WAVE], 1ull << (sl % WAVE));
-              }
+              // nearest circle point within reach: (d - rho)^2 + a^2 < reach^2
+              const double reach = rr + REACH_MARGIN;
+              const double E = ((d2 + rho2) + aa * aa) - reach * reach;
+              hit = (f & PB_NEARDEG) || E < 0.0 || 4.0 * d2 * rho2 > E * E;
             }
+            const uint64_t word = ballot(hit);
+            if (lane() == 0) brel[a][wd] = word;
           }
-          sl += WAVE;
-          while (sl >= nn) { sl -= nn; ++c; }
+          if (wave_any(cov) && lane() == 0) bflag[a] = f | PB_COVERED;
         }
       }
       __syncthreads();
