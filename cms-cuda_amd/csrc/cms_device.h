@@ -162,11 +162,23 @@ __host__ __device__ __forceinline__ int subdiv_count(double delta, double angle)
   return n;
 }
 
+// cos(a), sin(a) and atan2(y, x) out of line: the polynomial constants of
+// the library routines are then materialised inside the call instead of
+// being held in registers across the callers' loops.  Same operations, same
+// results.
+__device__ __attribute__((noinline)) inline double2 cos_sin(double a) {
+  return make_double2(cos(a), sin(a));
+}
+__device__ __attribute__((noinline)) inline double atan2_ool(double y, double x) {
+  return atan2(y, x);
+}
+
 // Circle point i: cen + x*c + y*s with c = rad*cos(a), s = rad*sin(a)
 __device__ __forceinline__ V3 subdiv_point(V3 cen, V3 x, V3 y, double rad, double delta, int i) {
   double a = delta * ((double)i + 0.5);
-  double c = rad * cos(a);
-  double s = rad * sin(a);
+  const double2 cs = cos_sin(a);
+  double c = rad * cs.x;
+  double s = rad * cs.y;
   return (cen + x * c) + y * s;
 }
 
@@ -183,7 +195,7 @@ __device__ __forceinline__ double arc_angle(V3 x, V3 v, V3 axis, V3* y_out) {
   V3 y = cross(axis, x);
   double dt1 = dot_seq(v, x);
   double dt2 = dot_seq(v, y);
-  double ang = atan2(dt2, dt1);
+  double ang = atan2_ool(dt2, dt1);
   if (ang < 0.0) ang = ang + TWO_PI;
   if (fabs(dt1) <= 1e-8 && fabs(dt2) <= 1e-8) ang = __builtin_nan("");
   *y_out = y;
