@@ -615,21 +615,46 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     }
     int* row = w.nbr + (int64_t)r * KCAP;
     uint8_t* fwd = w.fwd + (int64_t)r * KCAP;
+    // rank by d^2 alone (entries with a smaller d^2), and the forward rank:
+    // with ju = j - (g + 1) as unsigned, the forward entries (j > g) are
+    // exactly those with ju < 2^31 and keep their index order, the others
+    // wrap above them, so one unsigned compare counts the forward entries of
+    // smaller index.  Without exact d^2 ties the ranks are a permutation of
+    // 0 .. cnt-1 and sum to cnt (cnt - 1) / 2; tied entries share the lower
+    // rank, the sum falls short, and the row is ranked again with the
+    // (d^2, j) order.
+    const unsigned g1 = (unsigned)g + 1u;
+    int rsum = 0;
     for (int p = lane(); p < cnt; p += WAVE) {
       const double dp = ce[p].x;
-      const int jp = (int)__double_as_longlong(ce[p].y);
+      const unsigned jup = (unsigned)__double_as_longlong(ce[p].y) - g1;
       int rank = 0, frank = 0;
 #pragma unroll 4
       for (int t = 0; t < cnt; ++t) {
         const double2 e = ce[t];
-        const double dt = e.x;
-        const int jt = (int)__double_as_longlong(e.y);
-        rank += (dt < dp) || (dt == dp && jt < jp);
-        frank += (jt > g) & (jt < jp);
+        rank += e.x < dp;
+        frank += ((unsigned)__double_as_longlong(e.y) - g1) < jup;
       }
-      row[rank] = jp;
+      rsum += rank;
+      row[rank] = jup + g1;
       // forward neighbours (index > g) by ascending index: their row slots
-      if (jp > g) fwd[frank] = (uint8_t)rank;
+      if (jup < 0x80000000u) fwd[frank] = (uint8_t)rank;
+    }
+    if (wave_sum_int(rsum) != cnt * (cnt - 1) / 2) {
+      for (int p = lane(); p < cnt; p += WAVE) {
+        const double dp = ce[p].x;
+        const int jp = (int)__double_as_longlong(ce[p].y);
+        int rank = 0, frank = 0;
+        for (int t = 0; t < cnt; ++t) {
+          const double2 e = ce[t];
+          const double dt = e.x;
+          const int jt = (int)__double_as_longlong(e.y);
+          rank += (dt < dp) || (dt == dp && jt < jp);
+          frank += (jt > g) & (jt < jp);
+        }
+        row[rank] = jp;
+        if (jp > g) fwd[frank] = (uint8_t)rank;
+      }
     }
     int* brow = w.bur + (int64_t)r * BCAP;
     for (int p = lane(); p < cntb; p += WAVE) {
@@ -652,11 +677,8 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
       if (p < cnt) {
         jp = (int)__double_as_longlong(ce[p].y);
         if (jp > g) {
-          V3 u, t;
-          double rij;
-          bool between;
           const double4 aj = b.atom[jp];
-          ok = pair_torus(xi, ai.w, ld3(aj), aj.w, s.rp, &u, &t, &rij, &between);
+          ok = pair_geom_ok(xi, ai.w, ld3(aj), aj.w, s.rp);
           rel = ok && pair_relevant(w.atten[g], w.atten[jp], nbmax, jp);
         }
       }

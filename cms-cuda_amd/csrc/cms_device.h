@@ -90,13 +90,18 @@ __device__ __forceinline__ int wave_sum_int(int v) {
   return v;
 }
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
-// inclusive prefix sum over the wave
+// inclusive prefix sum over the wave, by DPP moves (no LDS traffic): sums
+// within each row of 16 lanes (row_shr 1, 2, 4, 8; lanes shifted in from
+// outside the row read 0), then row 1 and row 3 add the last lane of the
+// row before (row_bcast:15), rows 2 and 3 the last lane of row 1
+// (row_bcast:31); rows outside a row_mask keep old = 0 and add nothing.
 __device__ __forceinline__ int wave_incl_scan(int v) {
-  int l = lane();
-  for (int o = 1; o < WAVE; o <<= 1) {
-    int t = __shfl_up(v, o, WAVE);
-    if (l >= o) v += t;
-  }
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);    // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);    // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);    // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);    // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31
   return v;
 }
 __device__ __forceinline__ double wave_min_double(double v) {

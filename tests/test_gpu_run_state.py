@@ -3,7 +3,8 @@ reference's full-state parity harness compares (test/test_gpu_parity.py:
 153-204, compare_calcs) -- radmax, result counters and scalars, per-atom
 state, neighbour and buried tables, dots, trimmed dots, probes and the
 toroid queue -- on the known-answer complex and a 260 + 80-atom synthetic
-complex; and the CMS / max-CMS readouts on a surface generated in the other
+complex; neighbour rows with exact distance ties (a lattice complex) and
+what is built on them; and the CMS / max-CMS readouts on a surface generated in the other
 all_atoms mode (the reference reads whatever dots exist,
 cms-cuda/_core.py:946-1034).
 
@@ -128,6 +129,50 @@ def test_run_state_matches_oracle(which, golden_complex):
     say("  %d toroid-queue pairs, %d + %d trimmed dots, rows up to %d / %d"
         % (len(q), len(rg.trimmed_dots[0]), len(rg.trimmed_dots[1]), rg.neighbor_array.natom.shape[1],
            rg.buried_array.natom.shape[1]))
+
+
+def lattice_complex():
+    """Two slabs of a cubic lattice with spacing 3.5 A (exact in binary, so
+    every lattice distance is computed exactly and neighbour rows hold many
+    exact d^2 ties: 6 at 3.5 A, 12 at 3.5 sqrt 2, 8 at 3.5 sqrt 3); radii
+    alternate so that the bridge distances differ between rows."""
+    k = np.arange(6, dtype=np.float64)
+    g = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3) * 3.5
+    rad = np.array([1.6, 1.8, 1.7, 1.9])
+    b = g[g[:, 2] <= 3.5]
+    t = g[g[:, 2] >= 7.0] + np.array([0.25, 0.5, 0.5])
+    return dict(binder_xyz=b, binder_radii=rad[np.arange(len(b)) % 4],
+                target_xyz=t, target_radii=rad[(np.arange(len(t)) + 1) % 4])
+
+
+def test_lattice_ties_match_oracle():
+    """Neighbour rows with exact d^2 ties: the (d^2, j) order, and everything
+    built on it (pairs, probes, dots, CMS, SC), equal the oracle's."""
+    import cms_cuda
+    from oracle.cms_oracle import OracleCalculator
+    c = lattice_complex()
+    bx, br, tx, tr = c["binder_xyz"], c["binder_radii"], c["target_xyz"], c["target_radii"]
+    say("lattice complex with exact distance ties (%d + %d atoms)" % (len(br), len(tr)))
+    o = OracleCalculator()
+    o.add_binder_and_target(bx, br, tx, tr)
+    ocms = o.contact_ms(True)
+    osc = o.shape_complementarity()
+    calc = cms_cuda.MolecularSurfaceCalculator()
+    calc.add_binder_and_target(bx, br, tx, tr)
+    cms, per = calc.CalcLoaded()
+    sc = calc.CalcLoadedSC()
+    rg = calc.run
+    # the rows do hold ties: some row has two entries at the same distance
+    d2 = np.where(o.nbr >= 0, ((o.xyz[np.maximum(o.nbr, 0)] - o.xyz[:, None, :]) ** 2).sum(-1), np.nan)
+    assert any(len(np.unique(r[~np.isnan(r)])) < int((~np.isnan(r)).sum()) for r in d2)
+    compare_rows(rg.neighbor_array, o.nbr, o.nn, o.xyz, o.radius)
+    compare_rows(rg.buried_array, o.bur, o.nb, o.xyz, o.radius)
+    for m in (0, 1):
+        compare_dots(rg.dots[m], o.dots[m])
+    assert len(rg.probes) == len(o.pr_h)
+    assert rel_err(cms, ocms[0]) <= TOL_OUT and rel_err(per, ocms[1]) <= TOL_OUT
+    assert rel_err(sc, osc) <= TOL_OUT
+    assert len(rg.dots[0]) > 0 and len(rg.dots[1]) > 0
 
 
 def test_run_state_before_generation(golden_complex):
