@@ -65,7 +65,7 @@ WORKLOAD_TEXT = {
 
 KERNELS = ["attention", "region", "neighbors", "pairs_emit", "probes_count", "probes_count_long", "torus_count",
            "torus_count_long", "probes_emit", "near",
-           "convex_frames", "convex_count", "convex_emit", "concave_frames", "toroid_count", "toroid_emit", "concave_count", "concave_emit",
+           "convex_frames", "convex_count", "convex_count_long", "convex_emit", "convex_emit_long", "concave_frames", "toroid_count", "toroid_emit", "concave_count", "concave_emit",
            "nearest", "nearest32", "trim", "owner_sums", "sc_sides", "cms_adjoint", "dot_grad"]
 
 

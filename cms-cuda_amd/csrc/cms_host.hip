@@ -899,7 +899,9 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   if (est[M_R] > 0) {
     fill_int(ctx->ccnt.as<int>(), cR, ctx->cap[M_R], 0, st);
     TIMED("convex_frames", k_convex_frames<<<grid_for(est[M_R], 256, 8192), 256, 0, st>>>(b, s, w));
-    TIMED("convex_count", k_convex<false><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+    // rows of at most RC_SMALL neighbours, then the longer ones
+    TIMED("convex_count", (k_convex<false, RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
+    TIMED("convex_count_long", (k_convex<false, KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
   }
   if (est[M_P] > 0 && !by_row) {
     // the count pass visits only live toroid-queue pairs
@@ -940,7 +942,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
       if (est[k] == 0) est[k] = v / 2;
     }
   }
-  if (est[M_R] > 0) TIMED("convex_emit", k_convex<true><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
+  if (est[M_R] > 0) {
+    TIMED("convex_emit", (k_convex<true, RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
+    TIMED("convex_emit_long", (k_convex<true, KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
+  }
   if (est[M_P] > 0) {
     // the pairs with toroidal dots, then their dots in batches of pairs
     const int gP = grid_for(est[M_P], 256, 4096);

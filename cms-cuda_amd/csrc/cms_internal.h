@@ -168,7 +168,7 @@ constexpr int RC_SMALL = 128;
 template <int RC> __global__ void k_probe_rows(BatchView, Settings, Work);
 __global__ void k_probe_place(BatchView, Work);
 template <int RC> __global__ void k_torus_rows(BatchView, Settings, Work);
-template <bool EMIT> __global__ void k_convex(BatchView, Settings, Work);
+template <bool EMIT, int RC> __global__ void k_convex(BatchView, Settings, Work);
 __global__ void k_convex_frames(BatchView, Settings, Work);
 __global__ void k_toroid_count(BatchView, Settings, Work);
 __global__ void k_toroid_dot_pairs(Count, const int*, const int*, int*);

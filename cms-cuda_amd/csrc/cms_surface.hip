@@ -1196,16 +1196,18 @@ __global__ __launch_bounds__(256) void k_convex_frames(BatchView b, Settings s, 
     o[3] = make_double4(f.cy.x, f.cy.y, f.cy.z, 0.0);
   }
 }
-template <bool EMIT>
-__global__ __launch_bounds__(64, EMIT ? 3 : 4) void k_convex(BatchView b, Settings s, Work w) {
-  // Occupancy: the count pass is held to 128 VGPRs (four wavefronts per
-  // SIMD) and its LDS to 11,264 B (fourteen 64-lane workgroups per CU), the
-  // emit pass (three per SIMD) to 13,312 B (twelve): the point spacings
-  // exist in the emit pass only, and the emit pass's list of kept points
-  // shares the neighbours' storage (listed and placed before they load)
-  __shared__ double4 lx[KCAP];
+template <bool EMIT, int RC>
+__global__ __launch_bounds__(64, EMIT ? 4 : 5) void k_convex(BatchView b, Settings s, Work w) {
+  // Occupancy: the count pass is held to 102 VGPRs (five wavefronts per
+  // SIMD), the emit pass to 128 (four); rows of at most RC_SMALL neighbours
+  // run in the RC_SMALL instantiation, whose neighbour staging takes half
+  // the LDS (under 8 KiB per workgroup, twenty workgroups per CU), longer
+  // rows in the KCAP one.  The point spacings exist in the emit pass only,
+  // and the emit pass's list of kept points shares the neighbours' storage
+  // (listed and placed before they load)
+  __shared__ double4 lx[RC];
   uint16_t* const klist = reinterpret_cast<uint16_t*>(lx);   // emit: kept points under the count pass's mask
-  static_assert(CMASK_WORDS * WAVE * sizeof(uint16_t) <= KCAP * sizeof(double4), "kept-point list fits");
+  static_assert(CMASK_WORDS * WAVE * sizeof(uint16_t) <= RC * sizeof(double4), "kept-point list fits");
   // latitude circle t: centre xi + north * lat_dt[t] (recomputed where used,
   // the same operations, so the same bits)
   __shared__ double lat_dt[MAX_SUBDIV];
@@ -1215,9 +1217,9 @@ __global__ __launch_bounds__(64, EMIT ? 3 : 4) void k_convex(BatchView b, Settin
   // count pass: accessible non-blocker atoms with neighbours (the counts of
   // the other rows are zero-filled before); emit pass: rows with dots
   auto live = [&](int64_t q) {
-    if (EMIT) return w.ccnt[q] != 0;
+    if (EMIT) return w.ccnt[q] != 0 && row_in_class<RC>(w.nn[q]);
     const int g = w.ratom[q];
-    return w.nn[q] > 0 && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER;
+    return w.nn[q] > 0 && row_in_class<RC>(w.nn[q]) && w.access[g] != 0 && w.atten[g] > ATTEN_BLOCKER;
   };
   CMS_SPARSE_LOOP(r, nR, live) {
     const int g = w.ratom[r];
@@ -1834,8 +1836,10 @@ __global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Setti
 // explicit instantiations used by the host pipeline
 template __global__ void k_probes<false>(BatchView, Settings, Work);
 template __global__ void k_probes<true>(BatchView, Settings, Work);
-template __global__ void k_convex<false>(BatchView, Settings, Work);
-template __global__ void k_convex<true>(BatchView, Settings, Work);
+template __global__ void k_convex<false, RC_SMALL>(BatchView, Settings, Work);
+template __global__ void k_convex<true, RC_SMALL>(BatchView, Settings, Work);
+template __global__ void k_convex<false, KCAP>(BatchView, Settings, Work);
+template __global__ void k_convex<true, KCAP>(BatchView, Settings, Work);
 template __global__ void k_concave<false>(BatchView, Settings, Work);
 template __global__ void k_concave<true>(BatchView, Settings, Work);
 template __global__ void k_probe_rows<RC_SMALL>(BatchView, Settings, Work);
