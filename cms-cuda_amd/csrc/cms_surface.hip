@@ -427,10 +427,11 @@ __device__ __forceinline__ void torus_point_arcs(V3 pm, V3 xi, V3 xj, double eri
 //      atoms after j; valid triples are queued;
 //   2. degenerate / kill tests of the queued triples, compacting the
 //      non-degenerate ones (candidates) in place;
-//   3. per (pair, row atom): the single-sphere containment test of the pairs
-//      that have candidates or belong to the toroid queue, and the row atoms
-//      that can reach the pair's torus circle (the only ones a probe of the
-//      pair can collide with);
+//   3. per (pair, row atom), for the pairs of the toroid queue: the
+//      single-sphere containment test, and the row atoms that can reach the
+//      pair's torus circle (the only ones a probe or torus point of the pair
+//      can collide with); the other pairs' probes take every row atom (their
+//      masks cost more than they save);
 //   4. probe positions, one candidate per lane, and their collisions with the
 //      reaching row atoms other than j and k;
 //   5. the torus points of the toroid-queue pairs whose circle is not inside
@@ -586,7 +587,12 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
                         (between ? PB_BETWEEN : 0) | (jatt >= ATTEN_2 ? PB_ARC2 : 0);
         bnp[lane()] = 0;
         bnm[lane()] = 0;
-        for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = 0ull;
+        // every row atom but j (what the toroid-queue pairs narrow in step 3)
+        for (int wd = 0; wd < PR_RELW; ++wd) {
+          const int rem = nn - wd * WAVE;
+          const uint64_t rowm = rem >= WAVE ? ~0ull : (rem > 0 ? (1ull << rem) - 1ull : 0ull);
+          brel[lane()][wd] = rowm & ~(wd == slot / WAVE ? 1ull << (slot % WAVE) : 0ull);
+        }
         // the pair's triangle items (pair, forward slot after j) are items
         // inc - cnt .. inc - 1; item it takes forward position it + bdel
         bdel[lane()] = first - (inc - cnt);
@@ -661,16 +667,14 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
       __syncthreads();
 
       // 3. containment (circle_in_one_sphere, in squared form) and the row
-      // atoms within reach of the circle, for the pairs with candidates or
-      // in the toroid queue: one pair at a time, the lanes over the row
-      // slots, the reach mask words by ballot; near-degenerate pairs take
-      // every row atom
+      // atoms within reach of the circle, for the toroid-queue pairs: one
+      // pair at a time, the lanes over the row slots, the reach mask words
+      // by ballot; near-degenerate pairs take every row atom.  (Probes of a
+      // pair inside one sphere all collide with it, so leaving the other
+      // pairs unflagged changes no result.)
       {
         bool need = false;
-        if (lane() < nb) {
-          const int f = bflag[lane()];
-          need = ((f & PB_CAND) && !(f & PB_KILLED)) || (f & PB_TOR);
-        }
+        if (lane() < nb) need = (bflag[lane()] & PB_TOR) != 0;
         uint64_t m = ballot(need);
         while (m) {
           const int a = __builtin_ctzll(m);
