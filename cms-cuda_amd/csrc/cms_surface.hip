@@ -171,6 +171,13 @@ __device__ __forceinline__ bool probe_points(V3 xi, double eri, V3 xk, double er
   return true;
 }
 
+// dt = uij . (xk - xi) / |xk - xi| is decided against 1 - 1e-6 without the
+// square root and division: with c = uij . v and vv = |v|^2 (v = xk - xi),
+// c^2 < (1 - 1.000001e-6)^2 vv puts |c| / |v| below 1 - 1e-6 - 1e-12, and the
+// computed dt differs from c / |v| by a few ulps, so it stays below 1 - 1e-6
+// (not degenerate, not near it); only the few other triples compute dt.
+constexpr double DT_CLEAR_SQ = (1.0 - 1.000001e-6) * (1.0 - 1.000001e-6);
+
 // Degenerate test of a valid triple (_core.py:1505-1543): dt >= 1, dt <= -1,
 // w <= 0 or sin w <= 0 with w = arccos(clip(dt)).  For |dt| < 1 - 1e-6, w
 // lies in (1.4e-3, pi - 1.4e-3), so none of these holds and arccos / sin are
@@ -648,16 +655,23 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
           const int a = e >> 8, sl = e & 255;
           const double4 ak = lx[sl], pu = bu[a];
           const V3 xk = ld3(ak);
-          const double erk = ak.w + s.rp;
-          const double dik = norm_seq(xk - xi);
-          const V3 uik = (xk - xi) / dik;
-          const double dt = dot_alt(ld3(pu), uik);
-          if (triple_degenerate(dt)) {
-            const V3 dk = ld3(bt[a]) - xk;
-            if (dot_seq(dk, dk) < erk * erk - pu.w * pu.w) atomicOr(&bflag[a], PB_KILLED);
+          const V3 v = xk - xi;
+          const double vv = dot_seq(v, v);
+          const double c = dot_alt(ld3(pu), v);
+          if (c * c < DT_CLEAR_SQ * vv) {
+            cand = true;   // |dt| < 1 - 1e-6: neither degenerate nor near it
           } else {
-            cand = true;
-            atomicOr(&bflag[a], (fabs(dt) < 1.0 - 1e-10) ? PB_CAND : (PB_CAND | PB_NEARDEG));
+            const double erk = ak.w + s.rp;
+            const double dik = sqrt(vv);   // norm_seq(xk - xi)
+            const V3 uik = v / dik;
+            const double dt = dot_alt(ld3(pu), uik);
+            if (triple_degenerate(dt)) {
+              const V3 dk = ld3(bt[a]) - xk;
+              if (dot_seq(dk, dk) < erk * erk - pu.w * pu.w) atomicOr(&bflag[a], PB_KILLED);
+            } else {
+              cand = true;
+              if (!(fabs(dt) < 1.0 - 1e-10)) atomicOr(&bflag[a], PB_NEARDEG);
+            }
           }
         }
         const uint64_t m = ballot(cand);
