@@ -751,7 +751,7 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
           while (rm && (okp || okm)) {
             const int t = wd * WAVE + __builtin_ctzll(rm);
             rm &= rm - 1;
-            if (lj[t] == k) continue;
+            if (t == sl) continue;   // k (row slots hold distinct atoms)
             const double4 ax = lx[t];
             const double rr = ax.w + s.rp;
             const double lim = rr * rr;
