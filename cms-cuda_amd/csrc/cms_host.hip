@@ -84,7 +84,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf elist, nelist, ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dgrp;
+  DBuf elist, nelist, ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dss, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -299,7 +299,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->elist, &ctx->nelist, &ctx->ccnt, &ctx->cvf,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->ccf, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->ccf, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dss, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
                   &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -719,6 +719,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CK(ctx->prgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->dgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->dbase.ensure(sizeof(int64_t) * 4 * G));
+  CK(ctx->dss.ensure(sizeof(int4) * 2 * G));
   // two slots (molecule 0, molecule 1) per cell
   const int64_t cell_cap = 2 * (8ll * n + 64ll * B);
   CK(ctx->grid.ensure(sizeof(GridInfo) * (B + 1)));
@@ -928,9 +929,11 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   scan_excl(ctx->t1cnt.as<int>(), ctx->t1scan.as<int>(), cP, ctx->cap[M_P], part, st);
   scan_excl(ctx->t2cnt.as<int>(), ctx->t2scan.as<int>(), cP, ctx->cap[M_P], part, st);
   scan_excl(ctx->kcnt.as<int>(), ctx->kscan.as<int>(), cQ, ctx->cap[M_Q], part, st);
-  k_dot_bases<<<1, 256, 0, st>>>(G, ctx->rgrp.as<int>(), ctx->pgrp.as<int>(), ctx->prgrp.as<int>(),
-                                 ctx->cscan.as<int>(), ctx->t1scan.as<int>(), ctx->t2scan.as<int>(),
-                                 ctx->kscan.as<int>(), ctx->dbase.as<int64_t>(), ctx->dgrp.as<int>());
+  k_dot_stage_starts<<<grid_for(G, 256, 4096), 256, 0, st>>>(G, ctx->rgrp.as<int>(), ctx->pgrp.as<int>(),
+                                                              ctx->prgrp.as<int>(), ctx->cscan.as<int>(),
+                                                              ctx->t1scan.as<int>(), ctx->t2scan.as<int>(),
+                                                              ctx->kscan.as<int>(), ctx->dss.as<int4>());
+  k_dot_bases<<<1, DOT_BASES_THREADS, 0, st>>>(G, ctx->dss.as<int4>(), ctx->dbase.as<int64_t>(), ctx->dgrp.as<int>());
   // ---- dots
   k_stage_total<<<1, 256, 0, st>>>(ctx->dgrp.as<int>(), host_count(G), exact ? NOLIMIT : ctx->cap[M_D], meta, M_D,
                                     ctx->dgrp.as<int>(), G);

@@ -50,9 +50,10 @@ __global__ void k_group_starts_from_atoms(Count n, const int* items, const int* 
 __global__ void k_gather_scan(int G, const int* idx, int mul, const int* scan, int* out);
 __global__ void k_pack3(int n, const double* xyz, double4* out);
 __global__ void k_points_frame(int nq, const double4* q, int nr, const double4* r, double4* frame);
-__global__ void k_dot_bases(int G, const int* rgrp, const int* pgrp, const int* prgrp,
-                            const int* cscan, const int* t1scan, const int* t2scan,
-                            const int* kscan, int64_t* dbase, int* dgrp);
+constexpr int DOT_BASES_THREADS = 1024;   // k_dot_bases runs as one workgroup of this size
+__global__ void k_dot_stage_starts(int G, const int* rgrp, const int* pgrp, const int* prgrp, const int* cscan,
+                                   const int* t1scan, const int* t2scan, const int* kscan, int4* ss);
+__global__ void k_dot_bases(int G, const int4* ss, int64_t* dbase, int* dgrp);
 
 __global__ void k_drop_failed_rows(Count cnt, const int* ratom, const int* acplx, const int* status, int* nn,
                                    int* nb, int* pcnt);

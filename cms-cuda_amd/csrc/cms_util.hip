@@ -330,49 +330,49 @@ __global__ void k_gather_scan(int G, const int* __restrict__ idx, int mul, const
 
 // Dot group bases: per group the four stage totals, group offsets (scan over
 // G) and per-stage bases so that a unit's output offset is
-// dbase[4g + stage] + stage_scan[unit].
-__global__ __launch_bounds__(256) void k_dot_bases(int G, const int* __restrict__ rgrp,
-                                                   const int* __restrict__ pgrp,
-                                                   const int* __restrict__ prgrp,
-                                                   const int* __restrict__ cscan,
-                                                   const int* __restrict__ t1scan,
-                                                   const int* __restrict__ t2scan,
-                                                   const int* __restrict__ kscan,
-                                                   int64_t* __restrict__ dbase,
-                                                   int* __restrict__ dgrp) {
-  __shared__ int wsum[4];
-  int carry = 0;
-  for (int base = 0; base < G; base += 256) {
-    const int g = base + threadIdx.x;
-    int T0 = 0, T1 = 0, T2 = 0, T3 = 0;
-    if (g < G) {
-      T0 = cscan[rgrp[g + 1]] - cscan[rgrp[g]];
-      T1 = t1scan[pgrp[g + 1]] - t1scan[pgrp[g]];
-      T2 = t2scan[pgrp[g + 1]] - t2scan[pgrp[g]];
-      T3 = kscan[prgrp[g + 1]] - kscan[prgrp[g]];
-    }
-    const int D = T0 + T1 + T2 + T3;
-    const int inc = wave_incl_scan(D);
-    const int wid = threadIdx.x / WAVE;
-    if (lane() == WAVE - 1) wsum[wid] = inc;
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int i = 0; i < 4; ++i) {
-      if (i < wid) off += wsum[i];
-      tot += wsum[i];
-    }
-    const int start = carry + off + inc - D;
-    if (g < G) {
-      dgrp[g] = start;
-      dbase[4 * g + 0] = (int64_t)start - cscan[rgrp[g]];
-      dbase[4 * g + 1] = (int64_t)start + T0 - t1scan[pgrp[g]];
-      dbase[4 * g + 2] = (int64_t)start + T0 + T1 - t2scan[pgrp[g]];
-      dbase[4 * g + 3] = (int64_t)start + T0 + T1 + T2 - kscan[prgrp[g]];
-    }
-    carry += tot;
-    __syncthreads();
+// dbase[4g + stage] + stage_scan[unit].  k_dot_stage_starts gathers each
+// group's four stage starts and ends (a thread per group: the gathers of all
+// groups are in flight together); k_dot_bases, one workgroup of 1,024
+// threads each taking a contiguous run of groups, scans the run totals and
+// writes the bases.
+__global__ void k_dot_stage_starts(int G, const int* __restrict__ rgrp, const int* __restrict__ pgrp,
+                                   const int* __restrict__ prgrp, const int* __restrict__ cscan,
+                                   const int* __restrict__ t1scan, const int* __restrict__ t2scan,
+                                   const int* __restrict__ kscan, int4* __restrict__ ss) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+    ss[2 * g] = make_int4(cscan[rgrp[g]], t1scan[pgrp[g]], t2scan[pgrp[g]], kscan[prgrp[g]]);
+    ss[2 * g + 1] = make_int4(cscan[rgrp[g + 1]], t1scan[pgrp[g + 1]], t2scan[pgrp[g + 1]], kscan[prgrp[g + 1]]);
   }
-  if (threadIdx.x == 0) dgrp[G] = carry;
+}
+__global__ __launch_bounds__(DOT_BASES_THREADS) void k_dot_bases(int G, const int4* __restrict__ ss,
+                                                                 int64_t* __restrict__ dbase,
+                                                                 int* __restrict__ dgrp) {
+  __shared__ int wsum[DOT_BASES_THREADS / WAVE];
+  const int per = (G + DOT_BASES_THREADS - 1) / DOT_BASES_THREADS;
+  const int g0 = min(G, (int)threadIdx.x * per), g1 = min(G, g0 + per);
+  int S = 0;
+  for (int g = g0; g < g1; ++g) {
+    const int4 a = ss[2 * g], e = ss[2 * g + 1];
+    S += (((e.x - a.x) + (e.y - a.y)) + (e.z - a.z)) + (e.w - a.w);
+  }
+  const int inc = wave_incl_scan(S);
+  const int wid = threadIdx.x / WAVE;
+  if (lane() == WAVE - 1) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0;
+  for (int i = 0; i < wid; ++i) off += wsum[i];
+  int start = off + inc - S;
+  for (int g = g0; g < g1; ++g) {
+    const int4 a = ss[2 * g], e = ss[2 * g + 1];
+    const int T0 = e.x - a.x, T1 = e.y - a.y, T2 = e.z - a.z, T3 = e.w - a.w;
+    dgrp[g] = start;
+    dbase[4 * g + 0] = (int64_t)start - a.x;
+    dbase[4 * g + 1] = (int64_t)start + T0 - a.y;
+    dbase[4 * g + 2] = (int64_t)start + T0 + T1 - a.z;
+    dbase[4 * g + 3] = (int64_t)start + T0 + T1 + T2 - a.w;
+    start += ((T0 + T1) + T2) + T3;
+  }
+  if (threadIdx.x == DOT_BASES_THREADS - 1) dgrp[G] = off + inc;
 }
 
 // ------------------------------------------------------------ failed complexes
