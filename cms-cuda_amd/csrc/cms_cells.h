@@ -77,15 +77,17 @@ __global__ void k_cell_build(BatchView b, const GridInfo* grid, int* ccount, int
                              int* catom, double4* cxyz);
 __global__ void k_attention_cells(BatchView b, Settings s, int all_atoms, int full, const GridInfo* grid,
                                   const int* cstart, const int* catom, const double4* cxyz, int8_t* atten,
-                                  int8_t* access, int* need, unsigned long long* coinc, int* status);
+                                  int8_t* catt, int8_t* access, int* need, unsigned long long* coinc, int* status);
 __global__ void k_region_cells(BatchView b, Settings s, const GridInfo* grid, const int* cstart, const int* catom,
                                const double4* cxyz, const int8_t* atten, const int* list, const int* list_n,
                                int* need);
 __global__ void k_attention_cells_t(BatchView b, Settings s, int all_atoms, int full, const GridInfo* grid,
                                     const int* cstart, const int* catom, const double4* cxyz, int8_t* atten,
-                                    int8_t* access, int* need, unsigned long long* coinc, int* status, int* list,
-                                    int* list_n);
+                                    int8_t* catt, int8_t* access, int* need, unsigned long long* coinc, int* status,
+                                    int* list, int* list_n);
+// catt: the atoms' attention values in cell-slot order (the attention pass
+// writes them beside atten)
 __global__ void k_neighbors_cells(BatchView b, Settings s, Work w, const GridInfo* grid, const int* cstart,
-                                  const int* catom, const double4* cxyz);
+                                  const int* catom, const double4* cxyz, const int8_t* catt);
 
 }  // namespace cms

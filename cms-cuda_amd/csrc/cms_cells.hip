@@ -334,9 +334,10 @@ __device__ __forceinline__ int cell_slot(const int* cbeg, const int* cpre, int i
 // flag (non-blockers; every atom when full != 0).  Atoms are visited in
 // cell order so that consecutive workgroups read the same cells.
 // Returns whether the atom is a non-blocker.
-__device__ __forceinline__ bool attention_store(int g, int c, double mn, unsigned long long ckey, int all_atoms,
-                                                int full, const Settings& s, int8_t* atten, int8_t* access,
-                                                int* need, unsigned long long* coinc, int* status) {
+__device__ __forceinline__ bool attention_store(int g, int sidx, int c, double mn, unsigned long long ckey,
+                                                int all_atoms, int full, const Settings& s, int8_t* atten,
+                                                int8_t* catt, int8_t* access, int* need,
+                                                unsigned long long* coinc, int* status) {
   // atten, access and need come pre-filled in atom order (atten: blocker,
   // or buried in all-atoms mode; access 0; need 1 in full mode, else 0), so
   // only the non-blockers of a normal run are stored here: few scattered
@@ -345,6 +346,7 @@ __device__ __forceinline__ bool attention_store(int g, int c, double mn, unsigne
   const bool nonblocker = !all_atoms && !(dmin >= s.sep);
   if (nonblocker) {
     atten[g] = ATTEN_BURIED;
+    catt[sidx] = ATTEN_BURIED;   // the same in cell-slot order
     if (!full) need[g] = 1;
   }
   if (ckey != ~0ull) {
@@ -359,7 +361,8 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
                                                         const double4* __restrict__ cxyz,
-                                                        int8_t* __restrict__ atten, int8_t* __restrict__ access,
+                                                        int8_t* __restrict__ atten, int8_t* __restrict__ catt,
+                                                        int8_t* __restrict__ access,
                                                         int* __restrict__ need,
                                                         unsigned long long* __restrict__ coinc,
                                                         int* __restrict__ status) {
@@ -401,7 +404,8 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
       const unsigned long long t = __shfl_xor(ckey, o, WAVE);
       ckey = ckey < t ? ckey : t;
     }
-    if (lane() == 0) attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
+    if (lane() == 0)
+      attention_store(g, (int)sidx, c, mn, ckey, all_atoms, full, s, atten, catt, access, need, coinc, status);
     __syncthreads();
   }
 }
@@ -457,7 +461,8 @@ __global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Setti
                                                            const int* __restrict__ cstart,
                                                            const int* __restrict__ catom,
                                                            const double4* __restrict__ cxyz,
-                                                           int8_t* __restrict__ atten, int8_t* __restrict__ access,
+                                                           int8_t* __restrict__ atten, int8_t* __restrict__ catt,
+                                                        int8_t* __restrict__ access,
                                                            int* __restrict__ need,
                                                            unsigned long long* __restrict__ coinc,
                                                            int* __restrict__ status, int* __restrict__ list,
@@ -522,7 +527,8 @@ __global__ __launch_bounds__(256, 8) void k_attention_cells_t(BatchView b, Setti
     FOR_NEIGHBOR_CELLS(gi, ix, iy, iz, cmask, cell) {
       for (int t = cstart[2 * cell + m]; t < cstart[2 * cell + m + 1]; ++t) coincident(t);
     }
-    const bool nonblocker = attention_store(g, c, mn, ckey, all_atoms, full, s, atten, access, need, coinc, status);
+    const bool nonblocker =
+        attention_store(g, sidx, c, mn, ckey, all_atoms, full, s, atten, catt, access, need, coinc, status);
     if (list) {
       // the region pass's list of non-blocker cell slots: one atomic per
       // wavefront
@@ -545,7 +551,8 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
                                                         const GridInfo* __restrict__ grid,
                                                         const int* __restrict__ cstart,
                                                         const int* __restrict__ catom,
-                                                        const double4* __restrict__ cxyz) {
+                                                        const double4* __restrict__ cxyz,
+                                                        const int8_t* __restrict__ catt) {
   // row entries (d^2, j) in one 16 B word each: one LDS read per entry in
   // the rank loop
   __shared__ double2 ce[KCAP];
@@ -588,9 +595,9 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
           const bool inb = d2 < br * br;
           if (own) {
             isn = inb && j != g;
-            if (isn && w.atten[j] > ATTEN_BLOCKER) nbmax = max(nbmax, j);
+            if (isn && catt[t] > ATTEN_BLOCKER) nbmax = max(nbmax, j);
           } else {
-            isb = inb && w.atten[j] >= ATTEN_BURIED;
+            isb = inb && catt[t] >= ATTEN_BURIED;
           }
         }
         const uint64_t mn = ballot(isn), mb = ballot(isb);

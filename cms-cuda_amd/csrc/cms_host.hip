@@ -79,7 +79,7 @@ struct cms_ctx {
   int gen_all_atoms = -1;
   int nR = 0, nP = 0, nQ = 0, nD = 0;
   std::vector<int> h_dgrp, h_prgrp;
-  DBuf atten, access, status, coinc, rsel, rseln, need, rpos, ratom, part;
+  DBuf atten, catt, access, status, coinc, rsel, rseln, need, rpos, ratom, part;
   DBuf grid, gframe, ncellc, ncscan, acell, arank, ccount, cstart, ccur, catom, cxyz;
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
@@ -291,7 +291,7 @@ static Work work(cms_ctx* ctx) {
 
 static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
   return {&ctx->atom, &ctx->dens, &ctx->acplx, &ctx->amol, &ctx->aoff, &ctx->nb0,
-                  &ctx->agrp, &ctx->atten, &ctx->access, &ctx->status, &ctx->coinc,
+                  &ctx->agrp, &ctx->atten, &ctx->catt, &ctx->access, &ctx->status, &ctx->coinc,
                   &ctx->rsel, &ctx->rseln, &ctx->grid, &ctx->gframe, &ctx->ncellc,
                   &ctx->ncscan, &ctx->acell, &ctx->arank, &ctx->ccount, &ctx->cstart, &ctx->ccur, &ctx->catom, &ctx->cxyz, &ctx->need, &ctx->rpos, &ctx->ratom,
                   &ctx->part, &ctx->nn, &ctx->nbmax, &ctx->fwd, &ctx->nbr, &ctx->nb, &ctx->bur, &ctx->pcnt,
@@ -705,6 +705,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   ctx->host_groups = false;
 
   CK(ctx->atten.ensure(n + 1));
+  CK(ctx->catt.ensure(n + 1));
   CK(ctx->access.ensure(n + 1));
   CK(ctx->status.ensure(sizeof(int) * B));
   CK(ctx->coinc.ensure(sizeof(unsigned long long) * B));
@@ -781,6 +782,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     // the attention passes store only the non-blockers over these
     // (and the count of the region pass's list of non-blockers)
     fill_batch({fill_of(ctx->atten.as<int8_t>(), n, (int8_t)(all_atoms ? ATTEN_BURIED : ATTEN_BLOCKER)),
+                fill_of(ctx->catt.as<int8_t>(), n, (int8_t)(all_atoms ? ATTEN_BURIED : ATTEN_BLOCKER)),
                 fill_of(ctx->access.as<int8_t>(), n, (int8_t)0), fill_of(ctx->need.as<int>(), (int64_t)n, full),
                 fill_of(ctx->rseln.as<int>(), 1, 0)},
                st);
@@ -789,16 +791,16 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
       // cent of the atoms) for the region pass
       int* list = full ? nullptr : ctx->rsel.as<int>();
       TIMED("attention", k_attention_cells_t<<<grid_for(n, 256, 65536), 256, 0, st>>>(
-          b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
-          ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>(), list,
-          ctx->rseln.as<int>()));
+          b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->catt.as<int8_t>(),
+          ctx->access.as<int8_t>(), ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>(),
+          list, ctx->rseln.as<int>()));
       if (!full)
         TIMED("region", k_region_cells<<<grid_for(n, 16, 65536), 64, 0, st>>>(
             b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), list, ctx->rseln.as<int>(), ctx->need.as<int>()));
     } else {
       TIMED("attention", k_attention_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
-          b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->access.as<int8_t>(),
-          ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
+          b, s, all_atoms, full, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), ctx->catt.as<int8_t>(),
+          ctx->access.as<int8_t>(), ctx->need.as<int>(), ctx->coinc.as<unsigned long long>(), ctx->status.as<int>()));
       if (!full)
         TIMED("region", k_region_cells<<<grid_for(n, 1, 65536), 64, 0, st>>>(
             b, s, grid, cst, cat, cxyz, ctx->atten.as<int8_t>(), nullptr, nullptr, ctx->need.as<int>()));
@@ -826,7 +828,8 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     TIMED("neighbors", k_neighbors_cells<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w, ctx->grid.as<GridInfo>(),
                                                                                 ctx->cstart.as<int>(),
                                                                                 ctx->catom.as<int>(),
-                                                                                ctx->cxyz.as<double4>()));
+                                                                                ctx->cxyz.as<double4>(),
+                                                                                ctx->catt.as<int8_t>()));
     // (the neighbour pass counted each row's pairs)
     k_drop_failed_rows<<<grid_for(est[M_R], 256, 4096), 256, 0, st>>>(cR, ctx->ratom.as<int>(), ctx->acplx.as<int>(),
                                                                       ctx->status.as<int>(), ctx->nn.as<int>(),
