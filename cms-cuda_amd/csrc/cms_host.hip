@@ -129,6 +129,10 @@ struct cms_ctx {
   bool capturing = false;
   hipStream_t gstream = nullptr;    // stream the graph is captured on and launched to
   hipEvent_t gev0 = nullptr, gev1 = nullptr;
+  // side streams of the surface stage's concurrent branches (default
+  // priority) and their fork / join events
+  hipStream_t sstream[2] = {nullptr, nullptr};
+  hipEvent_t fev[3] = {nullptr, nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint64_t> gkey, ekey;  // key of the captured graph / of the last eager run
@@ -409,6 +413,10 @@ int cms_destroy(cms_ctx* ctx) {
   if (ctx->gev0) (void)hipEventDestroy(ctx->gev0);
   if (ctx->gev1) (void)hipEventDestroy(ctx->gev1);
   if (ctx->gstream) (void)hipStreamDestroy(ctx->gstream);
+  for (hipEvent_t e : ctx->fev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t q : ctx->sstream)
+    if (q) (void)hipStreamDestroy(q);
   if (ctx->hmeta) (void)hipHostFree(ctx->hmeta);
   for (DBuf* b : all_bufs(ctx)) b->release();
   delete ctx;
@@ -606,6 +614,18 @@ static int size_dots(cms_ctx* ctx, int64_t c) {
   CK(ctx->dbur.ensure(c + 1));
   CK(ctx->dtype.ensure(c + 1));
   ctx->cap[M_D] = std::max(ctx->cap[M_D], c);
+  return CMS_OK;
+}
+
+// Side streams and events of the surface stage's concurrent branches,
+// created once (outside any capture: the first learned run is eager).
+static int ensure_side(cms_ctx* ctx) {
+  if (ctx->capturing && !(ctx->sstream[0] && ctx->sstream[1] && ctx->fev[0] && ctx->fev[1] && ctx->fev[2]))
+    return CMS_ERR_STATE;
+  for (hipStream_t& q : ctx->sstream)
+    if (!q) CK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+  for (hipEvent_t& e : ctx->fev)
+    if (!e) CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return CMS_OK;
 }
 
@@ -852,6 +872,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     if ((r = size_pairs(ctx, headroom(v)))) return r;
     w = work(ctx);
   }
+  bool fork = false;   // the surface stage's branches run on side streams
   // large batches count probes and torus points per neighbour row, and that
   // pass writes the pair records itself
   const bool by_row = ctx->probe_count_mode == 2 || (ctx->probe_count_mode == 0 && est[M_R] >= PROBE_ROWS_MIN);
@@ -871,8 +892,23 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
       // pairs with the reach masks of the pass before
       TIMED("probes_count", (k_probe_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
       TIMED("probes_count_long", (k_probe_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
-      TIMED("torus_count", (k_torus_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
-      TIMED("torus_count_long", (k_torus_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
+      // From here three branches touch disjoint data: the torus points (side
+      // stream 0), the convex counts (side stream 1, below) and the probe
+      // placement, near lists and concave counts (this stream); they join
+      // before the dot scans.  Run concurrently, each fills the register and
+      // LDS room the others' wavefronts leave on a CU.  Learned runs only
+      // (an exact run waits on the host between stages), and not while
+      // kernels are timed one by one.
+      fork = !exact && ctx->prof_filter.empty() && ensure_side(ctx) == CMS_OK;
+      if (fork) {
+        CK(hipEventRecord(ctx->fev[0], st));
+        CK(hipStreamWaitEvent(ctx->sstream[0], ctx->fev[0], 0));
+        CK(hipStreamWaitEvent(ctx->sstream[1], ctx->fev[0], 0));
+      }
+      hipStream_t tst = fork ? ctx->sstream[0] : st;
+      TIMED("torus_count", (k_torus_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, tst>>>(b, s, w)));
+      TIMED("torus_count_long", (k_torus_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, tst>>>(b, s, w)));
+      if (fork) CK(hipEventRecord(ctx->fev[1], tst));
     }
     else
       TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
@@ -906,12 +942,14 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CKL();
   // ---- dot counts
   if (est[M_R] > 0) {
-    fill_int(ctx->ccnt.as<int>(), cR, ctx->cap[M_R], 0, st);
-    TIMED("convex_frames", k_convex_frames<<<grid_for(est[M_R], 256, 8192), 256, 0, st>>>(b, s, w));
+    hipStream_t cst_ = fork ? ctx->sstream[1] : st;
+    fill_int(ctx->ccnt.as<int>(), cR, ctx->cap[M_R], 0, cst_);
+    TIMED("convex_frames", k_convex_frames<<<grid_for(est[M_R], 256, 8192), 256, 0, cst_>>>(b, s, w));
     // rows of at most RC_SMALL neighbours, then the longer ones
-    TIMED("convex_count", (k_convex<false, RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
-    TIMED("convex_count_long", (k_convex<false, KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
+    TIMED("convex_count", (k_convex<false, RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, cst_>>>(b, s, w)));
+    TIMED("convex_count_long", (k_convex<false, KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, cst_>>>(b, s, w)));
   }
+  if (fork) CK(hipEventRecord(ctx->fev[2], ctx->sstream[1]));
   if (est[M_P] > 0 && !by_row) {
     // the count pass visits only live toroid-queue pairs
     fill_int(ctx->t1cnt.as<int>(), ctx->cap[M_P], 0, st);
@@ -926,6 +964,10 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
     } else {
       fill_int(ctx->kcnt.as<int>(), ctx->cap[M_Q], 0, st);
     }
+  }
+  if (fork) {
+    CK(hipStreamWaitEvent(st, ctx->fev[1], 0));
+    CK(hipStreamWaitEvent(st, ctx->fev[2], 0));
   }
   CKL();
   scan_excl(ctx->ccnt.as<int>(), ctx->cscan.as<int>(), cR, ctx->cap[M_R], part, st);
@@ -1393,7 +1435,7 @@ static int ensure_gstream(cms_ctx* ctx) {
   if (!ctx->gstream) CK(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
   if (!ctx->gev0) CK(hipEventCreateWithFlags(&ctx->gev0, hipEventDisableTiming));
   if (!ctx->gev1) CK(hipEventCreateWithFlags(&ctx->gev1, hipEventDisableTiming));
-  return CMS_OK;
+  return ensure_side(ctx);
 }
 
 // Capture one learned step on the graph stream.  Any failure (an
