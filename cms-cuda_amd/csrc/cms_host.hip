@@ -93,6 +93,9 @@ struct cms_ctx {
   DBuf tcnt, toff, bmin, wv, ocnt, ostart, ocur, operm, omulti, onmulti;
   DBuf trflag, trscan, tlist, txyz, tnml, tgrp, tmin, tidx, tgof, tdist, trr, side;
   DBuf qpack, rpack, grp1, gtot, npd2, npidx;
+  // scan and search scratch of the SC readout when it runs beside the CMS
+  // readout on a side stream (swapped in for the length of that pass)
+  DBuf part_b, tcnt_b, toff_b, npd2_b, npidx_b;
   DBuf frame, bxyz32, txyz32, qpack32, rpack32, nresc;   // frame: cms_nearest's point frame
   DBuf bidx, bgof, adj, eatom, evpos, chk, qlast, tqidx;
   DBuf tgrid, tncell, tncscan, tcount, tstart, tsorted;   // trim grid
@@ -310,7 +313,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->bmin, &ctx->wv, &ctx->ocnt, &ctx->ostart, &ctx->ocur, &ctx->operm, &ctx->omulti, &ctx->onmulti,
                   &ctx->trflag, &ctx->trscan, &ctx->tlist, &ctx->txyz,
                   &ctx->tnml, &ctx->tgrp, &ctx->tmin, &ctx->tidx, &ctx->tgof, &ctx->tdist,
-                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
+                  &ctx->trr, &ctx->side, &ctx->qpack, &ctx->rpack, &ctx->part_b, &ctx->tcnt_b, &ctx->toff_b, &ctx->npd2_b, &ctx->npidx_b, &ctx->grp1, &ctx->gtot, &ctx->npd2, &ctx->npidx, &ctx->frame,
                   &ctx->bxyz32, &ctx->txyz32, &ctx->qpack32, &ctx->rpack32, &ctx->nresc, &ctx->bidx,
                   &ctx->bgof, &ctx->adj, &ctx->eatom, &ctx->evpos, &ctx->chk, &ctx->qlast, &ctx->tqidx, &ctx->tgrid, &ctx->tncell, &ctx->tncscan,
           &ctx->tcount, &ctx->tstart, &ctx->tsorted, &ctx->meta};
@@ -735,6 +738,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   CK(ctx->rpos.ensure(sizeof(int) * (n + 1)));
   CK(ctx->ratom.ensure(sizeof(int) * (n + 1)));
   CK(ctx->part.ensure(sizeof(int) * 1040));
+  CK(ctx->part_b.ensure(sizeof(int) * 1040));
   CK(ctx->rgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->pgrp.ensure(sizeof(int) * (G + 1)));
   CK(ctx->prgrp.ensure(sizeof(int) * (G + 1)));
@@ -1406,9 +1410,45 @@ int cms_shape_complementarity(cms_ctx* ctx, int precision, double* results) {
 
 // ------------------------------------------------------------ one step
 
+// The SC readout's scratch swapped with the CMS readout's (see part_b).
+static void swap_readout_scratch(cms_ctx* ctx) {
+  std::swap(ctx->part, ctx->part_b);
+  std::swap(ctx->tcnt, ctx->tcnt_b);
+  std::swap(ctx->toff, ctx->toff_b);
+  std::swap(ctx->npd2, ctx->npd2_b);
+  std::swap(ctx->npidx, ctx->npidx_b);
+}
+
+// Batches with at least this many buried dots run the SC readout beside the
+// CMS readout (below it the two are too short to overlap usefully).
+constexpr int64_t READOUT_FORK_MIN = 65536;
+
 static int readouts_pass(cms_ctx* ctx, int precision, double* per_atom, double* totals, double* sc) {
-  int r = contact_ms_pass(ctx, precision, per_atom, totals);
-  if (!r && sc) r = sc_pass(ctx, precision, sc);
+  const bool fork = sc && ctx->prof_filter.empty() && ctx->est[M_B] >= READOUT_FORK_MIN && ensure_side(ctx) == CMS_OK;
+  if (!fork) {
+    int r = contact_ms_pass(ctx, precision, per_atom, totals);
+    if (!r && sc) r = sc_pass(ctx, precision, sc);
+    return r;
+  }
+  // both readouts start from the buried / accessible lists; the SC readout
+  // (trim grid, trim, trimmed search, side statistics) then runs on side
+  // stream 0 with its own scan and search scratch while the CMS readout
+  // (search, weights, per-atom sums) runs on this stream, and this stream
+  // waits for it
+  int r = prepare_lists(ctx);
+  if (!r && precision == CMS_PREC_FP32_RESCORE) r = prepare_float_lists(ctx);
+  if (r) return r;
+  hipStream_t st = ctx->stream;
+  CK(hipEventRecord(ctx->fev[0], st));
+  CK(hipStreamWaitEvent(ctx->sstream[0], ctx->fev[0], 0));
+  swap_readout_scratch(ctx);
+  ctx->stream = ctx->sstream[0];
+  r = sc_pass(ctx, precision, sc);
+  ctx->stream = st;
+  swap_readout_scratch(ctx);
+  CK(hipEventRecord(ctx->fev[1], ctx->sstream[0]));
+  if (!r) r = contact_ms_pass(ctx, precision, per_atom, totals);
+  CK(hipStreamWaitEvent(st, ctx->fev[1], 0));
   return r;
 }
 
