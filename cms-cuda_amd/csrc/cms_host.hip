@@ -999,23 +999,41 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
       if (est[k] == 0) est[k] = v / 2;
     }
   }
+  // the three emit passes write disjoint stretches of the dot arrays (each
+  // stage at its own group bases): on large batches the convex dots go to
+  // side stream 0, the toroidal ones (with their pair list and its scan) to
+  // side stream 1, the concave ones stay here, and this stream waits for
+  // both before the readouts
+  const bool fork_emit = by_row && ctx->prof_filter.empty() && ensure_side(ctx) == CMS_OK;
+  if (fork_emit) {
+    CK(hipEventRecord(ctx->fev[0], st));
+    CK(hipStreamWaitEvent(ctx->sstream[0], ctx->fev[0], 0));
+    CK(hipStreamWaitEvent(ctx->sstream[1], ctx->fev[0], 0));
+  }
+  const hipStream_t cvs = fork_emit ? ctx->sstream[0] : st, tos = fork_emit ? ctx->sstream[1] : st;
   if (est[M_R] > 0) {
-    TIMED("convex_emit", (k_convex<true, RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
-    TIMED("convex_emit_long", (k_convex<true, KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
+    TIMED("convex_emit", (k_convex<true, RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, cvs>>>(b, s, w)));
+    TIMED("convex_emit_long", (k_convex<true, KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, cvs>>>(b, s, w)));
   }
   if (est[M_P] > 0) {
     // the pairs with toroidal dots, then their dots in batches of pairs
     const int gP = grid_for(est[M_P], 256, 4096);
-    k_toroid_dot_pairs<<<gP, 256, 0, st>>>(cP, ctx->t1cnt.as<int>(), ctx->t2cnt.as<int>(), ctx->tef.as<int>());
-    scan_excl(ctx->tef.as<int>(), ctx->tes.as<int>(), cP, ctx->cap[M_P], part, st);
-    k_compact_count<<<gP, 256, 0, st>>>(cP, ctx->tef.as<int>(), ctx->tes.as<int>(), ctx->tel.as<int>());
-    k_copy_word<<<1, 64, 0, st>>>(ctx->tes.as<int>(), cP, meta + M_TE);
-    TIMED("toroid_emit", k_toroid_emit<<<grid_for(est[M_P] / 16 + 1, 1), 64, 0, st>>>(b, s, w, ctx->tel.as<int>(),
-                                                                                      meta + M_TE));
+    k_toroid_dot_pairs<<<gP, 256, 0, tos>>>(cP, ctx->t1cnt.as<int>(), ctx->t2cnt.as<int>(), ctx->tef.as<int>());
+    scan_excl(ctx->tef.as<int>(), ctx->tes.as<int>(), cP, ctx->cap[M_P], part, tos);
+    k_compact_count<<<gP, 256, 0, tos>>>(cP, ctx->tef.as<int>(), ctx->tes.as<int>(), ctx->tel.as<int>());
+    k_copy_word<<<1, 64, 0, tos>>>(ctx->tes.as<int>(), cP, meta + M_TE);
+    TIMED("toroid_emit", k_toroid_emit<<<grid_for(est[M_P] / 16 + 1, 1), 64, 0, tos>>>(b, s, w, ctx->tel.as<int>(),
+                                                                                       meta + M_TE));
   }
   if (est[M_Q] > 0 && s.rp > 0.0) {
     TIMED("concave_emit", (k_concave_place<<<grid_for(est[M_Q] * KSLOTS, 256, 8192), 256, 0, st>>>(b, s, w),
                            k_concave<true><<<grid_for(est[M_Q] / 64 + 1, 1), 64, 0, st>>>(b, s, w)));
+  }
+  if (fork_emit) {
+    CK(hipEventRecord(ctx->fev[1], ctx->sstream[0]));
+    CK(hipEventRecord(ctx->fev[2], ctx->sstream[1]));
+    CK(hipStreamWaitEvent(st, ctx->fev[1], 0));
+    CK(hipStreamWaitEvent(st, ctx->fev[2], 0));
   }
   CKL();
   ctx->generated = true;
