@@ -727,6 +727,27 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
       }
       __syncthreads();
 
+      // the candidates of killed pairs and of pairs inside one sphere have
+      // no probes: the others are compacted in place, in order, so that
+      // the placement below runs on full wavefronts
+      {
+        int nl = 0;
+        for (int q0 = 0; q0 < nc; q0 += WAVE) {
+          const int q = q0 + lane();
+          int e = 0;
+          bool lv = false;
+          if (q < nc) {
+            e = queue[q];
+            lv = !(bflag[e >> 8] & (PB_KILLED | PB_COVERED));
+          }
+          const uint64_t m = ballot(lv);
+          if (lv) queue[nl + lanes_below(m)] = (uint16_t)e;
+          nl += __popcll(m);
+        }
+        nc = nl;
+      }
+      __syncthreads();
+
       // 4. probes of the candidates of live pairs and their collisions
       // (_core.py:1545-1623)
       for (int q0 = 0; q0 < nc; q0 += WAVE) {
@@ -739,26 +760,40 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
           const int e = queue[q];
           a = e >> 8;
           sl = e & 255;
-          if (!(bflag[a] & (PB_KILLED | PB_COVERED))) {
-            j = bj[a];
-            k = lj[sl];
-            const double4 ak = lx[sl], pu = bu[a], pt = bt[a];
-            okp = okm = probe_points(xi, eri, ld3(ak), ak.w + s.rp, ld3(pu), ld3(pt), &pp, &pm, &uijk, &h);
-          }
+          j = bj[a];
+          k = lj[sl];
+          const double4 ak = lx[sl], pu = bu[a], pt = bt[a];
+          okp = okm = probe_points(xi, eri, ld3(ak), ak.w + s.rp, ld3(pu), ld3(pt), &pp, &pm, &uijk, &h);
         }
         for (int wd = 0; wd < nw && (okp || okm); ++wd) {
           uint64_t rm = brel[a][wd];
+          // two row atoms per step, their loads issued together; k itself is
+          // skipped (row slots hold distinct atoms)
           while (rm && (okp || okm)) {
-            const int t = wd * WAVE + __builtin_ctzll(rm);
+            const int t0 = wd * WAVE + __builtin_ctzll(rm);
             rm &= rm - 1;
-            if (t == sl) continue;   // k (row slots hold distinct atoms)
-            const double4 ax = lx[t];
-            const double rr = ax.w + s.rp;
-            const double lim = rr * rr;
-            const V3 xs = ld3(ax);
-            const V3 dp = pp - xs, dm = pm - xs;
-            okp = okp && !(dot_alt(dp, dp) <= lim);
-            okm = okm && !(dot_alt(dm, dm) <= lim);
+            const int t1 = rm ? wd * WAVE + __builtin_ctzll(rm) : t0;
+            rm &= rm - 1;
+            const double4 ax0 = lx[t0], ax1 = lx[t1];
+            bool hp = false, hm = false;
+            if (t0 != sl) {
+              const double rr = ax0.w + s.rp;
+              const double lim = rr * rr;
+              const V3 xs = ld3(ax0);
+              const V3 dp = pp - xs, dm = pm - xs;
+              hp = dot_alt(dp, dp) <= lim;
+              hm = dot_alt(dm, dm) <= lim;
+            }
+            if (t1 != sl) {
+              const double rr = ax1.w + s.rp;
+              const double lim = rr * rr;
+              const V3 xs = ld3(ax1);
+              const V3 dp = pp - xs, dm = pm - xs;
+              hp = hp || dot_alt(dp, dp) <= lim;
+              hm = hm || dot_alt(dm, dm) <= lim;
+            }
+            okp = okp && !hp;
+            okm = okm && !hm;
           }
         }
         if (okp || okm) {
@@ -960,15 +995,19 @@ __global__ __launch_bounds__(64, 3) void k_torus_rows(BatchView b, Settings s, W
           const double rij = pu.w;
           const V3 pm = subdiv_point(tij, ld3(tcx[a]), ld3(tcy[a]), rij, ttd[a], m);
           bool free_pt = true;
+          // two reaching atoms per step, their loads issued together (the
+          // point is free when none of them holds it, in any order)
           for (int wd = 0; wd < nw && free_pt; ++wd) {
             uint64_t rm = brel[a][wd];
-            while (rm) {
-              const int t = wd * WAVE + __builtin_ctzll(rm);
+            while (rm && free_pt) {
+              const int t0 = wd * WAVE + __builtin_ctzll(rm);
               rm &= rm - 1;
-              const double4 ax = lx[t];
-              const V3 d = pm - ld3(ax);
-              const double lim = ax.w + s.rp;
-              if (dot_alt(d, d) < lim * lim) { free_pt = false; break; }
+              const int t1 = rm ? wd * WAVE + __builtin_ctzll(rm) : t0;
+              rm &= rm - 1;
+              const double4 ax0 = lx[t0], ax1 = lx[t1];
+              const V3 d0 = pm - ld3(ax0), d1 = pm - ld3(ax1);
+              const double lim0 = ax0.w + s.rp, lim1 = ax1.w + s.rp;
+              if (dot_alt(d0, d0) < lim0 * lim0 || dot_alt(d1, d1) < lim1 * lim1) free_pt = false;
             }
           }
           if (free_pt) {
