@@ -63,8 +63,7 @@ WORKLOAD_TEXT = {
     "mixed": "mixed: %d complexes drawn like c3",
 }
 
-KERNELS = ["attention", "region", "neighbors", "pairs_emit", "probes_count", "probes_count_long", "torus_count",
-           "torus_count_long", "probes_emit", "near",
+KERNELS = ["attention", "region", "neighbors", "pairs_emit", "probes_count", "probes_count_long", "probes_emit", "near",
            "convex_frames", "convex_count", "convex_count_long", "convex_emit", "convex_emit_long", "concave_frames", "toroid_count", "toroid_emit", "concave_count", "concave_emit",
            "nearest", "nearest32", "trim", "owner_sums", "sc_sides", "cms_adjoint", "dot_grad"]
 
@@ -254,17 +253,13 @@ def stage_models(counts, sc, nb, nt, fp32):
         "neighbors": ("hbm", float(np.sum(34.0 * R + 12.0 * En + 4.0 * Eb)), "GB/s", PEAK_HBM_GBS,
                       "34 B per row atom + 12 B per neighbour entry + 4 B per buried entry", False),
         "probes_count": ("hbm", float(np.sum(36.0 * En + 108.0 * Np)), "GB/s", PEAK_HBM_GBS,
-                         "36 B per neighbour entry (index + coordinates, rows read once) + 108 B per torus "
-                         "pair (74 B record in, 34 B counts, flags and reach masks out)", False),
+                         "36 B per neighbour entry (index + coordinates, rows read once) + 108 B per pair "
+                         "(at most: pair record 17 B, circle 64 B, toroidal counts, flag and free-point masks "
+                         "25 B out)", False),
         "trim": ("valu", float(np.sum(trim)), "TFLOP/s", PEAK_FP64_TFLOPS,
                  "8 flops per (buried, accessible) dot pair of a side (the dense search the cell-grid trim "
                  "replaces)", True),
     }
-    # torus points of the toroid-queue pairs: no per-item model; the executed
-    # flops come from the SQ counters
-    m["torus_count"] = ("valu", None, "TFLOP/s", PEAK_FP64_TFLOPS,
-                        "torus points, collisions and saddle-arc counts of the live toroid-queue pairs "
-                        "(executed flops from the SQ counters)", True)
     # two searches per step share the name; brute force, so the modelled
     # pairs are the pairs evaluated
     name = "nearest32" if fp32 else "nearest"

@@ -84,7 +84,7 @@ struct cms_ctx {
   DBuf nn, nbr, nb, bur, nbmax, fwd;
   DBuf pcnt, pscan, pi, pj, pu, pt, pflag, rgrp, pgrp, pstage, nps;
   DBuf prcnt, prscan, patom, ppos, palt, prgrp, nearl, nnear;
-  DBuf elist, nelist, ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, trel, rrel, rtor, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dss, dgrp;
+  DBuf elist, nelist, ccnt, cvf, ccf, cscan, cmask, t1cnt, t1scan, t2cnt, t2scan, tflag, tmask, tef, tes, tel, kcnt, kpos, kinfo, kscan, dbase, dss, dgrp;
   DBuf dpos, dnml, dbur, dtype, datom, ddesc, ddesc2;
   // readout state
   bool lists_ready = false;
@@ -273,9 +273,6 @@ static Work work(cms_ctx* ctx) {
   w.t2scan = ctx->t2scan.as<int>();
   w.tflag = ctx->tflag.as<int8_t>();
   w.tmask = ctx->tmask.as<uint64_t>();
-  w.trel = ctx->trel.as<uint64_t>();
-  w.rrel = ctx->rrel.as<uint64_t>();
-  w.rtor = ctx->rtor.as<int8_t>();
   w.kcnt = ctx->kcnt.as<int>();
   w.ccf = ctx->ccf.as<double4>();
   w.kpos = ctx->kpos.as<double4>();
@@ -306,7 +303,7 @@ static std::vector<DBuf*> all_bufs(cms_ctx* ctx) {
                   &ctx->rgrp, &ctx->pgrp, &ctx->prcnt, &ctx->prscan, &ctx->patom,
                   &ctx->ppos, &ctx->palt, &ctx->prgrp, &ctx->nearl, &ctx->nnear, &ctx->elist, &ctx->nelist, &ctx->ccnt, &ctx->cvf,
                   &ctx->cscan, &ctx->cmask, &ctx->t1cnt, &ctx->t1scan, &ctx->t2cnt, &ctx->t2scan,
-                  &ctx->tflag, &ctx->tmask, &ctx->trel, &ctx->rrel, &ctx->rtor, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->ccf, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dss, &ctx->dgrp, &ctx->dpos,
+                  &ctx->tflag, &ctx->tmask, &ctx->tef, &ctx->tes, &ctx->tel, &ctx->kcnt, &ctx->ccf, &ctx->kpos, &ctx->kinfo, &ctx->kscan, &ctx->dbase, &ctx->dss, &ctx->dgrp, &ctx->dpos,
                   &ctx->dnml, &ctx->dbur, &ctx->dtype, &ctx->datom, &ctx->ddesc, &ctx->ddesc2, &ctx->bflag,
                   &ctx->bscan, &ctx->blist, &ctx->bxyz, &ctx->bgrp,
                   &ctx->axyz, &ctx->agrpd, &ctx->tcnt, &ctx->toff,
@@ -549,8 +546,6 @@ static int size_rows(cms_ctx* ctx, int64_t c) {
   CK(ctx->cmask.ensure(sizeof(uint64_t) * (c * CMASK_WORDS + 1)));
   CK(ctx->pstage.ensure(sizeof(ProbeRec) * PSLOTS * (c + 1)));
   CK(ctx->nps.ensure(sizeof(int) * (c + 1)));
-  CK(ctx->rrel.ensure(sizeof(uint64_t) * (KCAP / WAVE) * (c + 1)));
-  CK(ctx->rtor.ensure(c + 1));
   CK(ctx->cvf.ensure(sizeof(double4) * 4 * (c + 1)));
   ctx->cap[M_R] = std::max(ctx->cap[M_R], c);
   return CMS_OK;
@@ -570,7 +565,6 @@ static int size_pairs(cms_ctx* ctx, int64_t c) {
   CK(ctx->t2scan.ensure(sizeof(int) * (c + 1)));
   CK(ctx->tflag.ensure(c + 1));
   CK(ctx->tmask.ensure(sizeof(uint64_t) * 2 * (c + 1)));
-  CK(ctx->trel.ensure(sizeof(uint64_t) * (KCAP / WAVE) * (c + 1)));
   CK(ctx->elist.ensure(sizeof(int) * (c + 1)));
   CK(ctx->nelist.ensure(sizeof(int)));
   CK(ctx->tef.ensure(sizeof(int) * (c + 1)));
@@ -883,36 +877,29 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
   if (est[M_R] > 0 && !by_row) TIMED("pairs_emit", k_pairs<<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w));
   if (est[M_P] > 0) {
     if (by_row) {
-      // the torus pass writes the toroidal counts of the pairs of rows with
-      // a live toroid-queue pair only
-      fill_batch({fill_of(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0), fill_of(ctx->rtor.as<int8_t>(), ctx->cap[M_R], 0),
-                  fill_of(ctx->nelist.as<int>(), 1, 0),
+      // the row pass writes the toroidal counts of the live toroid-queue
+      // pairs only
+      fill_batch({fill_of(ctx->nps.as<int>(), cR, ctx->cap[M_R], 0), fill_of(ctx->nelist.as<int>(), 1, 0),
                   fill_of(ctx->t1cnt.as<int>(), cP, ctx->cap[M_P], 0),
                   fill_of(ctx->t2cnt.as<int>(), cP, ctx->cap[M_P], 0),
                   fill_of(ctx->tflag.as<int8_t>(), ctx->cap[M_P], 0)},
                  st);
       // rows of at most RC_SMALL neighbours, then the longer ones (LDS staging
-      // sized for each); then the torus points of the live toroid-queue
-      // pairs with the reach masks of the pass before
+      // sized for each); the pass also takes the torus points of the live
+      // toroid-queue pairs
       TIMED("probes_count", (k_probe_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, st>>>(b, s, w)));
       TIMED("probes_count_long", (k_probe_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, st>>>(b, s, w)));
-      // From here three branches touch disjoint data: the torus points (side
-      // stream 0), the convex counts (side stream 1, below) and the probe
-      // placement, near lists and concave counts (this stream); they join
-      // before the dot scans.  Run concurrently, each fills the register and
-      // LDS room the others' wavefronts leave on a CU.  Learned runs only
-      // (an exact run waits on the host between stages), and not while
-      // kernels are timed one by one.
+      // From here two branches touch disjoint data: the convex counts (side
+      // stream 1, below) and the probe placement, near lists and concave
+      // counts (this stream); they join before the dot scans.  Run
+      // concurrently, each fills the register and LDS room the other's
+      // wavefronts leave on a CU.  Learned runs only (an exact run waits on
+      // the host between stages), and not while kernels are timed one by one.
       fork = !exact && ctx->prof_filter.empty() && ensure_side(ctx) == CMS_OK;
       if (fork) {
         CK(hipEventRecord(ctx->fev[0], st));
-        CK(hipStreamWaitEvent(ctx->sstream[0], ctx->fev[0], 0));
         CK(hipStreamWaitEvent(ctx->sstream[1], ctx->fev[0], 0));
       }
-      hipStream_t tst = fork ? ctx->sstream[0] : st;
-      TIMED("torus_count", (k_torus_rows<RC_SMALL><<<grid_for(est[M_R], 1), 64, 0, tst>>>(b, s, w)));
-      TIMED("torus_count_long", (k_torus_rows<KCAP><<<grid_for(est[M_R], 64, 2048), 64, 0, tst>>>(b, s, w)));
-      if (fork) CK(hipEventRecord(ctx->fev[1], tst));
     }
     else
       TIMED("probes_count", k_probes<false><<<grid_for(est[M_P], 1), 64, 0, st>>>(b, s, w));
@@ -969,10 +956,7 @@ static int generate_pass(cms_ctx* ctx, int all_atoms, int full_state, bool exact
       fill_int(ctx->kcnt.as<int>(), ctx->cap[M_Q], 0, st);
     }
   }
-  if (fork) {
-    CK(hipStreamWaitEvent(st, ctx->fev[1], 0));
-    CK(hipStreamWaitEvent(st, ctx->fev[2], 0));
-  }
+  if (fork) CK(hipStreamWaitEvent(st, ctx->fev[2], 0));
   CKL();
   scan_excl(ctx->ccnt.as<int>(), ctx->cscan.as<int>(), cR, ctx->cap[M_R], part, st);
   scan_excl(ctx->t1cnt.as<int>(), ctx->t1scan.as<int>(), cP, ctx->cap[M_P], part, st);

@@ -137,9 +137,6 @@ struct Work {
   int* t2scan;
   int8_t* tflag;
   uint64_t* tmask;        // count pass: free torus points of each pair (two words)
-  uint64_t* trel;         // per-row pass: row slots within reach of each toroid pair's circle (4 words)
-  uint64_t* rrel;         // per-row pass: per row, the union of its live toroid pairs' reach masks (4 words)
-  int8_t* rtor;           // per-row pass: the row has a live toroid pair
   int* kcnt;
   double4* ccf;           // concave frames, 7 per probe (k_concave_frames)
   double4* kpos;          // KSLOTS staged concave dots per probe
@@ -167,7 +164,6 @@ template <bool EMIT> __global__ void k_probes(BatchView, Settings, Work);
 constexpr int RC_SMALL = 128;
 template <int RC> __global__ void k_probe_rows(BatchView, Settings, Work);
 __global__ void k_probe_place(BatchView, Work);
-template <int RC> __global__ void k_torus_rows(BatchView, Settings, Work);
 template <bool EMIT, int RC> __global__ void k_convex(BatchView, Settings, Work);
 __global__ void k_convex_frames(BatchView, Settings, Work);
 __global__ void k_toroid_count(BatchView, Settings, Work);

@@ -455,7 +455,6 @@ constexpr int PR_PAIRS = 32;            // pairs per batch (one per lane in the 
 // most RC_SMALL neighbours run in the RC_SMALL instantiation of the per-row
 // kernels (half the LDS, so more workgroups per CU), longer rows (up to
 // KCAP) in the KCAP one; each launch visits only its own rows.
-constexpr int TRELW = KCAP / WAVE;      // reach mask words per pair in HBM (trel)
 template <int RC>
 __device__ __forceinline__ bool row_in_class(int nn) {
   return RC < KCAP ? nn <= RC : nn > RC_SMALL;
@@ -471,6 +470,20 @@ constexpr double REACH_MARGIN = 0.01;
 constexpr int LATT_ATT = 0x3f, LATT_IK = 0x40;
 constexpr int PB_KILLED = 1, PB_NEARDEG = 2, PB_CAND = 4, PB_TOR = 8, PB_BLOCKERS = 16, PB_COVERED = 32,
               PB_BETWEEN = 64, PB_ARC2 = 128, PB_EMIT = 256;
+
+// Step 5 of k_probe_rows, in the LDS room of the triple queue: the circle
+// frames, point prefix, free-point masks and arc counts of up to TL_PAIRS
+// live toroid-queue pairs of the batch, their lanes in the batch, and the
+// free points queued for their saddle arcs ((slot << 8) | point).
+constexpr int TL_PAIRS = 8;
+struct TorusLds {
+  double cx[TL_PAIRS][3], cy[TL_PAIRS][3];
+  double td[TL_PAIRS], ad[TL_PAIRS];
+  unsigned long long mk0[TL_PAIRS], mk1[TL_PAIRS];
+  int pre[TL_PAIRS], n1[TL_PAIRS], n2[TL_PAIRS];
+  uint16_t fq[2 * WAVE];
+  uint8_t pa[TL_PAIRS];
+};
 
 // Four wavefronts per SIMD (at most 128 VGPRs; LDS under 10 KiB per
 // workgroup for the 128-slot rows): the kernel waits on LDS and global loads
@@ -489,11 +502,15 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
   __shared__ int bj[PR_PAIRS], bflag[PR_PAIRS], bnp[PR_PAIRS], bnm[PR_PAIRS], bdel[PR_PAIRS];
   __shared__ uint8_t bsl[PR_PAIRS];   // row slot of the pair's j
   __shared__ uint64_t brel[PR_PAIRS][PR_RELW];
-  __shared__ uint64_t rrel_s[PR_RELW];   // union of the reach masks of the row's live toroid-queue pairs
-  __shared__ int rtor_s;                 // the row has a live toroid-queue pair
-  __shared__ uint16_t queue[PR_QCAP];
-  __shared__ uint32_t smark_w[(PR_QCAP + 3) / 4];   // item start marks, four per word
+  // the triple queue and the item start marks (four per word) of steps 1-4;
+  // step 5 takes their room
+  constexpr int QBYTES = 2 * PR_QCAP + 4 * ((PR_QCAP + 3) / 4);
+  __shared__ double qraw[QBYTES / 8 + 1];
+  uint16_t* const queue = reinterpret_cast<uint16_t*>(qraw);
+  uint32_t* const smark_w = reinterpret_cast<uint32_t*>(queue + PR_QCAP);
   uint8_t* const smark = reinterpret_cast<uint8_t*>(smark_w);
+  TorusLds& tl = *reinterpret_cast<TorusLds*>(qraw);
+  static_assert(sizeof(TorusLds) <= QBYTES && PR_QCAP % 2 == 0, "step 5 fits the room of the triple queue");
   static_assert(RC <= 256 && PR_PAIRS <= 32 && RC <= PR_QCAP, "queue entries pack (pair << 8) | slot in 16 bits");
   const int nR = *w.nR;
   const int nP = *w.nP;
@@ -559,8 +576,6 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
         np += __popcll(m);
       }
     }
-    if (lane() < PR_RELW) rrel_s[lane()] = 0ull;
-    if (lane() == 0) rtor_s = 0;
     __syncthreads();
     int prow = 0;   // surviving candidates offered to the row's probe slots
     for (int pb = p_lo; pb < p_hi;) {
@@ -828,6 +843,135 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
       }
       __syncthreads();
 
+      // 5. torus points of the batch's live toroid-queue pairs (circle not
+      // inside one neighbour sphere; _core.py:1863-2044), TL_PAIRS pairs at
+      // a time: the circle frames, then the points spread over the lanes
+      // across pair boundaries, each tested against the row atoms within
+      // reach of its pair's circle (strict <, _core.py:1905-1917), and the
+      // saddle-arc dot counts of the free ones, queued and taken a wave at
+      // a time.  Writes the arc dot counts of both atoms, whether any point
+      // is free and the free points among the first 128; the host zero-fills
+      // them for the other pairs.
+      {
+        const bool tlv = lane() < nb && (bflag[lane()] & (PB_TOR | PB_COVERED)) == PB_TOR;
+        const uint64_t tlm = ballot(tlv);
+        const int ntl = __popcll(tlm);
+        const int tix = lanes_below(tlm);
+        const bool iarc = iatt >= ATTEN_2;
+        for (int t0 = 0; t0 < ntl; t0 += TL_PAIRS) {
+          const int nt = min(TL_PAIRS, ntl - t0);
+          if (tlv && tix >= t0 && tix < t0 + nt) tl.pa[tix - t0] = (uint8_t)lane();
+          __syncthreads();
+          int M = 0;
+          if (lane() < nt) {
+            const int a = tl.pa[lane()];
+            const double4 pu = bu[a], aj = lx[bsl[a]];
+            const TorusFrame tf = torus_frame(ai.w, aj.w, s.rp, pu.w, ld3(pu), (b.dens[i] + b.dens[bj[a]]) * 0.5);
+            tl.cx[lane()][0] = tf.cx.x;
+            tl.cx[lane()][1] = tf.cx.y;
+            tl.cx[lane()][2] = tf.cx.z;
+            tl.cy[lane()][0] = tf.cy.x;
+            tl.cy[lane()][1] = tf.cy.y;
+            tl.cy[lane()][2] = tf.cy.z;
+            tl.td[lane()] = tf.tdelta;
+            tl.ad[lane()] = tf.adelta;
+            tl.mk0[lane()] = 0ull;
+            tl.mk1[lane()] = 0ull;
+            tl.n1[lane()] = 0;
+            tl.n2[lane()] = 0;
+            M = tf.M;
+          }
+          const int incM = wave_incl_scan(M);
+          if (lane() < nt) tl.pre[lane()] = incM - M;
+          const int T = __shfl(incM, WAVE - 1, WAVE);
+          __syncthreads();
+          auto tpoint = [&](int q, int m) {
+            const double4 pt = bt[tl.pa[q]];
+            return subdiv_point(ld3(pt), v3(tl.cx[q][0], tl.cx[q][1], tl.cx[q][2]),
+                                v3(tl.cy[q][0], tl.cy[q][1], tl.cy[q][2]), bu[tl.pa[q]].w, tl.td[q], m);
+          };
+          auto arcs = [&](int cnt) {
+            if (lane() < cnt) {
+              const int e = tl.fq[lane()];
+              const int q = e >> 8;
+              const int a = tl.pa[q];
+              const double4 pu = bu[a], aj = lx[bsl[a]], pt = bt[a];
+              const int f = bflag[a];
+              const V3 pm = tpoint(q, e & 255);
+              TorusSub ts_{};
+              ts_.n1 = ts_.n2 = 0;
+              torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, ld3(pt), ld3(pu), pu.w, s.rp, (f & PB_BETWEEN) != 0,
+                               iarc, (f & PB_ARC2) != 0, tl.ad[q], &ts_);
+              if (ts_.n1) atomicAdd(&tl.n1[q], ts_.n1);
+              if (ts_.n2) atomicAdd(&tl.n2[q], ts_.n2);
+            }
+          };
+          int nf = 0;   // queued free points
+          for (int it0 = 0; it0 < T; it0 += WAVE) {
+            const int it = it0 + lane();
+            int fe = -1;
+            if (it < T) {
+              int q = 0;
+#pragma unroll
+              for (int step = TL_PAIRS / 2; step > 0; step >>= 1)
+                if (q + step < nt && tl.pre[q + step] <= it) q += step;
+              const int m = it - tl.pre[q];
+              const int a = tl.pa[q];
+              const V3 pm = tpoint(q, m);
+              // two reaching atoms per step, their loads issued together
+              // (the point is free when none of them holds it, in any order)
+              bool free_pt = true;
+              for (int wd = 0; wd < nw && free_pt; ++wd) {
+                uint64_t rm = brel[a][wd];
+                while (rm && free_pt) {
+                  const int u0 = wd * WAVE + __builtin_ctzll(rm);
+                  rm &= rm - 1;
+                  const int u1 = rm ? wd * WAVE + __builtin_ctzll(rm) : u0;
+                  rm &= rm - 1;
+                  const double4 ax0 = lx[u0], ax1 = lx[u1];
+                  const V3 d0 = pm - ld3(ax0), d1 = pm - ld3(ax1);
+                  const double lim0 = ax0.w + s.rp, lim1 = ax1.w + s.rp;
+                  if (dot_alt(d0, d0) < lim0 * lim0 || dot_alt(d1, d1) < lim1 * lim1) free_pt = false;
+                }
+              }
+              if (free_pt) {
+                if (m < WAVE) atomicOr(&tl.mk0[q], 1ull << m);
+                else atomicOr(&tl.mk1[q], 1ull << (m - WAVE));
+                fe = (q << 8) | m;
+              }
+            }
+            // free points join the queue; a full wave of them gets its arcs
+            const uint64_t fm = ballot(fe >= 0);
+            if (fe >= 0) tl.fq[nf + lanes_below(fm)] = (uint16_t)fe;
+            nf += __popcll(fm);
+            if (nf >= WAVE) {
+              __syncthreads();
+              arcs(WAVE);
+              // move the rest (fewer than WAVE) to the front
+              const int rest = nf - WAVE;
+              uint16_t e = 0;
+              if (lane() < rest) e = tl.fq[WAVE + lane()];
+              __syncthreads();
+              if (lane() < rest) tl.fq[lane()] = e;
+              nf = rest;
+            }
+          }
+          __syncthreads();
+          if (nf > 0) arcs(nf);
+          __syncthreads();
+          if (lane() < nt) {
+            const int p = pb + tl.pa[lane()];
+            const unsigned long long m0 = tl.mk0[lane()], m1 = tl.mk1[lane()];
+            w.t1cnt[p] = tl.n1[lane()];
+            w.t2cnt[p] = tl.n2[lane()];
+            w.tflag[p] = (m0 | m1) ? 1 : 0;
+            w.tmask[2 * (int64_t)p] = m0;
+            w.tmask[2 * (int64_t)p + 1] = m1;
+          }
+          __syncthreads();
+        }
+      }
+
       // 6. per-pair results
       if (lane() < nb) {
         const int p = pb + lane();
@@ -847,9 +991,8 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
           }
         }
         // the pair record (what the pair pass's emit would write); the
-        // emit passes read the geometry only of pairs with probes, the
-        // torus pass that of the live toroid-queue pairs, with their reach
-        // masks
+        // emit passes read the geometry only of pairs with probes and of
+        // the live toroid-queue pairs
         const bool tor_live = (f & PB_TOR) && !(f & PB_COVERED);
         if (f & PB_ARC2) pf |= 16;
         w.pi[p] = i;
@@ -859,194 +1002,14 @@ __global__ __launch_bounds__(64, 4) void k_probe_rows(BatchView b, Settings s, W
           w.pu[p] = bu[lane()];
           w.pt[p] = bt[lane()];
         }
-        if (tor_live) {
-          for (int wd = 0; wd < TRELW; ++wd) {
-            const uint64_t m = wd < PR_RELW ? brel[lane()][wd] : 0ull;
-            w.trel[TRELW * (int64_t)p + wd] = m;
-            if (m) atomicOr((unsigned long long*)&rrel_s[wd], (unsigned long long)m);
-          }
-          rtor_s = 1;
-        }
         w.prcnt[vp] = np;
         w.prcnt[vm] = nm;
       }
       pb += nb;
       __syncthreads();
     }
-    if (lane() == 0) {
-      w.nps[r] = min(prow, w.pslots);
-      w.rtor[r] = (int8_t)rtor_s;
-    }
-    if (lane() < TRELW) w.rrel[TRELW * (int64_t)r + lane()] = lane() < PR_RELW ? rrel_s[lane()] : 0ull;
+    if (lane() == 0) w.nps[r] = min(prow, w.pslots);
     __syncthreads();
-  }
-}
-
-// Torus points of the toroid-queue pairs (_core.py:1863-1940) after
-// k_probe_rows, one wavefront per neighbour row: the row's coordinates are
-// staged once, its pairs are taken in batches of TR_PAIRS, and the batch's
-// torus points are spread over the lanes across pair boundaries.  Each
-// point is tested against the row atoms that k_probe_rows found within
-// reach of its pair's circle (strict <, _core.py:1905-1917); the free ones
-// are queued in LDS and get their saddle-arc dot counts (_core.py:1947-2044)
-// a wave of them at a time.  k_probe_rows left
-// each live pair's reach mask (trel), per row their union (rrel) and
-// whether the row has a live pair (rtor).  Writes, for every pair of such a row, the arc dot counts of
-// both atoms, whether any torus point is free and the free points among the
-// first 128; the host zero-fills them for the other pairs.
-constexpr int TR_PAIRS = 32;
-
-template <int RC>
-__global__ __launch_bounds__(64, 3) void k_torus_rows(BatchView b, Settings s, Work w) {
-  constexpr int PR_RELW = RC / WAVE;
-  __shared__ double4 lx[RC];
-  __shared__ double4 bu[TR_PAIRS], bt[TR_PAIRS], bx[TR_PAIRS], tcx[TR_PAIRS], tcy[TR_PAIRS];
-  __shared__ double ttd[TR_PAIRS], tad[TR_PAIRS];
-  __shared__ int bflag[TR_PAIRS], tpre[TR_PAIRS], tn1[TR_PAIRS], tn2[TR_PAIRS];
-  __shared__ unsigned long long tmk0[TR_PAIRS], tmk1[TR_PAIRS];
-  __shared__ uint64_t brel[TR_PAIRS][PR_RELW];
-  __shared__ uint16_t fq[2 * WAVE];   // queued free points: (pair << 8) | point
-  static_assert(MAX_SUBDIV <= 2 * WAVE, "two mask words per pair");
-  const int nR = *w.nR;
-  const int nP = *w.nP;
-  // rows with a live toroid-queue pair (toroid queue, circle not inside a
-  // neighbour sphere), as k_probe_rows marked them
-  auto live = [&](int64_t q) { return w.rtor[q] != 0 && row_in_class<RC>(w.nn[q]); };
-  CMS_SPARSE_LOOP(r, nR, live) {
-    const int i = w.ratom[r];
-    const int nn = w.nn[r];
-    const int p_lo = w.pscan[r];
-    const int p_hi = min(p_lo + w.pcnt[r], nP);
-    if (p_lo >= p_hi || nn <= 0) continue;
-    const double4 ai = b.atom[i];
-    const V3 xi = ld3(ai);
-    const double eri = ai.w + s.rp;
-    const bool iarc = w.atten[i] >= ATTEN_2;
-    const double idens = b.dens[i];
-    const int nw = (nn + WAVE - 1) / WAVE;
-    {
-      // only the row atoms within reach of a live pair's circle
-      const int* row = w.nbr + (int64_t)r * KCAP;
-      for (int s0 = 0; s0 < nn; s0 += WAVE) {
-        const int sl = s0 + lane();
-        if (sl < nn && ((w.rrel[TRELW * (int64_t)r + s0 / WAVE] >> lane()) & 1ull)) lx[sl] = b.atom[row[sl]];
-      }
-    }
-    for (int pb = p_lo; pb < p_hi; pb += TR_PAIRS) {
-      const int nb = min(TR_PAIRS, p_hi - pb);
-      const int p = pb + lane();
-      const int pfl = lane() < nb ? w.pflag[p] : 0;
-      const bool tl = (pfl & 6) == 2;
-      int M = 0;
-      if (lane() < nb) {
-        tn1[lane()] = 0;
-        tn2[lane()] = 0;
-        tmk0[lane()] = 0ull;
-        tmk1[lane()] = 0ull;
-      }
-      if (tl) {
-        const int j = w.pj[p];
-        const double4 pu = w.pu[p], aj = b.atom[j];
-        bu[lane()] = pu;
-        bt[lane()] = w.pt[p];
-        bx[lane()] = aj;
-        bflag[lane()] = ((pfl & 1) ? PB_BETWEEN : 0) | ((pfl & 16) ? PB_ARC2 : 0);
-        for (int wd = 0; wd < PR_RELW; ++wd) brel[lane()][wd] = w.trel[TRELW * (int64_t)p + wd];
-        const TorusFrame tf = torus_frame(ai.w, aj.w, s.rp, pu.w, ld3(pu), (idens + b.dens[j]) * 0.5);
-        tcx[lane()] = make_double4(tf.cx.x, tf.cx.y, tf.cx.z, 0.0);
-        tcy[lane()] = make_double4(tf.cy.x, tf.cy.y, tf.cy.z, 0.0);
-        ttd[lane()] = tf.tdelta;
-        tad[lane()] = tf.adelta;
-        M = tf.M;
-      }
-      const int incM = wave_incl_scan(M);
-      if (lane() < nb) tpre[lane()] = incM - M;
-      const int T = __shfl(incM, WAVE - 1, WAVE);
-      __syncthreads();
-      // the saddle arcs of the queued free points, one per lane (the queue
-      // keeps the arcs' heavier work off the lanes of the collision tests)
-      auto arcs = [&](int cnt) {
-        if (lane() < cnt) {
-          const int e = fq[lane()];
-          const int a = e >> 8;
-          const double4 pu = bu[a], aj = bx[a], pt = bt[a];
-          const int f = bflag[a];
-          const V3 pm = subdiv_point(ld3(pt), ld3(tcx[a]), ld3(tcy[a]), pu.w, ttd[a], e & 255);
-          TorusSub ts_{};
-          ts_.n1 = ts_.n2 = 0;
-          torus_point_arcs(pm, xi, ld3(aj), eri, aj.w + s.rp, ld3(pt), ld3(pu), pu.w, s.rp, (f & PB_BETWEEN) != 0,
-                           iarc, (f & PB_ARC2) != 0, tad[a], &ts_);
-          if (ts_.n1) atomicAdd(&tn1[a], ts_.n1);
-          if (ts_.n2) atomicAdd(&tn2[a], ts_.n2);
-        }
-      };
-      int nf = 0;   // queued free points
-      for (int it0 = 0; it0 < T; it0 += WAVE) {
-        const int it = it0 + lane();
-        int fe = -1;
-        if (it < T) {
-          int a = 0;
-#pragma unroll
-          for (int step = TR_PAIRS / 2; step > 0; step >>= 1)
-            if (a + step < nb && tpre[a + step] <= it) a += step;
-          const int m = it - tpre[a];
-          const double4 pu = bu[a], pt = bt[a], aj = bx[a];
-          const V3 uij = ld3(pu), tij = ld3(pt);
-          const double rij = pu.w;
-          const V3 pm = subdiv_point(tij, ld3(tcx[a]), ld3(tcy[a]), rij, ttd[a], m);
-          bool free_pt = true;
-          // two reaching atoms per step, their loads issued together (the
-          // point is free when none of them holds it, in any order)
-          for (int wd = 0; wd < nw && free_pt; ++wd) {
-            uint64_t rm = brel[a][wd];
-            while (rm && free_pt) {
-              const int t0 = wd * WAVE + __builtin_ctzll(rm);
-              rm &= rm - 1;
-              const int t1 = rm ? wd * WAVE + __builtin_ctzll(rm) : t0;
-              rm &= rm - 1;
-              const double4 ax0 = lx[t0], ax1 = lx[t1];
-              const V3 d0 = pm - ld3(ax0), d1 = pm - ld3(ax1);
-              const double lim0 = ax0.w + s.rp, lim1 = ax1.w + s.rp;
-              if (dot_alt(d0, d0) < lim0 * lim0 || dot_alt(d1, d1) < lim1 * lim1) free_pt = false;
-            }
-          }
-          if (free_pt) {
-            if (m < WAVE) atomicOr(&tmk0[a], 1ull << m);
-            else atomicOr(&tmk1[a], 1ull << (m - WAVE));
-            fe = (a << 8) | m;
-          }
-        }
-        // free points join the queue; a full wave of them gets its arcs
-        const uint64_t fm = ballot(fe >= 0);
-        if (fe >= 0) {
-          const int o = nf + lanes_below(fm);
-          fq[o] = (uint16_t)fe;
-        }
-        nf += __popcll(fm);
-        if (nf >= WAVE) {
-          __syncthreads();
-          arcs(WAVE);
-          // move the rest (fewer than WAVE) to the front
-          const int rest = nf - WAVE;
-          uint16_t e = 0;
-          if (lane() < rest) e = fq[WAVE + lane()];
-          __syncthreads();
-          if (lane() < rest) fq[lane()] = e;
-          nf = rest;
-        }
-      }
-      __syncthreads();
-      if (nf > 0) arcs(nf);
-      __syncthreads();
-      if (lane() < nb) {
-        w.t1cnt[p] = tn1[lane()];
-        w.t2cnt[p] = tn2[lane()];
-        w.tflag[p] = (tmk0[lane()] | tmk1[lane()]) ? 1 : 0;
-        w.tmask[2 * (int64_t)p] = tmk0[lane()];
-        w.tmask[2 * (int64_t)p + 1] = tmk1[lane()];
-      }
-      __syncthreads();
-    }
   }
 }
 
@@ -1901,8 +1864,6 @@ template __global__ void k_concave<false>(BatchView, Settings, Work);
 template __global__ void k_concave<true>(BatchView, Settings, Work);
 template __global__ void k_probe_rows<RC_SMALL>(BatchView, Settings, Work);
 template __global__ void k_probe_rows<KCAP>(BatchView, Settings, Work);
-template __global__ void k_torus_rows<RC_SMALL>(BatchView, Settings, Work);
-template __global__ void k_torus_rows<KCAP>(BatchView, Settings, Work);
 
 // Staged concave dots to their place in the dot arrays: one thread per
 // staging slot of probes with at most KSLOTS dots.
