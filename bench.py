@@ -63,9 +63,9 @@ WORKLOAD_TEXT = {
     "mixed": "mixed: %d complexes drawn like c3",
 }
 
-KERNELS = ["attention", "region", "neighbors", "pairs_emit", "probes_count", "probes_count_long", "probes_emit", "near",
+KERNELS = ["grid", "cell_lists", "attention", "region", "neighbors", "pairs_emit", "probes_count", "probes_count_long", "probes_emit", "near",
            "convex_frames", "convex_count", "convex_count_long", "convex_emit", "convex_emit_long", "concave_frames", "toroid_count", "toroid_emit", "concave_count", "concave_emit",
-           "nearest", "nearest32", "trim", "owner_sums", "sc_sides", "cms_adjoint", "dot_grad"]
+           "nearest", "nearest32", "trim_grid", "trim", "owner_sums", "group_sums", "sc_sides", "cms_adjoint", "dot_grad"]
 
 
 def log(msg):

@@ -92,14 +92,17 @@ template <int N> __device__ __forceinline__ Dual<N> t_sqrt(const Dual<N>& a) {
   return r;
 }
 template <int N> __device__ __forceinline__ Dual<N> t_sin(const Dual<N>& a) {
-  Dual<N> r; r.v = sin(a.v);
-  const double f = cos(a.v);
+  Dual<N> r;
+  double f;
+  sincos(a.v, &r.v, &f);   // one argument reduction, the bits of sin() and cos()
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
   return r;
 }
 template <int N> __device__ __forceinline__ Dual<N> t_cos(const Dual<N>& a) {
-  Dual<N> r; r.v = cos(a.v);
-  const double f = -sin(a.v);
+  Dual<N> r;
+  double sn;
+  sincos(a.v, &sn, &r.v);
+  const double f = -sn;
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
   return r;
 }
