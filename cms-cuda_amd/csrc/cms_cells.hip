@@ -297,8 +297,10 @@ __device__ __forceinline__ int col_index(const ColRanges& r, int t) {
 
 // Flattened candidate list of the cells around (ix, iy, iz) selected by
 // mask, molecule range sel: cbeg[k] / cpre[k] are the start in catom / cxyz and the
-// prefix count of cell k; cpre[27] is the total.  Wave-cooperative (lanes
-// 0..26 each own a cell).
+// prefix count of cell k; cpre[27] is the total, cpre[28 .. 31] INT_MAX
+// (sentinels for cell_slot's search).  Wave-cooperative (lanes 0..26 each
+// own a cell).
+constexpr int CPRE_N = 32;   // cpre entries: 27 prefixes, the total, three sentinels
 __device__ __forceinline__ void cells_around(const GridInfo& gi, int ix, int iy, int iz,
                                              const int* __restrict__ cstart, int sel, uint32_t mask, int* cbeg,
                                              int* cpre) {
@@ -314,15 +316,18 @@ __device__ __forceinline__ void cells_around(const GridInfo& gi, int ix, int iy,
   const int inc = wave_incl_scan(cnt);
   if (k < 27) { cbeg[k] = beg; cpre[k] = inc - cnt; }
   if (k == 26) cpre[27] = inc;
+  if (k >= 28 && k < CPRE_N) cpre[k] = INT_MAX;
   __syncthreads();
 }
 
+// The cell of flattened index idx < cpre[27] is the last k with cpre[k] <=
+// idx (it is non-empty: cpre[k + 1] > idx); a fixed five-step search over
+// the 32 entries (the sentinels are never <= idx), fully unrolled and
+// branch-free, so a wave issues no loop control and no exec-mask updates.
 __device__ __forceinline__ int cell_slot(const int* cbeg, const int* cpre, int idx) {
-  int lo = 0, hi = 27;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (cpre[mid] <= idx) lo = mid; else hi = mid;
-  }
+  int lo = 0;
+#pragma unroll
+  for (int step = 16; step > 0; step >>= 1) lo = cpre[lo + step] <= idx ? lo + step : lo;
   return cbeg[lo] + idx - cpre[lo];
 }
 
@@ -366,7 +371,7 @@ __global__ __launch_bounds__(64) void k_attention_cells(BatchView b, Settings s,
                                                         int* __restrict__ need,
                                                         unsigned long long* __restrict__ coinc,
                                                         int* __restrict__ status) {
-  __shared__ int cbeg[27], cpre[28];
+  __shared__ int cbeg[27], cpre[CPRE_N];
   CMS_UNIT_LOOP(sidx, b.ntot) {
     const int g = catom[sidx];
     const int c = b.acplx[g];
@@ -423,7 +428,7 @@ __global__ __launch_bounds__(64) void k_region_cells(BatchView b, Settings s, co
                                                      const int8_t* __restrict__ atten,
                                                      const int* __restrict__ list,
                                                      const int* __restrict__ list_n, int* __restrict__ need) {
-  __shared__ int cbeg[27], cpre[28];
+  __shared__ int cbeg[27], cpre[CPRE_N];
   CMS_UNIT_LOOP(u, list ? *list_n : b.ntot) {
     const int sidx = list ? list[u] : (int)u;
     const int g = catom[sidx];
@@ -557,7 +562,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
   // the rank loop
   __shared__ double2 ce[KCAP];
   __shared__ int bj[BCAP];
-  __shared__ int cbeg[27], cpre[28];
+  __shared__ int cbeg[27], cpre[CPRE_N];
   const int nR = *w.nR;
   CMS_UNIT_LOOP(r, nR) {
     const int g = w.ratom[r];
@@ -569,6 +574,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     int ix, iy, iz;
     cell_coords(gi, ai.x, ai.y, ai.z, &ix, &iy, &iz);
     int cnt = 0, cntb = 0, nbmax = -1;
+    const unsigned g1 = (unsigned)g + 1u;
     // both rows take atoms within the bridge distance r_i + r_j + 2 rp, at
     // most r_i + r_max + 2 rp: only the cells whose box comes that close
     // can hold them (one cell tested per lane)
@@ -602,8 +608,9 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
         }
         const uint64_t mn = ballot(isn), mb = ballot(isb);
         if (isn) {
+          // the entry keeps ju = j - (g + 1) as unsigned (see the rank loop)
           const int p = cnt + lanes_below(mn);
-          if (p < KCAP) ce[p] = make_double2(d2, __longlong_as_double((long long)j));
+          if (p < KCAP) ce[p] = make_double2(d2, __longlong_as_double((long long)((unsigned)j - g1)));
         }
         if (isb) {
           const int p = cntb + lanes_below(mb);
@@ -623,24 +630,23 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     int* row = w.nbr + (int64_t)r * KCAP;
     uint8_t* fwd = w.fwd + (int64_t)r * KCAP;
     // rank by d^2 alone (entries with a smaller d^2), and the forward rank:
-    // with ju = j - (g + 1) as unsigned, the forward entries (j > g) are
+    // with ju = j - (g + 1) as unsigned (what the entries hold), the forward entries (j > g) are
     // exactly those with ju < 2^31 and keep their index order, the others
     // wrap above them, so one unsigned compare counts the forward entries of
     // smaller index.  Without exact d^2 ties the ranks are a permutation of
     // 0 .. cnt-1 and sum to cnt (cnt - 1) / 2; tied entries share the lower
     // rank, the sum falls short, and the row is ranked again with the
     // (d^2, j) order.
-    const unsigned g1 = (unsigned)g + 1u;
     int rsum = 0;
     for (int p = lane(); p < cnt; p += WAVE) {
       const double dp = ce[p].x;
-      const unsigned jup = (unsigned)__double_as_longlong(ce[p].y) - g1;
+      const unsigned jup = (unsigned)__double_as_longlong(ce[p].y);
       int rank = 0, frank = 0;
 #pragma unroll 4
       for (int t = 0; t < cnt; ++t) {
         const double2 e = ce[t];
         rank += e.x < dp;
-        frank += ((unsigned)__double_as_longlong(e.y) - g1) < jup;
+        frank += (unsigned)__double_as_longlong(e.y) < jup;
       }
       rsum += rank;
       row[rank] = jup + g1;
@@ -650,12 +656,12 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
     if (wave_sum_int(rsum) != cnt * (cnt - 1) / 2) {
       for (int p = lane(); p < cnt; p += WAVE) {
         const double dp = ce[p].x;
-        const int jp = (int)__double_as_longlong(ce[p].y);
+        const int jp = (int)((unsigned)__double_as_longlong(ce[p].y) + g1);
         int rank = 0, frank = 0;
         for (int t = 0; t < cnt; ++t) {
           const double2 e = ce[t];
           const double dt = e.x;
-          const int jt = (int)__double_as_longlong(e.y);
+          const int jt = (int)((unsigned)__double_as_longlong(e.y) + g1);
           rank += (dt < dp) || (dt == dp && jt < jp);
           frank += (jt > g) & (jt < jp);
         }
@@ -682,7 +688,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
       bool ok = false, rel = false;
       int jp = -1;
       if (p < cnt) {
-        jp = (int)__double_as_longlong(ce[p].y);
+        jp = (int)((unsigned)__double_as_longlong(ce[p].y) + g1);
         if (jp > g) {
           const double4 aj = b.atom[jp];
           ok = pair_geom_ok(xi, ai.w, ld3(aj), aj.w, s.rp);

@@ -184,6 +184,19 @@ __device__ __attribute__((noinline)) inline double atan2_ool(double y, double x)
   return atan2(y, x);
 }
 
+// The last index k < n with a[k] <= q, for a non-decreasing a[0 .. n] with
+// a[0] <= q < a[n]: binary lifting whose step count depends on n alone (a
+// wave-uniform loop where n is uniform) and whose probes are clamped to n
+// instead of branching on the bound (a[n] > q never moves the result).
+__device__ __forceinline__ int last_le(const int* a, int n, int q) {
+  int lo = 0;
+  for (int step = n > 0 ? 1 << (31 - __builtin_clz(n)) : 0; step > 0; step >>= 1) {
+    const int c = min(lo + step, n);
+    lo = a[c] <= q ? c : lo;
+  }
+  return lo;
+}
+
 // Circle point i: cen + x*c + y*s with c = rad*cos(a), s = rad*sin(a)
 __device__ __forceinline__ V3 subdiv_point(V3 cen, V3 x, V3 y, double rad, double delta, int i) {
   double a = delta * ((double)i + 0.5);

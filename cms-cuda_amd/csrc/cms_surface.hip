@@ -1308,12 +1308,7 @@ __global__ __launch_bounds__(64, EMIT ? 4 : 5) void k_convex(BatchView b, Settin
           const int kq = k0 + lane();
           if (kq < nk) {
             const int q = klist[kq];
-            int lo = 0, hi = L;  // find t with lat_start[t] <= q < lat_start[t+1]
-            while (hi - lo > 1) {
-              const int mid = (lo + hi) >> 1;
-              if (lat_start[mid] <= q) lo = mid; else hi = mid;
-            }
-            const int t = lo;
+            const int t = last_le(lat_start, L, q);   // lat_start[t] <= q < lat_start[t + 1]
             const V3 pt = subdiv_point(xi + north * lat_dt[t], cx, cy, lat_rad[t], lat_delta[t],
                                        q - lat_start[t]);
             const V3 pc = xi + (pt - xi) * scale;
@@ -1338,12 +1333,7 @@ __global__ __launch_bounds__(64, EMIT ? 4 : 5) void k_convex(BatchView b, Settin
         V3 pt{0, 0, 0}, pc{0, 0, 0};
         int t = 0;
         if (q < C) {
-          int lo = 0, hi = L;  // find t with lat_start[t] <= q < lat_start[t+1]
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (lat_start[mid] <= q) lo = mid; else hi = mid;
-          }
-          t = lo;
+          t = last_le(lat_start, L, q);   // lat_start[t] <= q < lat_start[t + 1]
           const int kk = q - lat_start[t];
           pt = subdiv_point(xi + north * lat_dt[t], cx, cy, lat_rad[t], lat_delta[t], kk);
           pc = xi + (pt - xi) * scale;
@@ -1774,12 +1764,7 @@ __global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Setti
         const int qq = q0 + lane();
         bool cand = false;
         if (qq < c1) {
-          int lo = 0, hi = L;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (lat_start[mid] <= qq) lo = mid; else hi = mid;
-          }
-          const int t = lo;
+          const int t = last_le(lat_start, L, qq);   // lat_start[t] <= qq < lat_start[t + 1]
           const double ang = lat_delta[t] * ((double)(qq - lat_start[t]) + 0.5);
           float sa, ca;
           sincosf((float)ang, &sa, &ca);
@@ -1799,12 +1784,7 @@ __global__ __launch_bounds__(64, EMIT ? 2 : 3) void k_concave(BatchView b, Setti
         int t = 0, qq = 0;
         if (si < ns) {
           qq = c0 + clist[si];
-          int lo = 0, hi = L;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (lat_start[mid] <= qq) lo = mid; else hi = mid;
-          }
-          t = lo;
+          t = last_le(lat_start, L, qq);   // lat_start[t] <= qq < lat_start[t + 1]
           const int kk = qq - lat_start[t];
           const V3 rel = subdiv_point(v3(lat_cx[t], lat_cy[t], lat_cz[t]), cx, cy, lat_rad[t],
                                       lat_delta[t], kk);
