@@ -642,7 +642,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cells(BatchView b, Settings s,
       const double dp = ce[p].x;
       const unsigned jup = (unsigned)__double_as_longlong(ce[p].y);
       int rank = 0, frank = 0;
-#pragma unroll 4
+#pragma unroll 8
       for (int t = 0; t < cnt; ++t) {
         const double2 e = ce[t];
         rank += e.x < dp;
